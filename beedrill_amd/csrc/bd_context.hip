@@ -1,0 +1,247 @@
+// bd_context.hip -- context lifetime, errors, buffers and small queries.
+#include <stdio.h>
+#include <string.h>
+
+#include <algorithm>
+
+#include "bd_internal.h"
+
+namespace bd {
+
+int set_err(bd_ctx *c, int code, const std::string &msg) {
+    if (c) c->err = msg;
+    return code;
+}
+
+int hip_err(bd_ctx *c, hipError_t e, const char *what) {
+    std::string m = std::string(what) + ": " + hipGetErrorString(e);
+    return set_err(c, e == hipErrorOutOfMemory ? BD_ENOMEM : BD_EHIP, m);
+}
+
+int dev_alloc(bd_ctx *c, void **p, size_t bytes, const char *what) {
+    *p = nullptr;
+    hipError_t e = hipMalloc(p, std::max<size_t>(bytes, 256));
+    if (e != hipSuccess) {
+        *p = nullptr;
+        (void)hipGetLastError();
+        char buf[160];
+        snprintf(buf, sizeof(buf), "hipMalloc(%s, %zu bytes) failed: %s", what, bytes,
+                 hipGetErrorString(e));
+        return set_err(c, e == hipErrorOutOfMemory ? BD_ENOMEM : BD_EHIP, buf);
+    }
+    return BD_OK;
+}
+
+void dev_free(void *p) {
+    if (p) (void)hipFree(p);
+}
+
+static void free_search(bd_ctx *c) {
+    dev_free(c->d_depth);
+    dev_free(c->d_parent);
+    dev_free(c->d_vis);
+    dev_free(c->d_vis_save);
+    for (auto &b : c->d_bm) dev_free(b);
+    for (auto &q : c->d_q) dev_free(q);
+    dev_free(c->d_tasks);
+    for (auto &b : c->d_ms) dev_free(b);
+    dev_free(c->d_ms_depth);
+    c->d_depth = c->d_parent = nullptr;
+    c->d_vis = c->d_vis_save = nullptr;
+    for (auto &b : c->d_bm) b = nullptr;
+    for (auto &q : c->d_q) q = nullptr;
+    c->d_tasks = nullptr;
+    for (auto &b : c->d_ms) b = nullptr;
+    c->d_ms_depth = nullptr;
+    c->ms_depth_rows = 0;
+    c->task_cap = 0;
+    c->have_result = false;
+}
+
+void free_graph(bd_ctx *c) {
+    free_search(c);
+    dev_free(c->d_off);
+    dev_free(c->d_cols);
+    dev_free(c->d_dead);
+    dev_free(c->d_el);
+    c->d_off = nullptr;
+    c->d_cols = nullptr;
+    c->d_dead = nullptr;
+    c->d_el = nullptr;
+    c->el_m = 0;
+    c->nv = c->nnz = c->m = c->nwords = 0;
+}
+
+int64_t grid_for(const bd_ctx *c, int64_t work_items, int block) {
+    int64_t per_cu = c->tune.grid_per_cu > 0 ? c->tune.grid_per_cu : 8;
+    int64_t cap = (int64_t)c->cu_count * per_cu;
+    int64_t need = (work_items + block - 1) / block;
+    return std::max<int64_t>(1, std::min(cap, need));
+}
+
+int alloc_search(bd_ctx *c) {
+    free_search(c);
+    int rc;
+    const int64_t nv = c->nv, nw = c->nwords;
+    if ((rc = dev_alloc(c, (void **)&c->d_depth, sizeof(int32_t) * nv, "depth"))) return rc;
+    if ((rc = dev_alloc(c, (void **)&c->d_parent, sizeof(int32_t) * nv, "parent"))) return rc;
+    if ((rc = dev_alloc(c, (void **)&c->d_vis, sizeof(uint64_t) * nw, "visited"))) return rc;
+    if ((rc = dev_alloc(c, (void **)&c->d_vis_save, sizeof(uint64_t) * nw, "visited copy")))
+        return rc;
+    for (auto &b : c->d_bm)
+        if ((rc = dev_alloc(c, (void **)&b, sizeof(uint64_t) * nw, "frontier bitmap"))) return rc;
+    for (auto &q : c->d_q)
+        if ((rc = dev_alloc(c, (void **)&q, sizeof(int32_t) * nv, "frontier queue"))) return rc;
+    // Each vertex of degree >= task_threshold yields <= deg/task_edges + 1
+    // tasks and appears in at most one frontier, so this bound is exact.
+    const int64_t te = std::max(1, c->tune.task_edges);
+    const int64_t tt = std::max(1, c->tune.task_threshold);
+    c->task_cap = c->nnz / te + c->nnz / tt + 16;
+    if ((rc = dev_alloc(c, (void **)&c->d_tasks, sizeof(td_task) * c->task_cap, "tasks")))
+        return rc;
+    return BD_OK;
+}
+
+}  // namespace bd
+
+using namespace bd;
+
+extern "C" int bd_device_count(void) {
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) {
+        (void)hipGetLastError();
+        return 0;
+    }
+    return n;
+}
+
+extern "C" int bd_create(bd_ctx **out, int device) {
+    if (!out) return BD_EINVAL;
+    *out = nullptr;
+    int n = bd_device_count();
+    if (n <= 0) return BD_ENOEXT;
+    if (device < 0 || device >= n) return BD_EINVAL;
+    bd_ctx *c = new bd_ctx();
+    c->device = device;
+    c->tune.bu_block = 256;
+    c->tune.td_block = 256;
+    c->tune.grid_per_cu = 8;
+    c->tune.wave_threshold = 32;
+    c->tune.task_threshold = 4096;
+    c->tune.task_edges = 1024;
+    c->tune.level_driver = 1;
+    c->tune.time_steps = 2;
+    hipError_t e = hipSetDevice(device);
+    if (e == hipSuccess) {
+        hipDeviceProp_t prop;
+        e = hipGetDeviceProperties(&prop, device);
+        if (e == hipSuccess) c->cu_count = prop.multiProcessorCount;
+    }
+    int khz = 0;
+    if (e == hipSuccess) e = hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, device);
+    if (e == hipSuccess && khz > 0) c->wall_khz = khz;
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+    for (int i = 0; i < 4 && e == hipSuccess; ++i) e = hipEventCreate(&c->ev[i]);
+    if (e == hipSuccess) e = hipMalloc((void **)&c->d_ctl, sizeof(ctl_block));
+    if (e == hipSuccess) e = hipHostMalloc((void **)&c->h_ctl, sizeof(ctl_block), hipHostMallocDefault);
+    if (e == hipSuccess) {
+        c->log_cap = 1 << 16;
+        e = hipMalloc((void **)&c->d_log, sizeof(bd_level_rec) * c->log_cap);
+    }
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        bd_destroy(c);
+        return BD_EHIP;
+    }
+    *out = c;
+    return BD_OK;
+}
+
+extern "C" int bd_destroy(bd_ctx *c) {
+    if (!c) return BD_OK;
+    (void)hipSetDevice(c->device);
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    free_graph(c);
+    dev_free(c->d_ctl);
+    dev_free(c->d_log);
+    dev_free(c->d_script);
+    dev_free(c->d_mlp);
+    if (c->h_ctl) (void)hipHostFree(c->h_ctl);
+    for (auto &e : c->ev)
+        if (e) (void)hipEventDestroy(e);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+    return BD_OK;
+}
+
+extern "C" const char *bd_last_error(const bd_ctx *c) { return c ? c->err.c_str() : "null context"; }
+
+extern "C" int bd_graph_info(bd_ctx *c, int64_t *nv, int64_t *nnz, int64_t *m) {
+    if (!c) return BD_EINVAL;
+    if (nv) *nv = c->nv;
+    if (nnz) *nnz = c->nnz;
+    if (m) *m = c->m;
+    return BD_OK;
+}
+
+extern "C" int bd_graph_download(bd_ctx *c, int64_t *off, int32_t *cols) {
+    if (!c) return BD_EINVAL;
+    if (!c->d_off) return set_err(c, BD_ESTATE, "no graph");
+    BD_HIP(c, hipSetDevice(c->device));
+    if (off)
+        BD_HIP(c, hipMemcpy(off, c->d_off, sizeof(int64_t) * (c->nv + 1), hipMemcpyDeviceToHost));
+    if (cols)
+        BD_HIP(c, hipMemcpy(cols, c->d_cols, sizeof(int32_t) * c->nnz, hipMemcpyDeviceToHost));
+    return BD_OK;
+}
+
+// hybrid_bfs_main.cc:136-144: rng() % V from lcg(0) (lcg.h), rejecting
+// vertices of degree 0.  Degrees are read from the device CSR on demand.
+extern "C" int bd_pick_roots(bd_ctx *c, int64_t n, int64_t *roots) {
+    if (!c || n < 0 || (n > 0 && !roots)) return BD_EINVAL;
+    if (!c->d_off) return set_err(c, BD_ESTATE, "no graph");
+    BD_HIP(c, hipSetDevice(c->device));
+    std::vector<int64_t> off(c->nv + 1);
+    BD_HIP(c, hipMemcpy(off.data(), c->d_off, sizeof(int64_t) * (c->nv + 1), hipMemcpyDeviceToHost));
+    if (off[c->nv] == 0 && n > 0) return set_err(c, BD_ESTATE, "graph has no edges");
+    uint64_t st = 1;  // lcg(0): the skip-ahead loop leaves ran = 1
+    const uint64_t mul = 6364136223846793005ULL;
+    for (int64_t i = 0; i < n; ++i) {
+        int64_t s;
+        do {
+            st = mul * st + 1ULL;
+            s = (int64_t)(st % (uint64_t)c->nv);
+        } while (off[s + 1] - off[s] == 0);
+        roots[i] = s;
+    }
+    return BD_OK;
+}
+
+extern "C" int bd_set_tuning(bd_ctx *c, const bd_tuning *t) {
+    if (!c || !t) return BD_EINVAL;
+    bd_tuning n = c->tune;
+    if (t->bu_block > 0) {
+        if (t->bu_block % 64 || t->bu_block > 1024) return set_err(c, BD_EINVAL, "bu_block");
+        n.bu_block = t->bu_block;
+    }
+    if (t->td_block > 0) {
+        if (t->td_block % 64 || t->td_block > 1024) return set_err(c, BD_EINVAL, "td_block");
+        n.td_block = t->td_block;
+    }
+    if (t->grid_per_cu > 0) n.grid_per_cu = std::min(t->grid_per_cu, 64);
+    if (t->wave_threshold > 0) n.wave_threshold = t->wave_threshold;
+    if (t->task_threshold > 0) n.task_threshold = std::max(t->task_threshold, 64);
+    if (t->task_edges > 0) n.task_edges = std::max(t->task_edges, 64);
+    if (t->level_driver == 1 || t->level_driver == 2) n.level_driver = t->level_driver;
+    if (t->time_steps == 1 || t->time_steps == 2) n.time_steps = t->time_steps;
+    bool realloc = (n.task_edges != c->tune.task_edges || n.task_threshold != c->tune.task_threshold);
+    c->tune = n;
+    if (realloc && c->d_off) return alloc_search(c);
+    return BD_OK;
+}
+
+extern "C" int bd_get_tuning(bd_ctx *c, bd_tuning *t) {
+    if (!c || !t) return BD_EINVAL;
+    *t = c->tune;
+    return BD_OK;
+}

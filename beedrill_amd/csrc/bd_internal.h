@@ -1,0 +1,152 @@
+// bd_internal.h -- context, device control block and shared helpers of the
+// MI355X BFS library.  Not part of the public ABI (include/beedrill_amd.h).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <string>
+#include <vector>
+
+#include "../../include/beedrill_amd.h"
+
+namespace bd {
+
+constexpr int kWave = 64;
+
+// One task = a slice of a huge vertex's adjacency list (the GPU counterpart of
+// worklist<Edge>'s 64-edge granules, worklist.h:96-117).
+struct td_task {
+    int64_t beg;
+    int64_t end;
+    int32_t src;
+    int32_t pad;
+};
+
+// Device-resident control block.  The step kernels accumulate into acc_*;
+// the decide kernel closes a step, writes its record and picks the next
+// direction, so the level loop can run without the host.
+struct ctl_block {
+    // run constants
+    int64_t nv;
+    int64_t m2;        // 2 * num_edges()   (hybrid_bfs.cc:162)
+    int64_t alpha;
+    int64_t beta;
+    int32_t policy;
+    int32_t has_mlp;
+    int32_t script_len;
+    int32_t log_cap;
+    // Beamer state (hybrid_bfs.cc:162-190)
+    int64_t edges_to_check;
+    int64_t scout;
+    int64_t old_awake;
+    int32_t in_bu_loop;
+    // current step
+    int32_t level;
+    int32_t dir;
+    int32_t done;
+    int32_t td_bitmap;   // TD reads F from the bitmap (previous step was BU)
+    int32_t prev_dir;    // -1 before the first step
+    int64_t frontier;
+    int64_t frontier_deg;
+    int64_t max_frontier_deg;
+    int64_t unvisited;
+    int64_t unvisited_deg;
+    // accumulators of the running step
+    unsigned long long acc_new;
+    unsigned long long acc_new_deg;
+    unsigned long long acc_examined;
+    unsigned long long acc_max_deg;
+    unsigned long long qlen[2];
+    unsigned long long ntasks;
+    unsigned long long task_overflow;
+    // totals
+    int64_t steps;
+    int64_t td_steps;
+    int64_t bu_steps;
+    int64_t reached;
+    int64_t reached_deg;
+    // device timestamps (wall_clock64 ticks)
+    uint64_t t_start;
+    uint64_t t_end;
+};
+
+struct ctx_state;
+
+}  // namespace bd
+
+struct bd_ctx {
+    int device = 0;
+    int cu_count = 256;
+    int64_t wall_khz = 100000;
+    hipStream_t stream = nullptr;
+    std::string err;
+
+    // graph (flat device CSR)
+    int64_t nv = 0, nnz = 0, m = 0, nwords = 0;
+    int64_t *d_off = nullptr;
+    int32_t *d_cols = nullptr;
+    uint64_t *d_dead = nullptr;   // 1 = vertex of degree 0, or padding bit
+    uint64_t *d_el = nullptr;     // el64 order, packed (src << 32) | dst
+    int64_t el_m = 0;
+
+    // search state
+    int32_t *d_depth = nullptr;
+    int32_t *d_parent = nullptr;
+    uint64_t *d_vis = nullptr;
+    uint64_t *d_vis_save = nullptr;
+    uint64_t *d_bm[3] = {nullptr, nullptr, nullptr};
+    int32_t *d_q[2] = {nullptr, nullptr};
+    bd::td_task *d_tasks = nullptr;
+    int64_t task_cap = 0;
+    bd::ctl_block *d_ctl = nullptr;
+    bd::ctl_block *h_ctl = nullptr;  // pinned mirror
+    bd_level_rec *d_log = nullptr;
+    int64_t log_cap = 0;
+    int32_t *d_script = nullptr;
+    int64_t script_len = 0;
+    std::vector<int32_t> script;
+
+    // results of the last run
+    bool have_result = false;
+    int64_t last_source = -1;
+    int64_t last_steps = 0;
+    std::vector<bd_level_rec> last_log;
+
+    // learned policy
+    float *d_mlp = nullptr;
+    int mlp_h1 = 0, mlp_h2 = 0;
+    bool has_mlp = false;
+    std::vector<float> h_mlp;
+
+    // multi-root state
+    uint64_t *d_ms[4] = {nullptr, nullptr, nullptr, nullptr};
+    int32_t *d_ms_depth = nullptr;
+    int64_t ms_depth_rows = 0;
+
+    bd_tuning tune{};
+    hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+};
+
+namespace bd {
+
+int set_err(bd_ctx *c, int code, const std::string &msg);
+int hip_err(bd_ctx *c, hipError_t e, const char *what);
+// Allocate n bytes of device memory (zero-size requests allocate 256 B).
+int dev_alloc(bd_ctx *c, void **p, size_t bytes, const char *what);
+void dev_free(void *p);
+// Release all graph-dependent buffers.
+void free_graph(bd_ctx *c);
+// (Re)allocate search buffers for the resident graph.
+int alloc_search(bd_ctx *c);
+// Build device CSR from a device edge list (packed src<<32|dst, el order).
+int build_csr_from_el(bd_ctx *c, int64_t nv, int64_t m);
+int64_t grid_for(const bd_ctx *c, int64_t work_items, int block);
+
+}  // namespace bd
+
+#define BD_HIP(c, call)                                            \
+    do {                                                           \
+        hipError_t e__ = (call);                                   \
+        if (e__ != hipSuccess) return bd::hip_err((c), e__, #call); \
+    } while (0)

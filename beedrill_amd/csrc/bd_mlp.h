@@ -1,0 +1,121 @@
+// bd_mlp.h -- direction-cost network shared by the decide kernel and the
+// training/inference kernels.  8 inputs -> h1 -> h2 -> 2 outputs (predicted
+// log2 microseconds of a top-down and of a bottom-up step), ReLU, h1,h2 <= 32.
+// Parameter layout (row-major, out x in):
+//   W1[h1*8] b1[h1] W2[h2*h1] b2[h2] W3[2*h2] b3[2]
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <math.h>
+#include <stdint.h>
+
+namespace bd {
+
+constexpr int kMlpIn = 8;
+constexpr int kMlpOut = 2;
+constexpr int kMlpMaxHidden = 32;
+
+__host__ __device__ inline int64_t mlp_nparams(int h1, int h2) {
+    return (int64_t)h1 * kMlpIn + h1 + (int64_t)h2 * h1 + h2 + (int64_t)kMlpOut * h2 + kMlpOut;
+}
+
+__host__ __device__ inline float lg1p(double x) { return (float)log2(1.0 + (x > 0 ? x : 0)); }
+
+// Step features, all computable before the step from direction-independent
+// quantities: |F|, sum deg(F), max deg(F), unvisited count and degree sum,
+// level, average degree, and whether the previous step was bottom-up (a
+// top-down step then reads F from the bitmap).
+__host__ __device__ inline void mlp_features_dev(int64_t frontier, int64_t frontier_deg,
+                                                 int64_t max_frontier_deg, int64_t unvisited,
+                                                 int64_t unvisited_deg, int64_t level, int64_t nv,
+                                                 int64_t m, int64_t prev_dir, float *x) {
+    const double lv = log2(1.0 + (double)nv);
+    const double le = log2(1.0 + 2.0 * (double)m);
+    x[0] = (float)(lg1p((double)frontier) / lv);
+    x[1] = (float)(lg1p((double)frontier_deg) / le);
+    x[2] = (float)(lg1p((double)unvisited) / lv);
+    x[3] = (float)(lg1p((double)unvisited_deg) / le);
+    x[4] = (float)((level < 64 ? level : 64) / 64.0);
+    x[5] = (float)(lg1p((double)max_frontier_deg) / lv);
+    x[6] = (float)(lg1p(2.0 * (double)m / (double)(nv > 0 ? nv : 1)) / 8.0);
+    x[7] = prev_dir == 1 ? 1.0f : 0.0f;
+}
+
+typedef short bf16x4 __attribute__((ext_vector_type(4)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ short f32_to_bf16(float f) {
+    uint32_t u = __float_as_uint(f);
+    if ((u & 0x7F800000u) == 0x7F800000u) return (short)(u >> 16);  // inf / nan
+    u += 0x7FFFu + ((u >> 16) & 1u);
+    return (short)(u >> 16);
+}
+
+// One dense layer on one wave with v_mfma_f32_16x16x16bf16_1k:
+// Y[16 x nout] = act(X[16 x k] * W^T + b), X and Y in LDS (row stride 32).
+// Operand layout of the 16x16x16 instruction (lane l, element j = 0..3):
+//   A: row l%16, k 4*(l/16)+j     B: col l%16, k 4*(l/16)+j
+//   D: col l%16, row 4*(l/16)+j
+__device__ __forceinline__ void mlp_layer_mfma(const float *W, const float *b, int k, int nout,
+                                               const float (*X)[32], float (*Y)[32], bool relu) {
+    const int l = threadIdx.x & 63;
+    const int r = l & 15, q = l >> 4;
+    for (int nt = 0; nt * 16 < nout; ++nt) {
+        f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+        for (int kt = 0; kt * 16 < k; ++kt) {
+            bf16x4 a, bw;
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const int kk = kt * 16 + 4 * q + j;
+                const int n = nt * 16 + r;
+                a[j] = f32_to_bf16(kk < k ? X[r][kk] : 0.f);
+                bw[j] = f32_to_bf16((kk < k && n < nout) ? W[n * k + kk] : 0.f);
+            }
+            acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, bw, acc, 0, 0, 0);
+        }
+        const int n = nt * 16 + r;
+        if (n < nout) {
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                float v = acc[j] + b[n];
+                Y[4 * q + j][n] = relu ? (v > 0.f ? v : 0.f) : v;
+            }
+        }
+    }
+}
+
+// Forward pass of up to 16 samples (rows of X) on one wave with bf16 MFMA
+// and fp32 accumulation.  X: [16][32] LDS, rows beyond n must be zero.
+// Writes Y[16][32] (columns 0..1 are the outputs).
+__device__ __forceinline__ void mlp_forward_rows_mfma(const float *p, int h1, int h2,
+                                                      float (*X)[32], float (*H)[32],
+                                                      float (*Y)[32]) {
+    const float *w1 = p, *b1 = w1 + h1 * kMlpIn, *w2 = b1 + h1, *b2 = w2 + h2 * h1;
+    const float *w3 = b2 + h2, *b3 = w3 + kMlpOut * h2;
+    mlp_layer_mfma(w1, b1, kMlpIn, h1, X, H, true);
+    __syncthreads();
+    mlp_layer_mfma(w2, b2, h1, h2, H, X, true);
+    __syncthreads();
+    mlp_layer_mfma(w3, b3, h2, kMlpOut, X, Y, false);
+    __syncthreads();
+}
+
+// Single-sample forward on one wave (used by the decide kernel).  x is in
+// LDS; every lane returns y.
+__device__ inline void mlp_forward_wave(const float *p, int h1, int h2, const float *x,
+                                        float *y) {
+    __shared__ float X[16][32];
+    __shared__ float H[16][32];
+    __shared__ float Y[16][32];
+    const int l = threadIdx.x & 63;
+    for (int i = l; i < 16 * 32; i += 64) {
+        const int r = i / 32, cidx = i % 32;
+        X[r][cidx] = (r == 0 && cidx < kMlpIn) ? x[cidx] : 0.f;
+    }
+    __syncthreads();
+    mlp_forward_rows_mfma(p, h1, h2, X, H, Y);
+    y[0] = Y[0][0];
+    y[1] = Y[0][1];
+}
+
+}  // namespace bd

@@ -1,0 +1,330 @@
+// hybrid_bfs_main.cc -- command-line driver with the reference's flags and
+// output lines (hybrid_bfs_main.cc:10-309).  Additive flags: --generate (a
+// seeded device-generated RMAT graph instead of a file; the reference
+// generator's argument syntax, generator/rmat_args.h:26-55), --seed,
+// --policy, --mlp_weights and --device.
+#include <getopt.h>
+#include <math.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include <chrono>
+#include <regex>
+#include <string>
+#include <vector>
+
+#include "graph.h"
+#include "hybrid_bfs.h"
+
+#define LOG(...)                      \
+    do {                              \
+        fprintf(stdout, __VA_ARGS__); \
+        fflush(stdout);               \
+    } while (0)
+
+#ifndef BD_VERSION
+#define BD_VERSION "beedrill_amd-gfx950"
+#endif
+
+static const struct option long_options[] = {{"graph_filename", required_argument, 0, 0},
+                                             {"distributed_load", no_argument, 0, 0},
+                                             {"heavy_threshold", required_argument, 0, 0},
+                                             {"num_trials", required_argument, 0, 0},
+                                             {"source_vertex", required_argument, 0, 0},
+                                             {"algorithm", required_argument, 0, 0},
+                                             {"alpha", required_argument, 0, 0},
+                                             {"beta", required_argument, 0, 0},
+                                             {"sort_edge_blocks", no_argument, 0, 0},
+                                             {"dump_edge_list", no_argument, 0, 0},
+                                             {"check_graph", no_argument, 0, 0},
+                                             {"dump_graph", no_argument, 0, 0},
+                                             {"check_results", no_argument, 0, 0},
+                                             {"version", no_argument, 0, 0},
+                                             {"help", no_argument, 0, 0},
+                                             {"generate", required_argument, 0, 0},
+                                             {"seed", required_argument, 0, 0},
+                                             {"policy", required_argument, 0, 0},
+                                             {"mlp_weights", required_argument, 0, 0},
+                                             {"device", required_argument, 0, 0},
+                                             {nullptr, 0, 0, 0}};
+
+static void print_help(const char *argv0) {
+    LOG("Usage: %s [OPTIONS]\n", argv0);
+    LOG("\t--graph_filename     Path to graph file to load\n");
+    LOG("\t--distributed_load   Accepted for compatibility; the file is read once\n");
+    LOG("\t--heavy_threshold    Accepted for compatibility (unused, as in the reference)\n");
+    LOG("\t--num_trials         Run BFS this many times.\n");
+    LOG("\t--source_vertex      Use this as the source vertex. If unspecified, pick random vertices.\n");
+    LOG("\t--algorithm          Select BFS implementation to run\n");
+    LOG("\t--alpha              Alpha parameter for direction-optimizing BFS\n");
+    LOG("\t--beta               Beta parameter for direction-optimizing BFS\n");
+    LOG("\t--sort_edge_blocks   Sort each adjacency list by vertex id\n");
+    LOG("\t--dump_edge_list     Print the edge list to stdout after loading (slow)\n");
+    LOG("\t--check_graph        Validate the constructed graph against the edge list (slow)\n");
+    LOG("\t--dump_graph         Print the graph to stdout after construction (slow)\n");
+    LOG("\t--check_results      Validate the BFS results (slow)\n");
+    LOG("\t--version            Print version info\n");
+    LOG("\t--help               Print command line help\n");
+    LOG("\t--generate SPEC      Generate graph500-scaleN or A-B-C-D-E-V.rmat on the GPU\n");
+    LOG("\t--seed N             Generator seed (default 0)\n");
+    LOG("\t--policy NAME        beamer (default) | learned | td_only | bu_only\n");
+    LOG("\t--mlp_weights FILE   float32 parameters of the 8-32-32-2 direction network\n");
+    LOG("\t--device N           GPU ordinal (default 0)\n");
+}
+
+struct bfs_args {
+    const char *graph_filename = nullptr;
+    long num_trials = 1;
+    long source_vertex = -1;
+    const char *algorithm = "beamer_hybrid";
+    long alpha = 15, beta = 18;
+    bool sort_edge_blocks = false, dump_edge_list = false, check_graph = false;
+    bool dump_graph = false, check_results = false;
+    const char *generate = nullptr;
+    unsigned seed = 0;
+    int policy = BD_POLICY_BEAMER;
+    const char *mlp_weights = nullptr;
+    int device = 0;
+
+    static bfs_args parse(int argc, char **argv) {
+        bfs_args a;
+        int idx = 0;
+        while (true) {
+            int c = getopt_long(argc, argv, "", long_options, &idx);
+            if (c == -1) break;
+            if (c == '?') {
+                LOG("Invalid arguments\n");
+                print_help(argv[0]);
+                exit(1);
+            }
+            const std::string n = long_options[idx].name;
+            if (n == "graph_filename") a.graph_filename = optarg;
+            else if (n == "num_trials") a.num_trials = atol(optarg);
+            else if (n == "source_vertex") a.source_vertex = atol(optarg);
+            else if (n == "algorithm") a.algorithm = optarg;
+            else if (n == "alpha") a.alpha = atol(optarg);
+            else if (n == "beta") a.beta = atol(optarg);
+            else if (n == "sort_edge_blocks") a.sort_edge_blocks = true;
+            else if (n == "dump_edge_list") a.dump_edge_list = true;
+            else if (n == "check_graph") a.check_graph = true;
+            else if (n == "dump_graph") a.dump_graph = true;
+            else if (n == "check_results") a.check_results = true;
+            else if (n == "version") {
+                LOG("%s\n", BD_VERSION);
+                exit(0);
+            } else if (n == "help") {
+                print_help(argv[0]);
+                exit(1);
+            } else if (n == "generate") a.generate = optarg;
+            else if (n == "seed") a.seed = (unsigned)strtoul(optarg, nullptr, 10);
+            else if (n == "policy") {
+                const std::string p = optarg;
+                if (p == "beamer") a.policy = BD_POLICY_BEAMER;
+                else if (p == "learned") a.policy = BD_POLICY_LEARNED;
+                else if (p == "td_only") a.policy = BD_POLICY_TD_ONLY;
+                else if (p == "bu_only") a.policy = BD_POLICY_BU_ONLY;
+                else {
+                    LOG("Unknown policy '%s'\n", optarg);
+                    exit(1);
+                }
+            } else if (n == "mlp_weights") a.mlp_weights = optarg;
+            else if (n == "device") a.device = atoi(optarg);
+        }
+        if (a.graph_filename == nullptr && a.generate == nullptr) {
+            LOG("Missing graph filename\n");
+            exit(1);
+        }
+        if (a.num_trials <= 0) { LOG("num_trials must be > 0\n"); exit(1); }
+        if (a.alpha <= 0) { LOG("alpha must be > 0\n"); exit(1); }
+        if (a.beta <= 0) { LOG("beta must be > 0\n"); exit(1); }
+        return a;
+    }
+};
+
+// generator/rmat_args.h:26-55
+static bool parse_rmat(const std::string &s, double *abcd, long *ne, long *nv) {
+    std::smatch m;
+    auto suffixed = [](const std::string &t) {
+        long long n = std::stoll(t);
+        switch (t.back()) {
+        case 'K': n <<= 10; break;
+        case 'M': n <<= 20; break;
+        case 'G': n <<= 30; break;
+        case 'T': n <<= 40; break;
+        default: break;
+        }
+        return (long)n;
+    };
+    std::regex r1(R"((\d+[.]\d+)-(\d+[.]\d+)-(\d+[.]\d+)-(\d+[.]\d+)-(\d+[KMGT]?)-(\d+[KMGT]?)\.rmat)");
+    std::regex r2(R"(graph500-scale(\d+)(\.mtx)?)");
+    if (std::regex_match(s, m, r1)) {
+        for (int i = 0; i < 4; ++i) abcd[i] = std::stod(m[i + 1]);
+        *ne = suffixed(m[5]);
+        *nv = suffixed(m[6]);
+        return true;
+    }
+    if (std::regex_match(s, m, r2)) {
+        abcd[0] = 0.57; abcd[1] = 0.19; abcd[2] = 0.19; abcd[3] = 0.05;
+        *nv = 1L << std::stol(m[1]);
+        *ne = 16 * *nv;
+        return true;
+    }
+    return false;
+}
+
+int main(int argc, char **argv) {
+    bool success = true;
+    bfs_args args = bfs_args::parse(argc, argv);
+    const uint64_t lcg_mul = 6364136223846793005ULL;
+    uint64_t lcg_state = 1;  // lcg rng(0) (hybrid_bfs_main.cc:159)
+
+    std::unique_ptr<graph> g;
+    try {
+        auto t0 = std::chrono::steady_clock::now();
+        dist_edge_list::handle el;
+        bd_handle ctx;
+        if (args.generate) {
+            double abcd[4];
+            long ne = 0, nv = 0;
+            if (!parse_rmat(args.generate, abcd, &ne, &nv)) {
+                LOG("Cannot parse --generate '%s'\n", args.generate);
+                exit(1);
+            }
+            ctx = bd_open(args.device);
+            int64_t m = 0;
+            bd_check(ctx.get(), bd_graph_generate(ctx.get(), BD_GEN_RMAT, nv, ne, abcd[0], abcd[1],
+                                                  abcd[2], abcd[3], args.seed, &m),
+                     "generate");
+            g.reset(new graph(ctx));
+        } else {
+            el = dist_edge_list::load_binary(args.graph_filename, args.device);
+            g = create_graph_from_edge_list<graph>(*el);
+        }
+        double ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0)
+                        .count();
+        LOG("Loaded %li edges in %3.2f ms, %3.2f MB/s\n", g->num_edges(), ms,
+            (1e-6 * g->num_edges() * 16) / (1e-3 * ms));
+        if (args.dump_edge_list && el) {
+            LOG("Dumping edge list...\n");
+            el->dump();
+        }
+        LOG("Constructing graph...\n");
+        if (args.sort_edge_blocks) LOG("Sorting edge lists is not needed on one GPU; skipped.\n");
+        g->print_distribution();
+        if (args.check_graph) {
+            LOG("Checking graph...");
+            if (!el) {
+                LOG("(generated graph: no edge list file) ");
+            }
+            if (!el || g->check(*el)) {
+                LOG("PASS\n");
+            } else {
+                LOG("FAIL\n");
+                success = false;
+            }
+        }
+        if (args.dump_graph) {
+            LOG("Dumping graph...\n");
+            g->dump();
+        }
+        if (args.source_vertex >= g->num_vertices()) {
+            LOG("Source vertex %li out of range.\n", args.source_vertex);
+            exit(1);
+        }
+        if (!strcmp(args.algorithm, "none")) return !success;
+
+        enum { REMOTE_WRITES, MIGRATING_THREADS, REMOTE_WRITES_HYBRID, BEAMER_HYBRID } alg;
+        if (!strcmp(args.algorithm, "remote_writes")) alg = REMOTE_WRITES;
+        else if (!strcmp(args.algorithm, "migrating_threads")) alg = MIGRATING_THREADS;
+        else if (!strcmp(args.algorithm, "remote_writes_hybrid")) alg = REMOTE_WRITES_HYBRID;
+        else if (!strcmp(args.algorithm, "beamer_hybrid")) alg = BEAMER_HYBRID;
+        else {
+            LOG("Algorithm '%s' not implemented!\n", args.algorithm);
+            exit(1);
+        }
+        LOG("Initializing BFS data structures...\n");
+        hybrid_bfs bfs(*g);
+        bfs.set_policy(args.policy);
+        if (args.mlp_weights) {
+            FILE *fp = fopen(args.mlp_weights, "rb");
+            if (!fp) {
+                LOG("Unable to open %s\n", args.mlp_weights);
+                exit(1);
+            }
+            std::vector<float> p(32 * 8 + 32 + 32 * 32 + 32 + 2 * 32 + 2);
+            size_t got = fread(p.data(), sizeof(float), p.size(), fp);
+            fclose(fp);
+            if (got != p.size()) {
+                LOG("Short weight file %s\n", args.mlp_weights);
+                exit(1);
+            }
+            bd_check(g->context(), bd_mlp_set(g->context(), p.data(), 32, 32), "mlp_set");
+        }
+
+        long edges_all = 0;
+        double ms_all = 0;
+        std::vector<double> teps;
+        for (long s = 0; s < args.num_trials; ++s) {
+            long source;
+            if (args.source_vertex >= 0) {
+                source = args.source_vertex;
+            } else {
+                // pick_random_vertex (hybrid_bfs_main.cc:136-144)
+                do {
+                    lcg_state = lcg_mul * lcg_state + 1ULL;
+                    source = (long)(lcg_state % (uint64_t)g->num_vertices());
+                } while (g->out_degree(source) == 0);
+            }
+            bfs.clear();
+            LOG("Doing breadth-first search from vertex %li (sample %li of %li)\n", source, s + 1,
+                args.num_trials);
+            switch (alg) {
+            case REMOTE_WRITES: bfs.run_with_remote_writes(source); break;
+            case MIGRATING_THREADS: bfs.run_with_migrating_threads(source); break;
+            case REMOTE_WRITES_HYBRID:
+                bfs.run_with_remote_writes_hybrid(source, args.alpha, args.beta);
+                break;
+            case BEAMER_HYBRID: bfs.run_beamer(source, args.alpha, args.beta); break;
+            }
+            const double time_ms = bfs.last_stats().ms;
+            if (args.check_results) {
+                LOG("Checking results...\n");
+                if (bfs.check(source)) {
+                    LOG("PASS\n");
+                } else {
+                    LOG("FAIL\n");
+                    success = false;
+                }
+            }
+            const long traversed = bfs.count_num_traversed_edges();
+            edges_all += traversed;
+            ms_all += time_ms;
+            teps.push_back(traversed / (time_ms / 1000));
+            LOG("Traversed %li edges in %3.2f ms, %3.2f MTEPS \n", traversed, time_ms,
+                (1e-6 * traversed) / (time_ms / 1000));
+        }
+        LOG("Mean performance over all trials: %3.2f MTEPS \n",
+            (1e-6 * edges_all) / (ms_all / 1000));
+        if (args.num_trials > 1) {
+            // combined.cc:108-146, 315-327
+            double rs = 0, sum = 0, mn = teps[0], mx = teps[0];
+            for (double x : teps) {
+                rs += 1.0 / x;
+                sum += x;
+                mn = x < mn ? x : mn;
+                mx = x > mx ? x : mx;
+            }
+            const double mean = sum / teps.size(), hm = teps.size() / rs;
+            double sq = 0;
+            for (double x : teps) sq += (x - mean) * (x - mean);
+            const double sd = sqrt(sq / (teps.size() - 1));
+            LOG("    BFS: %3.2f +/- %3.2f GTEPS, min/max %3.2f/%3.2f GTEPS\n", 1e-9 * hm, 1e-9 * sd,
+                1e-9 * mn, 1e-9 * mx);
+        }
+    } catch (const std::exception &e) {
+        LOG("%s\n", e.what());
+        return 1;
+    }
+    return !success;
+}

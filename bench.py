@@ -1,0 +1,184 @@
+"""Benchmark: harmonic-mean GTEPS over 64 roots on RMAT scale 24, learned
+direction policy vs Beamer (alpha 15, beta 18), one MI355X.
+
+A step is one breadth-first search from one root (hybrid_bfs::run_beamer on
+the device, clear() untimed), its time the device time from the first step
+kernel to the end of the last (the reference times run_beamer alone,
+hybrid_bfs_main.cc:267-282).  TEPS = count_num_traversed_edges() / time
+(hybrid_bfs.cc:401-414); the aggregate is the harmonic mean over roots
+(combined.cc:108-146).  Roots are the reference's lcg(0) picks
+(hybrid_bfs_main.cc:136-144).  The graph is generated on the GPU with the
+build's seeded RMAT generator and is resident in HBM before timing starts.
+
+Prints one JSON line.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "harmonic-mean GTEPS over 64 roots, RMAT s24; learned vs Beamer policy"
+HBM_PEAK_GBS = 8000.0
+MODEL = os.path.join(ROOT, "beedrill_amd", "models", "direction_mlp_32x32.f32")
+TRAFFIC = os.path.join(ROOT, "profiles", "bottom_up_traffic_rmat_s24.json")
+
+
+def hmean(x):
+    x = np.asarray(x, np.float64)
+    return float(len(x) / np.sum(1.0 / x))
+
+
+def run_policy(ctx, roots, policy, warmup):
+    for r in roots[:warmup]:
+        ctx.bfs(int(r), 15, 18, policy)
+    gteps, ms, steps = [], [], []
+    t0 = time.perf_counter()
+    for r in roots:
+        st = ctx.bfs(int(r), 15, 18, policy)
+        gteps.append(st["traversed_edges"] / (st["ms"] * 1e-3) / 1e9)
+        ms.append(st["ms"])
+        steps.append((st["td_steps"], st["bu_steps"]))
+    wall = time.perf_counter() - t0
+    return {"hmean_gteps": hmean(gteps), "mean_ms": float(np.mean(ms)),
+            "sum_edges_over_sum_time_gteps": float(
+                sum(g * m for g, m in zip(gteps, ms)) / sum(ms)),
+            "min_gteps": float(np.min(gteps)), "max_gteps": float(np.max(gteps)),
+            "wall_s": wall, "per_root_gteps": [round(g, 3) for g in gteps]}
+
+
+def bu_bytes(rec, nv, n_isolated):
+    """Algorithmic bytes of one bottom-up step: visited-bitmap scan (V/8),
+    both offsets of every unvisited vertex with edges (16 U), the neighbour ids
+    read up to the first hit (4 E), one frontier bit per id read (E/8),
+    depth and parent of each newly reached vertex (8 N) and the next-frontier
+    bitmap (V/8)."""
+    U = rec["unvisited"] - n_isolated
+    E = rec["edges_examined"]
+    return nv / 8 + 16 * U + 4 * E + E / 8 + 8 * rec["next"] + nv / 8
+
+
+def roofline(ctx, roots, policy, nv, n_isolated):
+    t = ctx.get_tuning()
+    ctx.set_tuning(time_steps=1)
+    tot_b, tot_ms, n = 0.0, 0.0, 0
+    try:
+        for r in roots:
+            ctx.bfs(int(r), 15, 18, policy)
+            for rec in ctx.levels():
+                if rec["dir"] == 1 and rec["ms"] > 0:
+                    tot_b += bu_bytes(rec, nv, n_isolated)
+                    tot_ms += rec["ms"]
+                    n += 1
+    finally:
+        ctx.set_tuning(time_steps=2, level_driver=t["level_driver"])
+    achieved = tot_b / (tot_ms * 1e-3) / 1e9 if tot_ms > 0 else 0.0
+    traffic = None
+    if os.path.exists(TRAFFIC):
+        with open(TRAFFIC) as f:
+            traffic = json.load(f).get("bytes_per_launch")
+    return {"kernel": "k_bottom_up", "bound": "hbm", "achieved": round(achieved, 1),
+            "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+            "traffic": traffic, "launches": n,
+            "algorithmic_bytes_per_launch": tot_b / max(n, 1),
+            "avg_launch_ms": tot_ms / max(n, 1)}
+
+
+def cpu_baseline(ctx, roots, budget_s):
+    import oracle
+    off, adj = ctx.csr()
+    _, _, m = ctx.info()
+    gteps, used = [], []
+    t_all = time.perf_counter()
+    for r in roots:
+        t0 = time.perf_counter()
+        parent = oracle.beamer_timed(off, adj, m, int(r))
+        dt = time.perf_counter() - t0
+        gteps.append(oracle.count_traversed(off, parent) / dt / 1e9)
+        used.append(int(r))
+        if time.perf_counter() - t_all > budget_s:
+            break
+    return {"value": round(hmean(gteps), 4), "unit": "GTEPS", "cores": 1, "kind": "port",
+            "sample": f"{len(used)} of the same lcg(0) roots on the same graph, serial oracle "
+                      f"run_beamer (harmonic mean), {time.perf_counter() - t_all:.1f} s"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=64, help="timed roots per policy")
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--scale", type=int, default=24)
+    ap.add_argument("--edge-factor", type=int, default=16)
+    ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("--roofline-roots", type=int, default=8)
+    ap.add_argument("--cpu-budget", type=float, default=20.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only; --gpus {args.gpus} is not supported "
+              "(the search is not sharded across GPUs in this build)", file=sys.stderr)
+        sys.exit(2)
+
+    import beedrill_amd as bd
+    ctx = bd.Context(0)
+    t0 = time.perf_counter()
+    m = ctx.generate_rmat(args.scale, args.edge_factor, seed=args.seed)
+    gen_s = time.perf_counter() - t0
+    nv, nnz, _ = ctx.info()
+    roots = ctx.pick_roots(args.steps)
+    off_h = None
+    n_isolated = None
+    # degree-0 vertices are never scanned by a bottom-up step
+    off_h, _ = ctx.csr() if not args.no_cpu else (None, None)
+    if off_h is None:
+        off_h = np.empty(nv + 1, np.int64)
+        ctx._L.bd_graph_download(ctx._h, off_h.ctypes.data, None)
+    n_isolated = int(np.sum(np.diff(off_h) == 0))
+
+    results = {"beamer": run_policy(ctx, roots, "beamer", args.warmup)}
+    have_model = os.path.exists(MODEL)
+    if have_model:
+        ctx.mlp_set(np.fromfile(MODEL, np.float32), 32, 32)
+        results["learned"] = run_policy(ctx, roots, "learned", args.warmup)
+    head = results["learned"] if have_model else results["beamer"]
+    policy = "learned" if have_model else "beamer"
+    roof = roofline(ctx, roots[:args.roofline_roots], policy, nv, n_isolated)
+    cpu = None if args.no_cpu else cpu_baseline(ctx, roots, args.cpu_budget)
+
+    out = {
+        "metric": METRIC,
+        "value": round(head["hmean_gteps"], 3),
+        "unit": "GTEPS",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(head["mean_ms"], 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "int32",
+        "data": "synthetic: seeded RMAT generated on the GPU (reference generator stream, "
+                "build-defined relabel/shuffle)",
+        "config": {"workload": f"RMAT scale {args.scale}, edge factor {args.edge_factor}, "
+                               f"A/B/C/D .57/.19/.19/.05, seed {args.seed}, {args.steps} lcg(0) "
+                               f"roots, alpha 15 beta 18, policy {policy}",
+                   "num_vertices": nv, "num_edges": m, "nnz": nnz,
+                   "graph_build_s": round(gen_s, 2)},
+        "roofline": roof,
+        "cpu_baseline": cpu,
+        "policies": {k: {kk: vv for kk, vv in v.items() if kk != "per_root_gteps"}
+                     for k, v in results.items()},
+        "learned_vs_beamer": (round(results["learned"]["hmean_gteps"] /
+                                    results["beamer"]["hmean_gteps"], 4) if have_model else None),
+    }
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

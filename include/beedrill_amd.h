@@ -1,0 +1,236 @@
+/*
+ * beedrill_amd.h -- C ABI of the MI355X (gfx950) direction-optimizing BFS.
+ *
+ * The reference (ehein6/beedrill) exposes this path as the C++ class
+ * `hybrid_bfs` (hybrid_bfs.h:11-47) over the CSR view `graph`
+ * (graph_base.h:231-305), built by create_graph_from_edge_list
+ * (graph_base.h:335-436) from an el64 file read by dist_edge_list::load_binary
+ * (dist_edge_list.cc:50-143).  Each entry point below names the reference
+ * interface it replaces.  The C++ drop-in classes in beedrill_amd/host/ wrap
+ * these calls with the reference's own method names.
+ *
+ * Conventions
+ *   - Every call returns BD_OK (0) or a negative bd_status; bd_last_error()
+ *     gives the message.  No C++ exception crosses this boundary.
+ *   - One bd_ctx owns one HIP stream and all device buffers.  A context is
+ *     single-caller and not re-entrant, as a hybrid_bfs instance is
+ *     (hybrid_bfs.h:22-23 shared scout/awake counters).
+ *   - Host buffers are caller-owned; a NULL output pointer means "skip".
+ *   - Vertex ids are int64 at the boundary (the reference's `long`), stored
+ *     on the device as int32 neighbour ids with int64 CSR offsets.
+ */
+#ifndef BEEDRILL_AMD_H
+#define BEEDRILL_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct bd_ctx bd_ctx;
+
+typedef enum {
+    BD_OK = 0,
+    BD_EINVAL = -1,  /* bad argument (reference: LOG + exit(1))            */
+    BD_ENOMEM = -2,  /* device allocation failed (reference: std::terminate) */
+    BD_EHIP = -3,    /* HIP runtime error                                   */
+    BD_ESTATE = -4,  /* call out of order (e.g. BFS before a graph)          */
+    BD_EIO = -5,     /* file missing, short or malformed                    */
+    BD_ERANGE = -6,  /* vertex id out of range                              */
+    BD_ENOEXT = -7   /* no usable GPU                                       */
+} bd_status;
+
+/* Direction policies.  BEAMER is run_beamer (hybrid_bfs.cc:152-191);
+ * TD_ONLY is run_with_migrating_threads (hybrid_bfs.cc:197-213); LEARNED
+ * replaces the two predicates at hybrid_bfs.cc:167 and :178-179 with the MLP
+ * cost model and keeps Beamer as the fallback when no model is loaded;
+ * BU_ONLY and SCRIPT exist for testing and for timing both directions. */
+typedef enum {
+    BD_POLICY_BEAMER = 0,
+    BD_POLICY_LEARNED = 1,
+    BD_POLICY_TD_ONLY = 2,
+    BD_POLICY_BU_ONLY = 3,
+    BD_POLICY_SCRIPT = 4
+} bd_policy;
+
+enum { BD_DIR_TD = 0, BD_DIR_BU = 1 };
+
+/* Per-step record (one per top-down or bottom-up step).  The first fields
+ * mirror the reference's step-level quantities: |F| (sliding_queue
+ * combined_size, sliding_queue.h:126-136), scout/awake returned by the step
+ * (hybrid_bfs.cc:99, :142) and edges_to_check (:162, :182). */
+typedef struct {
+    int64_t level;            /* depth of the frontier expanded            */
+    int64_t dir;              /* BD_DIR_TD or BD_DIR_BU                    */
+    int64_t frontier;         /* |F|                                       */
+    int64_t frontier_deg;     /* sum deg(F)                                */
+    int64_t unvisited;        /* vertices unreached before the step        */
+    int64_t unvisited_deg;    /* sum of their degrees                      */
+    int64_t edges_to_check;   /* Beamer counter after this step's update   */
+    int64_t result;           /* scout (TD) or awake (BU)                  */
+    int64_t next;             /* |N| newly reached                         */
+    int64_t next_deg;         /* sum deg(N)                                */
+    int64_t edges_examined;   /* TD: sum deg(F); BU: neighbour ids read    */
+    int64_t max_frontier_deg; /* max deg over F                            */
+    int64_t td_bitmap_input;  /* 1 when a TD step read F from the bitmap   */
+    int64_t reserved;
+    double ms;                /* device time of the step (timed runs only) */
+    double ms_other;          /* profile runs: time of the other direction */
+} bd_level_rec;
+
+typedef struct {
+    int64_t source;
+    int64_t steps;            /* TD + BU steps                             */
+    int64_t td_steps;
+    int64_t bu_steps;
+    int64_t reached;          /* vertices with a parent                    */
+    int64_t traversed_edges;  /* count_num_traversed_edges()               */
+    double ms;                /* device time of the whole search           */
+} bd_run_stats;
+
+/* ---------------------------------------------------------------- context */
+
+/* hybrid_bfs::hybrid_bfs(graph&) allocation half (hybrid_bfs.cc:10-21) and
+ * ack_control_init (ack_control.h) -- on the GPU: pick the device, create
+ * the stream. */
+int bd_create(bd_ctx **out, int device);
+int bd_destroy(bd_ctx *ctx);
+const char *bd_last_error(const bd_ctx *ctx);
+/* Number of visible GPUs (0 when none). */
+int bd_device_count(void);
+
+/* ----------------------------------------------------------------- graphs */
+
+/* Upload an already built symmetric CSR (graph_base.h:30-35 view):
+ * offsets[nv+1] (int64), cols[nnz] (int32), m_undirected = num_edges()
+ * (graph_base.h:235-237).  Validates offsets are monotone and ids in range. */
+int bd_graph_upload(bd_ctx *ctx, const int64_t *offsets, const int32_t *cols,
+                    int64_t nv, int64_t nnz, int64_t m_undirected);
+
+/* create_graph_from_edge_list<graph> (graph_base.h:335-436) on the device:
+ * both directions of every edge, each adjacency list in edge-list order.
+ * Out-of-range ids are rejected (the reference asserts, graph_base.h:362-363);
+ * like the reference, self-loops and duplicates are stored as given. */
+int bd_graph_from_edges(bd_ctx *ctx, const int64_t *src, const int64_t *dst,
+                        int64_t nv, int64_t m);
+
+/* dist_edge_list::load_binary (dist_edge_list.cc:50-143) + the above:
+ * reads an el64 file (header edge_list.cc:20-89) and builds the CSR. */
+int bd_graph_load_el64(bd_ctx *ctx, const char *path);
+
+/* Seeded generators, run on the device.  kind 0 = RMAT (a,b,c,d used;
+ * restates generator/rmat_generator.h and edge_list_utils.h flip/sort/dedup,
+ * then a seeded relabel and shuffle), 1 = Erdos-Renyi with uniform
+ * endpoints, 2 = rows x cols 4-neighbour lattice (nv = rows*cols, rows in
+ * `num_edges`).  *m_out receives the deduplicated edge count. */
+enum { BD_GEN_RMAT = 0, BD_GEN_ER = 1, BD_GEN_LATTICE = 2 };
+int bd_graph_generate(bd_ctx *ctx, int kind, int64_t nv, int64_t num_edges,
+                      double a, double b, double c, double d, uint32_t seed,
+                      int64_t *m_out);
+
+/* Size of the resident graph (num_vertices()/num_edges(), graph_base.h:231-237). */
+int bd_graph_info(bd_ctx *ctx, int64_t *nv, int64_t *nnz, int64_t *m_undirected);
+/* Copy the device CSR back (for checks and for the CPU baseline). */
+int bd_graph_download(bd_ctx *ctx, int64_t *offsets, int32_t *cols);
+/* Copy the generated/loaded edge list (el64 order) back; BD_ESTATE when the
+ * graph came from bd_graph_upload. */
+int bd_graph_edges_download(bd_ctx *ctx, int64_t *src, int64_t *dst);
+/* dist_edge_list -> el64 file (generator/edge_list_utils.h:171-198 dump_bin). */
+int bd_graph_save_el64(bd_ctx *ctx, const char *path);
+/* pick_random_vertex (hybrid_bfs_main.cc:136-144) n times from lcg(0). */
+int bd_pick_roots(bd_ctx *ctx, int64_t n, int64_t *roots);
+
+/* -------------------------------------------------------------------- BFS */
+
+/* hybrid_bfs::run_beamer(source, alpha, beta) (hybrid_bfs.cc:152-191) and the
+ * other policies above, preceded by clear() (hybrid_bfs.cc:433-443), which is
+ * not timed, as in hybrid_bfs_main.cc:261-282.  alpha, beta > 0
+ * (hybrid_bfs_main.cc:127-130).  source in [0, nv) (hybrid_bfs.cc:155). */
+int bd_bfs_run(bd_ctx *ctx, int64_t source, int64_t alpha, int64_t beta, int policy,
+               bd_run_stats *out);
+
+/* For BD_POLICY_SCRIPT: dirs[i] is the direction of step i; steps past n use
+ * the last entry. */
+int bd_bfs_set_script(bd_ctx *ctx, const int32_t *dirs, int64_t n);
+
+/* Results of the last run.  depth: -1 unreached.  parent: -1 unreached, the
+ * source is its own parent (reference: parent_ after run_beamer; the
+ * reference leaves -deg(v) in unreached entries, normalised here to -1). */
+int bd_bfs_fetch(bd_ctx *ctx, int32_t *depth, int64_t *parent);
+
+/* Per-step records of the last run (bd_bfs_run or bd_bfs_profile). */
+int bd_bfs_levels(bd_ctx *ctx, bd_level_rec *recs, int64_t cap, int64_t *n);
+
+/* hybrid_bfs::count_num_traversed_edges (hybrid_bfs.cc:401-414). */
+int bd_bfs_count_traversed(bd_ctx *ctx, int64_t *out);
+
+/* hybrid_bfs::check (hybrid_bfs.cc:281-379) against the last run, computed on
+ * the host from a downloaded copy.  *ok = 1 PASS, 0 FAIL. */
+int bd_bfs_check(bd_ctx *ctx, int64_t source, int *ok);
+
+/* Time both directions at every step from the same state (no reference
+ * counterpart: the per-step hook sites are hybrid_bfs.cc:173-177, 184-188).
+ * The search follows `policy`; each record gets ms (chosen direction) and
+ * ms_other.  Used to log training data for the learned policy. */
+int bd_bfs_profile(bd_ctx *ctx, int64_t source, int64_t alpha, int64_t beta, int policy,
+                   bd_run_stats *out);
+
+/* Launch tuning (autotuner output).  Any field <= 0 keeps the default. */
+typedef struct {
+    int32_t bu_block;        /* threads per bottom-up workgroup             */
+    int32_t td_block;        /* threads per top-down workgroup              */
+    int32_t grid_per_cu;     /* workgroups per CU for grid-stride kernels   */
+    int32_t wave_threshold;  /* degree at which a vertex gets a whole wave   */
+    int32_t task_threshold;  /* degree at which a vertex is split into tasks */
+    int32_t task_edges;      /* edges per task                             */
+    int32_t level_driver;    /* 1 = direction chosen on device and steps
+                                enqueued without a per-step host sync;
+                                2 = host reads the counters after each step */
+    int32_t time_steps;      /* 1 = time each step's kernels with HIP events
+                                (implies the host-driven loop), 2 = off */
+} bd_tuning;
+int bd_set_tuning(bd_ctx *ctx, const bd_tuning *t);
+int bd_get_tuning(bd_ctx *ctx, bd_tuning *t);
+
+/* ------------------------------------------------------- batched searches */
+
+/* 64-root bit-parallel search (no reference counterpart; replaces 64 trips
+ * round the trial loop hybrid_bfs_main.cc:252-302).  n <= 64 roots share one
+ * launch sequence; depths[r*nv + v] (may be NULL) receives root r's depths. */
+int bd_bfs_multi_root(bd_ctx *ctx, const int64_t *roots, int n, int32_t *depths,
+                      double *ms, int64_t *levels);
+
+/* Many independent graphs, one root each, in one launch (no reference
+ * counterpart).  Graphs are given as concatenated CSRs: goff[g] is the first
+ * vertex of graph g in the concatenated offsets, gcol[g] its first column.
+ * Offsets are local to each graph.  depths (may be NULL) is concatenated in
+ * the same vertex order. */
+int bd_bfs_multi_graph(bd_ctx *ctx, int ngraphs, const int64_t *vbase, const int64_t *offsets,
+                       const int32_t *cols, const int64_t *roots, int32_t *depths,
+                       double *ms);
+
+/* --------------------------------------------------- learned direction MLP */
+
+/* 8 -> h1 -> h2 -> 2 ReLU MLP predicting log step cost of (TD, BU).
+ * Parameters fp32, layout W1[h1*8] b1[h1] W2[h2*h1] b2[h2] W3[2*h2] b3[2]. */
+int bd_mlp_set(bd_ctx *ctx, const float *params, int h1, int h2);
+int bd_mlp_get(bd_ctx *ctx, float *params, int64_t cap, int *h1, int *h2);
+/* Features of a step record, as fed to the network (8 floats). */
+int bd_mlp_features(const bd_level_rec *rec, int64_t nv, int64_t m_undirected,
+                    int64_t prev_dir, float *x8);
+/* Device forward pass (fp32 reference kernel and bf16 MFMA single-wave
+ * kernel, selected by use_mfma). */
+int bd_mlp_infer(bd_ctx *ctx, const float *x, int64_t n, float *y, int use_mfma);
+/* Device loss and gradient (MSE over masked outputs). */
+int bd_mlp_loss_grad(bd_ctx *ctx, const float *x, const float *t, const float *mask,
+                     int64_t n, float *grad, float *loss);
+/* Device training: `epochs` full-batch Adam steps at learning rate lr;
+ * *loss receives the final loss. */
+int bd_mlp_train(bd_ctx *ctx, const float *x, const float *t, const float *mask, int64_t n,
+                 int epochs, float lr, uint32_t seed, float *loss);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -86,6 +86,7 @@ def lib():
         L.oracle_lcg_next.restype = ctypes.c_uint64
         L.oracle_pick_roots.argtypes = [ctypes.c_int64, _i64p, ctypes.c_int64, _i64p]
         L.oracle_aggregate.argtypes = [_f64p, ctypes.c_int64, _f64p]
+        L.oracle_set_mutation.argtypes = [ctypes.c_int]
         L.oracle_mlp_nparams.argtypes = [ctypes.c_int, ctypes.c_int]
         L.oracle_mlp_nparams.restype = ctypes.c_int64
         L.oracle_mlp_forward.argtypes = [_f32p, ctypes.c_int, ctypes.c_int, _f32p, ctypes.c_int64,
@@ -174,7 +175,7 @@ def build_csr(nv, src, dst):
     off = np.empty(nv + 1, np.int64)
     adj = np.empty(max(2 * len(s), 1), np.int32)
     if lib().oracle_build_csr(nv, len(s), s, d, off, adj):
-        raise ValueError("edge list has an out-of-range id or a self-loop")
+        raise ValueError("edge list has an out-of-range id")
     return off, adj[:2 * len(s)]
 
 
@@ -246,6 +247,20 @@ def pick_roots(off, n):
     roots = np.empty(n, np.int64)
     lib().oracle_pick_roots(len(off) - 1, off, n, roots)
     return roots
+
+
+class mutation:
+    """Context manager planting mistake `id` in the oracle (tests only)."""
+
+    def __init__(self, mid):
+        self.mid = mid
+
+    def __enter__(self):
+        lib().oracle_set_mutation(self.mid)
+        return self
+
+    def __exit__(self, *a):
+        lib().oracle_set_mutation(0)
 
 
 def aggregate(x):

@@ -87,7 +87,8 @@ int oracle_write_el64(const char *path, int64_t nv, int64_t m, const int64_t *sr
 /* ---------------- graph (graph_base.h:335-436) */
 
 /* Symmetric CSR in edge-list insertion order.  off[V+1], adj[2M].
- * Returns 0, or -1 when an id is out of range or a self-loop is present. */
+ * Returns 0, or -1 when an id is out of range (the reference asserts,
+ * graph_base.h:362-363).  Self-loops and duplicates are stored as given. */
 int oracle_build_csr(int64_t nv, int64_t m, const int64_t *src, const int64_t *dst,
                      int64_t *off, int32_t *adj);
 
@@ -152,6 +153,10 @@ void oracle_pick_roots(int64_t nv, const int64_t *off, int64_t n, int64_t *roots
 /* aggregate_stats::compute (combined.cc:108-146):
  * out = {sum, mean, hmean, stddev, min, max}. */
 void oracle_aggregate(const double *x, int64_t n, double out[6]);
+
+/* Plant a known mistake in the BFS restatement (0 = none; ids documented in
+ * oracle_bfs.c).  Tests use it to show their known answers catch each one. */
+void oracle_set_mutation(int id);
 
 /* ---------------- direction-cost MLP (no reference counterpart)
  * 8 -> H1 -> H2 -> 2, ReLU, fp32.  Parameter layout (row-major, out x in):

@@ -15,6 +15,16 @@
 #include <stdlib.h>
 #include <string.h>
 
+/* Planted mistakes, so the tests can show that their known answers catch
+ * each kind of error (0 = the faithful restatement).
+ *   1: BU loop continues only while awake > old_awake   (:178 uses >=)
+ *   2: TD steps do not charge scout to edges_to_check  (:182)
+ *   3: scout counts vertices instead of their degrees (:93 uses -deg)
+ *   4: BU commits parents during the search pass        (:117-141 two-phase)
+ *   5: check() does not test the parent's depth         (:356-360)        */
+static int g_mutation = 0;
+void oracle_set_mutation(int id) { g_mutation = id; }
+
 /* ------------------------------------------------------------ graph */
 
 int oracle_build_csr(int64_t nv, int64_t m, const int64_t *src, const int64_t *dst,
@@ -23,7 +33,7 @@ int oracle_build_csr(int64_t nv, int64_t m, const int64_t *src, const int64_t *d
     memset(off, 0, sizeof(int64_t) * (size_t)(nv + 1));
     for (int64_t i = 0; i < m; ++i) {
         int64_t s = src[i], d = dst[i];
-        if (s < 0 || s >= nv || d < 0 || d >= nv || s == d) return -1;
+        if (s < 0 || s >= nv || d < 0 || d >= nv) return -1;
         off[s + 1]++;
         off[d + 1]++;
     }
@@ -156,7 +166,7 @@ static int64_t top_down_step(bfs_state *b) {
             if (curr < 0) {
                 b->parent[dst] = src;
                 push(b, dst);
-                scout += -curr;
+                scout += g_mutation == 3 ? 1 : -curr;
             }
         }
     }
@@ -200,6 +210,11 @@ static int64_t bottom_up_step(bfs_state *b) {
         }
         int64_t u = find_parent_unrolled(b, child, &dummy);
         if (u >= 0) b->new_parent[child] = u;
+        if (u >= 0 && g_mutation == 4) {
+            b->parent[child] = u;
+            push(b, child);
+            awake += 1;
+        }
     }
     for (int64_t v = 0; v < b->nv; ++v) {
         if (b->parent[v] < 0 && b->new_parent[v] >= 0) {
@@ -279,10 +294,12 @@ int64_t oracle_beamer(int64_t nv, int64_t m, const int64_t *off, const int32_t *
                 }
                 ++nlog;
                 b.level++;
-            } while (awake_count >= old_awake_count || (awake_count > nv / beta));
+            } while ((g_mutation == 1 ? awake_count > old_awake_count
+                                      : awake_count >= old_awake_count) ||
+                     (awake_count > nv / beta));
             scout_count = 1;
         } else {
-            edges_to_check -= scout_count;
+            if (g_mutation != 2) edges_to_check -= scout_count;
             oracle_level_t r;
             memset(&r, 0, sizeof(r));
             if (log) level_stats(&b, &r);
@@ -374,7 +391,7 @@ int oracle_check(int64_t nv, const int64_t *off, const int32_t *adj, int64_t sou
             for (int64_t e = off[u]; e < off[u + 1]; ++e) {
                 int64_t v = adj[e];
                 if (v == parent[u]) {
-                    if (depth[v] != depth[u] - 1) break;
+                    if (g_mutation != 5 && depth[v] != depth[u] - 1) break;
                     found = 1;
                     break;
                 }

@@ -1,0 +1,269 @@
+"""GPU parity: the HIP path, called through the C ABI, against the oracle on
+the same seeded inputs.  Integer work, so every comparison is bit-exact."""
+import os
+import tempfile
+
+import numpy as np
+import pytest
+
+import beedrill_amd as bd
+from conftest import complete_graph, path_graph, star_graph, two_components
+
+pytestmark = pytest.mark.gpu
+
+LEVEL_KEYS = ("level", "dir", "frontier", "frontier_deg", "unvisited", "unvisited_deg",
+              "edges_to_check", "result", "next", "next_deg", "max_frontier_deg")
+
+
+def oracle_csr(orc, nv, s, d):
+    off, adj = orc.build_csr(nv, s, d)
+    return off, adj, len(s)
+
+
+def assert_same_levels(gpu_levels, orc_levels, keys=LEVEL_KEYS):
+    assert len(gpu_levels) == len(orc_levels)
+    for g, o in zip(gpu_levels, orc_levels):
+        for k in keys:
+            assert g[k] == o[k], (k, g, o)
+
+
+def run_and_compare(ctx, orc, off, adj, m, root, policy="beamer", alpha=15, beta=18,
+                    levels=True):
+    st = ctx.bfs(root, alpha, beta, policy)
+    depth, parent = ctx.fetch()
+    ref_parent, ref_depth, ref_lv = orc.beamer(off, adj, m, root, alpha, beta)
+    np.testing.assert_array_equal(depth, ref_depth)
+    assert orc.check(off, adj, root, parent)
+    assert ((parent < 0) == (depth < 0)).all()
+    assert st["traversed_edges"] == orc.count_traversed(off, ref_parent)
+    assert ctx.count_traversed() == st["traversed_edges"]
+    if levels and policy == "beamer":
+        assert_same_levels(ctx.levels(), ref_lv)
+    return st
+
+
+# ------------------------------------------------------------- generator
+
+@pytest.mark.parametrize("scale,ef,seed,abcd", [(10, 16, 0, (0.57, 0.19, 0.19, 0.05)),
+                                                (12, 8, 3, (0.45, 0.25, 0.25, 0.05)),
+                                                (4, 16, 1, (0.57, 0.19, 0.19, 0.05))])
+def test_device_rmat_matches_oracle(ctx, orc, scale, ef, seed, abcd):
+    m = ctx.generate_rmat(scale, ef, *abcd, seed=seed)
+    s, d = ctx.edges()
+    nv, os_, od = orc.make_rmat(scale, ef, *abcd, seed=seed)
+    assert m == len(os_)
+    np.testing.assert_array_equal(s, os_)
+    np.testing.assert_array_equal(d, od)
+    off, cols = ctx.csr()
+    ooff, oadj = orc.build_csr(nv, os_, od)
+    np.testing.assert_array_equal(off, ooff)
+    np.testing.assert_array_equal(cols, oadj)
+
+
+def test_device_er_and_lattice_match_oracle(ctx, orc):
+    ctx.generate_er(5000, 40000, seed=1)
+    s, d = ctx.edges()
+    _, os_, od = orc.make_er(5000, 40000, seed=1)
+    np.testing.assert_array_equal(s, os_)
+    np.testing.assert_array_equal(d, od)
+    ctx.generate_lattice(20, 30, seed=2)
+    s, d = ctx.edges()
+    _, os_, od = orc.make_lattice(20, 30, seed=2)
+    np.testing.assert_array_equal(s, os_)
+    np.testing.assert_array_equal(d, od)
+
+
+# ------------------------------------------------------ hand-built graphs
+
+@pytest.mark.parametrize("g,root", [(path_graph(100), 0), (star_graph(100), 1),
+                                    (star_graph(100), 0), (complete_graph(20), 0),
+                                    (two_components(), 0), (two_components(), 3)])
+def test_small_graphs_match_oracle(ctx, orc, g, root):
+    nv, s, d = g
+    ctx.from_edges(nv, s, d)
+    off, adj, m = oracle_csr(orc, nv, s, d)
+    goff, gcols = ctx.csr()
+    np.testing.assert_array_equal(goff, off)
+    np.testing.assert_array_equal(gcols, adj)
+    run_and_compare(ctx, orc, off, adj, m, root)
+    assert ctx.check(root)
+
+
+def test_isolated_source_and_self_loops(ctx, orc):
+    # vertex 5 isolated; a self-loop on 2 and a duplicate edge stay as given
+    nv = 6
+    s = np.array([0, 1, 2, 0], np.int64)
+    d = np.array([1, 2, 2, 1], np.int64)
+    ctx.from_edges(nv, s, d)
+    off, adj, m = oracle_csr(orc, nv, s, d)
+    np.testing.assert_array_equal(ctx.csr()[1], adj)
+    run_and_compare(ctx, orc, off, adj, m, 0)
+    st = ctx.bfs(5)
+    depth, parent = ctx.fetch()
+    assert depth.tolist() == [-1, -1, -1, -1, -1, 0] and parent[5] == 5
+    assert st["steps"] == 1 and st["traversed_edges"] == 0
+
+
+# ------------------------------------------------------------ RMAT scale 16
+
+@pytest.fixture(scope="module")
+def s16(ctx, orc):
+    """Config: RMAT scale 16, edge factor 16, A/B/C/D .57/.19/.19/.05, seed 0."""
+    ctx.generate_rmat(16, 16)
+    off, cols = ctx.csr()
+    _, m_, _ = ctx.info()
+    return off, cols, ctx.info()[2]
+
+
+def test_s16_beamer_parity(ctx, orc, s16):
+    ctx.generate_rmat(16, 16)
+    off, adj, m = s16
+    roots = ctx.pick_roots(8)
+    np.testing.assert_array_equal(roots, orc.pick_roots(off, 8))
+    for root in roots:
+        run_and_compare(ctx, orc, off, adj, m, int(root))
+
+
+@pytest.mark.parametrize("policy", ["td_only", "bu_only"])
+def test_s16_other_policies_same_depth(ctx, orc, s16, policy):
+    ctx.generate_rmat(16, 16)
+    off, adj, m = s16
+    root = int(ctx.pick_roots(1)[0])
+    st = run_and_compare(ctx, orc, off, adj, m, root, policy=policy, levels=False)
+    lv = ctx.levels()
+    want = bd.DIR_TD if policy == "td_only" else bd.DIR_BU
+    assert all(r["dir"] == want for r in lv)
+    if policy == "td_only":
+        _, ref_depth = orc.td_only(off, adj, m, root)
+        np.testing.assert_array_equal(ctx.fetch()[0], ref_depth)
+    assert st["steps"] == len(lv)
+
+
+def test_s16_scripted_directions_same_depth(ctx, orc, s16):
+    ctx.generate_rmat(16, 16)
+    off, adj, m = s16
+    rng = np.random.default_rng(0)
+    for root in ctx.pick_roots(3):
+        ctx.set_script(rng.integers(0, 2, size=32).astype(np.int32))
+        run_and_compare(ctx, orc, off, adj, m, int(root), policy="script", levels=False)
+
+
+def test_s16_host_driven_levels_identical(ctx, orc, s16):
+    ctx.generate_rmat(16, 16)
+    off, adj, m = s16
+    root = int(ctx.pick_roots(2)[1])
+    t0 = ctx.get_tuning()
+    try:
+        ctx.set_tuning(level_driver=2)
+        run_and_compare(ctx, orc, off, adj, m, root)
+        ctx.set_tuning(level_driver=1, bu_block=128, td_block=512, wave_threshold=8,
+                       task_threshold=256, task_edges=128)
+        run_and_compare(ctx, orc, off, adj, m, root)
+    finally:
+        ctx.set_tuning(**t0)
+
+
+def test_s16_alpha_beta_sweep(ctx, orc, s16):
+    ctx.generate_rmat(16, 16)
+    off, adj, m = s16
+    root = int(ctx.pick_roots(1)[0])
+    for alpha, beta in [(1, 1), (2, 100), (15, 18), (100, 2), (10 ** 9, 10 ** 9)]:
+        run_and_compare(ctx, orc, off, adj, m, root, alpha=alpha, beta=beta)
+
+
+def test_profile_times_both_directions(ctx, orc, s16):
+    ctx.generate_rmat(16, 16)
+    off, adj, m = s16
+    root = int(ctx.pick_roots(1)[0])
+    ctx.profile(root)
+    lv = ctx.levels()
+    depth, parent = ctx.fetch()
+    _, ref_depth, ref_lv = orc.beamer(off, adj, m, root)
+    np.testing.assert_array_equal(depth, ref_depth)
+    assert orc.check(off, adj, root, parent)
+    assert_same_levels(lv, ref_lv)
+    assert all(r["ms"] > 0 and r["ms_other"] > 0 for r in lv)
+
+
+# ------------------------------------------------------------ el64 files
+
+def test_el64_roundtrip_and_oracle_file(ctx, orc):
+    nv, s, d = orc.make_rmat(10, 16, seed=9)
+    off, adj, m = oracle_csr(orc, nv, s, d)
+    with tempfile.TemporaryDirectory() as t:
+        p = os.path.join(t, "g.el64")
+        orc.write_el64(p, nv, s, d)
+        ctx.load_el64(p)
+        goff, gcols = ctx.csr()
+        np.testing.assert_array_equal(goff, off)
+        np.testing.assert_array_equal(gcols, adj)
+        p2 = os.path.join(t, "g2.el64")
+        ctx.save_el64(p2)
+        with open(p, "rb") as f1, open(p2, "rb") as f2:
+            assert f1.read() == f2.read()
+        run_and_compare(ctx, orc, off, adj, m, int(orc.pick_roots(off, 1)[0]))
+
+
+def test_el64_errors(ctx):
+    with tempfile.TemporaryDirectory() as t:
+        def write(name, header, payload=b""):
+            p = os.path.join(t, name)
+            with open(p, "wb") as f:
+                f.write(header + payload)
+            return p
+        pairs = np.array([[0, 1], [1, 2]], np.int64).tobytes()
+        good = b" --format el64 --num_edges 2 --num_vertices 3 --is_deduped\n"
+        ctx.load_el64(write("ok", good, pairs))
+        assert ctx.info() == (3, 4, 2)
+        prefix = b" --form el64 --num_e 2 --num_v=3 --is_dedup\n"
+        ctx.load_el64(write("prefix", prefix, pairs))
+        for name, hdr, body, code in [
+                ("short", good, pairs[:16], -5),
+                ("notdedup", b" --format el64 --num_edges 2 --num_vertices 3\n", pairs, -5),
+                ("fmt", b" --format el32 --num_edges 2 --num_vertices 3 --is_deduped\n", pairs, -5),
+                ("size", b" --format el64 --num_edges 0 --num_vertices 3 --is_deduped\n", b"", -5),
+                ("unknown", b" --format el64 --bogus --num_edges 2 --num_vertices 3 --is_deduped\n",
+                 pairs, -5),
+                ("range", b" --format el64 --num_edges 2 --num_vertices 2 --is_deduped\n", pairs, -6)]:
+            with pytest.raises(bd.BeedrillError) as e:
+                ctx.load_el64(write(name, hdr, body))
+            assert e.value.code == code, name
+        with pytest.raises(bd.BeedrillError) as e:
+            ctx.load_el64(os.path.join(t, "missing"))
+        assert e.value.code == -5
+
+
+def test_argument_errors(ctx, orc):
+    nv, s, d = path_graph(10)
+    ctx.from_edges(nv, s, d)
+    for kw, code in [(dict(source=10), -6), (dict(source=-1), -6), (dict(source=0, alpha=0), -1),
+                     (dict(source=0, beta=0), -1)]:
+        with pytest.raises(bd.BeedrillError) as e:
+            ctx.bfs(**kw)
+        assert e.value.code == code
+    with pytest.raises(bd.BeedrillError) as e:
+        ctx.from_edges(4, np.array([0]), np.array([7]))
+    assert e.value.code == -6
+
+
+# ------------------------------------------------------------ RMAT scale 22
+
+def test_s22_depth_bit_exact_and_parents_valid(ctx, orc):
+    """Config: RMAT scale 22 (4,194,304 vertices, 67,108,864 generated edges).
+    Depth against the oracle's textbook BFS for 3 roots; parents through the
+    oracle's check(); per-level counters sum to the reached set."""
+    m = ctx.generate_rmat(22, 16)
+    off, adj = ctx.csr()
+    nv = len(off) - 1
+    assert nv == 1 << 22 and 0.85 * (16 << 22) < m < 16 << 22
+    deg = np.diff(off)
+    for root in ctx.pick_roots(3):
+        root = int(root)
+        st = ctx.bfs(root)
+        depth, parent = ctx.fetch()
+        np.testing.assert_array_equal(depth, orc.bfs_depth(off, adj, root))
+        assert orc.check(off, adj, root, parent)
+        lv = ctx.levels()
+        assert 1 + sum(r["next"] for r in lv) == st["reached"] == int((depth >= 0).sum())
+        assert st["traversed_edges"] == int(deg[depth >= 0].sum()) // 2
+        assert any(r["dir"] == bd.DIR_BU for r in lv)
