@@ -29,6 +29,10 @@ MODEL = os.path.join(ROOT, "beedrill_amd", "models", "direction_mlp_32x32.f32")
 TRAFFIC = os.path.join(ROOT, "profiles", "bottom_up_traffic_rmat_s24.json")
 
 
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
 def hmean(x):
     x = np.asarray(x, np.float64)
     return float(len(x) / np.sum(1.0 / x))
@@ -131,24 +135,24 @@ def main():
     m = ctx.generate_rmat(args.scale, args.edge_factor, seed=args.seed)
     gen_s = time.perf_counter() - t0
     nv, nnz, _ = ctx.info()
+    log(f"graph: scale {args.scale}, {nv} vertices, {m} edges, built in {gen_s:.2f} s")
     roots = ctx.pick_roots(args.steps)
-    off_h = None
-    n_isolated = None
     # degree-0 vertices are never scanned by a bottom-up step
-    off_h, _ = ctx.csr() if not args.no_cpu else (None, None)
-    if off_h is None:
-        off_h = np.empty(nv + 1, np.int64)
-        ctx._L.bd_graph_download(ctx._h, off_h.ctypes.data, None)
+    off_h = np.empty(nv + 1, np.int64)
+    ctx._ok(ctx._L.bd_graph_download(ctx._h, off_h.ctypes.data, None))
     n_isolated = int(np.sum(np.diff(off_h) == 0))
 
     results = {"beamer": run_policy(ctx, roots, "beamer", args.warmup)}
+    log(f"beamer: {results['beamer']['hmean_gteps']:.2f} GTEPS (harmonic mean)")
     have_model = os.path.exists(MODEL)
     if have_model:
         ctx.mlp_set(np.fromfile(MODEL, np.float32), 32, 32)
         results["learned"] = run_policy(ctx, roots, "learned", args.warmup)
+        log(f"learned: {results['learned']['hmean_gteps']:.2f} GTEPS (harmonic mean)")
     head = results["learned"] if have_model else results["beamer"]
     policy = "learned" if have_model else "beamer"
     roof = roofline(ctx, roots[:args.roofline_roots], policy, nv, n_isolated)
+    log(f"roofline: {roof['achieved']} GB/s over {roof['launches']} bottom-up launches")
     cpu = None if args.no_cpu else cpu_baseline(ctx, roots, args.cpu_budget)
 
     out = {
