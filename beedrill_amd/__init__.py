@@ -39,7 +39,7 @@ class LevelRec(ctypes.Structure):
     _fields_ = [(n, ctypes.c_int64) for n in (
         "level", "dir", "frontier", "frontier_deg", "unvisited", "unvisited_deg",
         "edges_to_check", "result", "next", "next_deg", "edges_examined",
-        "max_frontier_deg", "td_bitmap_input", "reserved")] + [
+        "max_frontier_deg", "tasks", "queued")] + [
         ("ms", ctypes.c_double), ("ms_other", ctypes.c_double)]
 
 
@@ -54,11 +54,11 @@ class RunStats(ctypes.Structure):
 
 class Tuning(ctypes.Structure):
     _fields_ = [(n, ctypes.c_int32) for n in (
-        "bu_block", "td_block", "grid_per_cu", "wave_threshold", "task_threshold",
-        "task_edges", "level_driver", "time_steps")]
+        "block", "grid_per_cu", "wave_threshold", "task_threshold", "task_edges",
+        "bu_words", "bu_cap", "level_driver", "time_steps", "reserved")]
 
     def as_dict(self):
-        return {f: getattr(self, f) for f, _ in self._fields_}
+        return {f: getattr(self, f) for f, _ in self._fields_ if f != "reserved"}
 
 
 class BeedrillError(RuntimeError):

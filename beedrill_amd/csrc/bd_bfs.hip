@@ -2,33 +2,39 @@
 // hybrid_bfs.cc:152-191).
 //
 // Device state per search (V vertices, W = ceil(V/64) 64-bit words):
-//   depth[V], parent[V]  int32     result arrays (-1 = unreached)
-//   vis[W]               bitmap    visited; degree-0 vertices pre-set
-//   bm[3][W]             bitmaps   frontier rotation: bm[L%3] = depth-L set,
-//                                  bm[(L+1)%3] = next, bm[(L+2)%3] is zeroed
-//                                  during step L for step L+1's use
-//   q[2][V]              int32     frontier queues (top-down input/output)
-//   tasks                          slices of huge adjacency lists
-//   ctl                            control block (counters, policy state)
+//   depth[V], parent[V]  int32     results (-1 = unreached)
+//   vis[W]               bitmap    visited; degree-0 vertices are pre-set
+//   bm[3][W]             bitmaps   bm[L%3] = depth-L set (bottom-up input),
+//                                  bm[(L+1)%3] = set found by step L,
+//                                  bm[(L+2)%3] zeroed during step L
+//   q[2][V]              int32     frontier vertices of degree < task_t
+//   tasks[2][..]                   edge slices of frontier vertices of degree
+//                                  >= task_t (the reference's 64-edge
+//                                  worklist granules, worklist.h:96-117)
+//   ctl                            counters and direction-policy state
 //
-// One step = one BFS level.  Top-down: k_top_down expands F (from the queue,
-// or from the bitmap after a bottom-up step), claims vertices with atomicOr
-// on vis, and appends them to the next queue with one atomic per wave;
-// k_top_down_tasks expands the slices of vertices whose degree exceeds the
-// task threshold.  Bottom-up: k_bottom_up gives each wave one 64-vertex word
-// of vis, so the wave owns that word and writes it (and the next-frontier
-// word) with plain stores; each unvisited lane scans its neighbours in CSR
-// order and stops at the first one whose bit is set in bm[L%3].  Reading the
-// depth-L bitmap rather than vis keeps the reference's two-phase rule
-// (hybrid_bfs.cc:117-141): a vertex found in this step is never seen as a
-// parent in the same step.  k_decide closes the step, writes its record and
-// picks the next direction on the device.
+// One BFS level is one launch of k_step.  Its direction is read from ctl.
+//   Top-down (hybrid_bfs.cc:71-100): blocks take tiles of the queue; per
+//   tile, vertices of degree >= wave_t get one wave each, the rest are
+//   concatenated by a block-wide scan and every thread takes one edge.
+//   Then waves take task slices.  A vertex is claimed by atomicOr on vis.
+//   Bottom-up (hybrid_bfs.cc:111-143): each wave owns chunks of `bu_words`
+//   visited words; lanes load the words, the unvisited vertices are ranked by
+//   a wave prefix sum, and then each lane takes one unvisited vertex and scans
+//   its list in CSR order, stopping at the first neighbour whose bit is set
+//   in bm[L%3].  Lists longer than bu_cap are finished by the whole wave.
+//   Reading the depth-L bitmap, not vis, keeps the reference's two-phase rule
+//   (:117-141): a vertex found in this step is never a parent in this step.
+// Both directions emit the next frontier the same way: queue entries staged
+// in LDS (one atomic per block flush) and task slices for high-degree
+// vertices, plus the depth-(L+1) bitmap.  The last workgroup to finish
+// closes the step (record, counters) and picks the next direction, so a
+// whole search is a stream of k_step launches with no host round trip.
 #include <math.h>
 #include <stdio.h>
 #include <string.h>
 
 #include <algorithm>
-#include <queue>
 #include <vector>
 
 #include "bd_internal.h"
@@ -39,44 +45,40 @@ namespace bd {
 namespace {
 
 constexpr int kMaxLevelsPerSync = 64;
+constexpr int kQCap = 2048;  // LDS-staged queue entries per workgroup
 
-struct td_args {
+struct step_args {
     const int64_t *off;
     const int32_t *cols;
     uint64_t *vis;
-    const int32_t *qin;
     const uint64_t *front;
     uint64_t *next_bm;
     uint64_t *clr;
+    const int32_t *qin;
     int32_t *qout;
+    const td_task *tin;
+    td_task *tout;
     int32_t *depth;
     int32_t *parent;
-    td_task *tasks;
     ctl_block *c;
+    bd_level_rec *log;
+    const int32_t *script;
+    const float *mlp;
     int64_t nwords;
     int64_t task_cap;
+    int64_t nv, m, nnz;
+    int64_t alpha, beta;
     int32_t level;
-    int32_t gate;
-    int32_t wave_t;
-    int32_t task_t;
-    int32_t task_edges;
-    int32_t pad;
+    int32_t gate;          // skip when the search is done or at another level
+    int32_t force_dir;     // -1: ctl->dir; 0/1: run this direction (profiling)
+    int32_t fused_decide;  // the last workgroup closes the step
+    int32_t wave_t, task_t, task_edges;
+    int32_t bu_words, bu_cap;
+    int32_t policy, script_len, log_cap;
+    int32_t mlp_h1, mlp_h2, has_mlp;
 };
 
-struct bu_args {
-    const int64_t *off;
-    const int32_t *cols;
-    uint64_t *vis;
-    const uint64_t *front;
-    uint64_t *next;
-    uint64_t *clr;
-    int32_t *depth;
-    int32_t *parent;
-    ctl_block *c;
-    int64_t nwords;
-    int32_t level;
-    int32_t gate;
-};
+// ----------------------------------------------------------------- helpers
 
 __device__ __forceinline__ unsigned long long wave_sum(unsigned long long x) {
 #pragma unroll
@@ -87,123 +89,176 @@ __device__ __forceinline__ unsigned long long wave_sum(unsigned long long x) {
 __device__ __forceinline__ unsigned long long wave_max(unsigned long long x) {
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) {
-        unsigned long long y = (unsigned long long)__shfl_xor((long long)x, o);
+        const unsigned long long y = (unsigned long long)__shfl_xor((long long)x, o);
         x = y > x ? y : x;
     }
     return x;
 }
 
-// Reduce four per-thread counters over the block; one atomic each per block.
+__device__ __forceinline__ int lane_id() { return threadIdx.x & 63; }
+
+__device__ __forceinline__ unsigned long long ld_acquire(const unsigned long long *p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Position of the k-th set bit (k from 0) of x.
+__device__ __forceinline__ int select_bit(uint64_t x, int k) {
+    int pos = 0;
+#pragma unroll
+    for (int s = 32; s > 0; s >>= 1) {
+        const uint64_t low = x & ((1ULL << s) - 1);
+        const int c = __popcll(low);
+        if (k >= c) {
+            k -= c;
+            x >>= s;
+            pos += s;
+        }
+    }
+    return pos;
+}
+
+// Per-workgroup LDS state shared by both directions (about 19 KB at 256
+// threads, so eight workgroups fit a CU).  The network scratch of the
+// direction decision reuses the queue staging area, which is empty by then.
+constexpr int kMaxBlock = 512;
+struct step_smem {
+    union {
+        int32_t q[kQCap];
+        mlp_scratch mlp;
+    };
+    int32_t qn;
+    int32_t qbase;
+    int last;
+    int use_mlp;
+    float x[8];
+    unsigned long long red[4][kMaxBlock / 64];
+    // top-down tile
+    int64_t t_beg[kMaxBlock];
+    int64_t t_end[kMaxBlock];
+    int32_t t_src[kMaxBlock];
+    int32_t t_pref[kMaxBlock + 1];
+    int32_t t_med[kMaxBlock];
+    int32_t t_wsum[kMaxBlock / 64];
+    int32_t t_nmed;
+    // bottom-up: found bits of the chunk each wave owns
+    uint64_t b_found[kMaxBlock / 64][64];
+};
+
+// Emit vertex u of the next frontier: high-degree vertices become task
+// slices, the others are staged in LDS.  Every lane of the wave must call
+// this together (emit = false for idle lanes).
+__device__ __forceinline__ void emit_next(const step_args &a, step_smem &s, bool emit, int32_t u,
+                                          int64_t b, int64_t e) {
+    const int64_t dg = e - b;
+    const bool huge = emit && dg >= a.task_t;
+    if (huge) {
+        const int64_t nt = (dg + a.task_edges - 1) / a.task_edges;
+        const int64_t base =
+            (int64_t)atomicAdd(&a.c->ntasks[(a.level + 1) & 1], (unsigned long long)nt);
+        for (int64_t t = 0; t < nt; ++t) {
+            const int64_t idx = base + t;
+            if (idx < a.task_cap) {
+                td_task tk;
+                tk.beg = b + t * a.task_edges;
+                tk.end = tk.beg + a.task_edges < e ? tk.beg + a.task_edges : e;
+                tk.src = u;
+                tk.pad = 0;
+                a.tout[idx] = tk;
+            } else {
+                atomicAdd(&a.c->task_overflow, 1ULL);
+            }
+        }
+    }
+    const bool q = emit && !huge;
+    const uint64_t m = __ballot(q);
+    if (m == 0) return;
+    const int lane = lane_id();
+    const int leader = __ffsll((unsigned long long)m) - 1;
+    const int cnt = __popcll(m);
+    int p = 0;
+    if (lane == leader) p = atomicAdd(&s.qn, cnt);
+    p = __shfl(p, leader);
+    const int rank = __popcll(m & ((1ULL << lane) - 1));
+    // Slots below kQCap are staged in LDS (the flush writes exactly those);
+    // the part of the batch past the end goes straight to HBM.
+    const int in_lds = p >= kQCap ? 0 : (cnt < kQCap - p ? cnt : kQCap - p);
+    long long g = 0;
+    if (in_lds < cnt) {
+        if (lane == leader)
+            g = (long long)atomicAdd(&a.c->qlen[(a.level + 1) & 1],
+                                     (unsigned long long)(cnt - in_lds));
+        g = __shfl(g, leader);
+    }
+    if (q) {
+        if (rank < in_lds)
+            s.q[p + rank] = u;
+        else
+            a.qout[g + (rank - in_lds)] = u;
+    }
+}
+
+// Write the staged queue entries to HBM (whole workgroup).
 template <int BLOCK>
-__device__ void block_flush(ctl_block *c, unsigned long long cnt, unsigned long long dsum,
-                            unsigned long long exam, unsigned long long dmax) {
-    __shared__ unsigned long long red[4][BLOCK / 64];
-    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-    cnt = wave_sum(cnt);
-    dsum = wave_sum(dsum);
-    exam = wave_sum(exam);
-    dmax = wave_max(dmax);
-    if (lane == 0) {
-        red[0][wid] = cnt;
-        red[1][wid] = dsum;
-        red[2][wid] = exam;
-        red[3][wid] = dmax;
+__device__ void flush_queue(const step_args &a, step_smem &s) {
+    __syncthreads();
+    const int n = s.qn < kQCap ? s.qn : kQCap;
+    if (n > 0) {
+        if (threadIdx.x == 0)
+            s.qbase = (int32_t)atomicAdd(&a.c->qlen[(a.level + 1) & 1], (unsigned long long)n);
+        __syncthreads();
+        const int64_t base = s.qbase;
+        for (int i = threadIdx.x; i < n; i += BLOCK) a.qout[base + i] = s.q[i];
     }
     __syncthreads();
-    if (threadIdx.x == 0) {
-        for (int w = 1; w < BLOCK / 64; ++w) {
-            cnt += red[0][w];
-            dsum += red[1][w];
-            exam += red[2][w];
-            dmax = red[3][w] > dmax ? red[3][w] : dmax;
-        }
-        if (cnt) atomicAdd(&c->acc_new, cnt);
-        if (dsum) atomicAdd(&c->acc_new_deg, dsum);
-        if (exam) atomicAdd(&c->acc_examined, exam);
-        if (dmax) atomicMax(&c->acc_max_deg, dmax);
-    }
+    if (threadIdx.x == 0) s.qn = 0;
+    __syncthreads();
 }
 
-__device__ __forceinline__ bool step_gated_out(const ctl_block *c, int32_t level, int32_t dir) {
-    return c->done || c->dir != dir || c->level != level;
-}
+// ---------------------------------------------------------------- top-down
 
-// Visit edge (src -> u) for one lane.  Every lane of the wave must call this
-// together (valid = false for idle lanes): the queue append is one atomic per
-// wave.
-__device__ __forceinline__ void td_visit(const td_args &a, bool valid, int32_t u, int32_t src,
-                                         unsigned long long &cnt, unsigned long long &dsum,
-                                         unsigned long long &dmax,
-                                         unsigned long long *qlen_out) {
+// Visit edge (src -> u).  Wave-collective.
+__device__ __forceinline__ void td_visit(const step_args &a, step_smem &s, bool valid, int32_t u,
+                                         int32_t src, unsigned long long &cnt,
+                                         unsigned long long &dsum, unsigned long long &dmax) {
     bool won = false;
     const uint64_t bit = 1ULL << (u & 63);
     if (valid) {
         uint64_t *wp = a.vis + (u >> 6);
         if (!(*wp & bit)) {
-            const unsigned long long old = atomicOr((unsigned long long *)wp, (unsigned long long)bit);
+            const unsigned long long old =
+                atomicOr((unsigned long long *)wp, (unsigned long long)bit);
             won = !(old & bit);
         }
     }
+    int64_t b = 0, e = 0;
     if (won) {
         a.depth[u] = a.level + 1;
         a.parent[u] = src;
-        const unsigned long long d = (unsigned long long)(a.off[u + 1] - a.off[u]);
+        b = a.off[u];
+        e = a.off[u + 1];
+        const unsigned long long d = (unsigned long long)(e - b);
         ++cnt;
         dsum += d;
         dmax = d > dmax ? d : dmax;
         atomicOr((unsigned long long *)(a.next_bm + (u >> 6)), (unsigned long long)bit);
     }
-    const uint64_t m = __ballot(won);
-    if (m) {
-        const int lane = threadIdx.x & 63;
-        const int leader = __ffsll((unsigned long long)m) - 1;
-        long long base = 0;
-        if (lane == leader) base = (long long)atomicAdd(qlen_out, (unsigned long long)__popcll(m));
-        base = __shfl(base, leader);
-        if (won) a.qout[base + __popcll(m & ((1ULL << lane) - 1))] = u;
-    }
+    emit_next(a, s, won, u, b, e);
 }
 
-// Top-down step (hybrid_bfs.cc:71-100).  Each block takes tiles of BLOCK
-// frontier slots.  Per tile: degree >= task_t -> split into tasks for
-// k_top_down_tasks; degree >= wave_t -> one wave strides the list;
-// otherwise the tile's short lists are concatenated by a block-wide scan and
-// every thread takes one edge (owner found by binary search in LDS).
 template <int BLOCK>
-__global__ __launch_bounds__(BLOCK) void k_top_down(td_args a) {
+__device__ void td_body(const step_args &a, step_smem &s, unsigned long long &cnt,
+                        unsigned long long &dsum, unsigned long long &dmax) {
     ctl_block *c = a.c;
-    if (a.gate && step_gated_out(c, a.level, BD_DIR_TD)) return;
-    for (int64_t i = (int64_t)blockIdx.x * BLOCK + threadIdx.x; i < a.nwords;
-         i += (int64_t)gridDim.x * BLOCK)
-        a.clr[i] = 0;
-    const bool bitmap = c->td_bitmap != 0;
-    const int64_t nitems = bitmap ? a.nwords * 64 : (int64_t)c->qlen[a.level & 1];
-    unsigned long long *qlen_out = &c->qlen[(a.level + 1) & 1];
-
-    __shared__ int64_t s_beg[BLOCK];
-    __shared__ int64_t s_end[BLOCK];
-    __shared__ int32_t s_src[BLOCK];
-    __shared__ int32_t s_pref[BLOCK + 1];
-    __shared__ int32_t s_med[BLOCK];
-    __shared__ int32_t s_wsum[BLOCK / 64];
-    __shared__ int32_t s_nmed;
-
     const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
-    unsigned long long cnt = 0, dsum = 0, dmax = 0;
+    const int64_t nitems = (int64_t)c->qlen[a.level & 1];
+    const int64_t nt_all = (int64_t)c->ntasks[a.level & 1];
+    const int64_t ntasks = nt_all < a.task_cap ? nt_all : a.task_cap;
 
     for (int64_t tile = blockIdx.x; tile * BLOCK < nitems; tile += gridDim.x) {
-        if (tid == 0) s_nmed = 0;
+        if (tid == 0) s.t_nmed = 0;
         __syncthreads();
         const int64_t slot = tile * BLOCK + tid;
-        int32_t v = -1;
-        if (slot < nitems) {
-            if (bitmap) {
-                const uint64_t fw = a.front[slot >> 6];
-                if ((fw >> (slot & 63)) & 1) v = (int32_t)slot;
-            } else {
-                v = a.qin[slot];
-            }
-        }
+        const int32_t v = slot < nitems ? a.qin[slot] : -1;
         int64_t b = 0, e = 0;
         if (v >= 0) {
             b = a.off[v];
@@ -211,56 +266,39 @@ __global__ __launch_bounds__(BLOCK) void k_top_down(td_args a) {
         }
         const int64_t dg = e - b;
         int32_t small = 0;
-        if (dg >= a.task_t) {
-            const int64_t nt = (dg + a.task_edges - 1) / a.task_edges;
-            const int64_t base = (int64_t)atomicAdd(&c->ntasks, (unsigned long long)nt);
-            for (int64_t t = 0; t < nt; ++t) {
-                const int64_t idx = base + t;
-                if (idx < a.task_cap) {
-                    td_task tk;
-                    tk.beg = b + t * a.task_edges;
-                    tk.end = tk.beg + a.task_edges < e ? tk.beg + a.task_edges : e;
-                    tk.src = v;
-                    tk.pad = 0;
-                    a.tasks[idx] = tk;
-                } else {
-                    atomicAdd(&c->task_overflow, 1ULL);
-                }
-            }
-        } else if (dg >= a.wave_t) {
-            const int p = atomicAdd(&s_nmed, 1);
-            s_med[p] = tid;
+        if (dg >= a.wave_t) {
+            const int p = atomicAdd(&s.t_nmed, 1);
+            s.t_med[p] = tid;
         } else {
             small = (int32_t)dg;
         }
-        s_beg[tid] = b;
-        s_end[tid] = e;
-        s_src[tid] = v;
+        s.t_beg[tid] = b;
+        s.t_end[tid] = e;
+        s.t_src[tid] = v;
         int32_t incl = small;
 #pragma unroll
         for (int o = 1; o < 64; o <<= 1) {
             const int32_t y = __shfl_up(incl, o);
             if (lane >= o) incl += y;
         }
-        if (lane == 63) s_wsum[wid] = incl;
+        if (lane == 63) s.t_wsum[wid] = incl;
         __syncthreads();
         int32_t wbase = 0;
-        for (int w2 = 0; w2 < wid; ++w2) wbase += s_wsum[w2];
-        s_pref[tid] = wbase + incl - small;
-        if (tid == BLOCK - 1) s_pref[BLOCK] = wbase + incl;
+        for (int w2 = 0; w2 < wid; ++w2) wbase += s.t_wsum[w2];
+        s.t_pref[tid] = wbase + incl - small;
+        if (tid == BLOCK - 1) s.t_pref[BLOCK] = wbase + incl;
         __syncthreads();
-        const int32_t total = s_pref[BLOCK];
-        const int nmed = s_nmed;
-
+        const int32_t total = s.t_pref[BLOCK];
+        const int nmed = s.t_nmed;
         for (int i = wid; i < nmed; i += BLOCK / 64) {
-            const int t = s_med[i];
-            const int64_t mb = s_beg[t], me = s_end[t];
-            const int32_t src = s_src[t];
+            const int t = s.t_med[i];
+            const int64_t mb = s.t_beg[t], me = s.t_end[t];
+            const int32_t src = s.t_src[t];
             for (int64_t k0 = mb; k0 < me; k0 += 64) {
                 const int64_t k = k0 + lane;
                 const bool valid = k < me;
                 const int32_t u = valid ? a.cols[k] : 0;
-                td_visit(a, valid, u, src, cnt, dsum, dmax, qlen_out);
+                td_visit(a, s, valid, u, src, cnt, dsum, dmax);
             }
         }
         for (int32_t i0 = 0; i0 < total; i0 += BLOCK) {
@@ -271,93 +309,165 @@ __global__ __launch_bounds__(BLOCK) void k_top_down(td_args a) {
                 int lo = 0, hi = BLOCK;
                 while (hi - lo > 1) {
                     const int mid = (lo + hi) >> 1;
-                    if (s_pref[mid] <= i)
+                    if (s.t_pref[mid] <= i)
                         lo = mid;
                     else
                         hi = mid;
                 }
-                src = s_src[lo];
-                u = a.cols[s_beg[lo] + (i - s_pref[lo])];
+                src = s.t_src[lo];
+                u = a.cols[s.t_beg[lo] + (i - s.t_pref[lo])];
             }
-            td_visit(a, valid, u, src, cnt, dsum, dmax, qlen_out);
+            td_visit(a, s, valid, u, src, cnt, dsum, dmax);
         }
         __syncthreads();
+        if (s.qn >= kQCap / 2) flush_queue<BLOCK>(a, s);
     }
-    block_flush<BLOCK>(c, cnt, dsum, 0, dmax);
-}
-
-// Slices of huge adjacency lists, one wave per slice (the reference's
-// 64-edge granules, worklist.h:96-117, with 64 lanes instead of 64 threads).
-template <int BLOCK>
-__global__ __launch_bounds__(BLOCK) void k_top_down_tasks(td_args a) {
-    ctl_block *c = a.c;
-    if (a.gate && step_gated_out(c, a.level, BD_DIR_TD)) return;
-    const int64_t nt_all = (int64_t)c->ntasks;
-    const int64_t ntasks = nt_all < a.task_cap ? nt_all : a.task_cap;
-    if (ntasks == 0) return;
-    unsigned long long *qlen_out = &c->qlen[(a.level + 1) & 1];
-    const int lane = threadIdx.x & 63;
-    const int64_t wave0 = (int64_t)blockIdx.x * (BLOCK / 64) + (threadIdx.x >> 6);
-    const int64_t nwaves = (int64_t)gridDim.x * (BLOCK / 64);
-    unsigned long long cnt = 0, dsum = 0, dmax = 0;
-    for (int64_t t = wave0; t < ntasks; t += nwaves) {
-        const td_task tk = a.tasks[t];
-        for (int64_t k0 = tk.beg; k0 < tk.end; k0 += 64) {
-            const int64_t k = k0 + lane;
-            const bool valid = k < tk.end;
-            const int32_t u = valid ? a.cols[k] : 0;
-            td_visit(a, valid, u, tk.src, cnt, dsum, dmax, qlen_out);
+    // task slices, one wave each, in workgroup-synchronous rounds
+    for (int64_t t0 = (int64_t)blockIdx.x * (BLOCK / 64); t0 < ntasks;
+         t0 += (int64_t)gridDim.x * (BLOCK / 64)) {
+        const int64_t t = t0 + wid;
+        if (t < ntasks) {
+            const td_task tk = a.tin[t];
+            for (int64_t k0 = tk.beg; k0 < tk.end; k0 += 64) {
+                const int64_t k = k0 + lane;
+                const bool valid = k < tk.end;
+                const int32_t u = valid ? a.cols[k] : 0;
+                td_visit(a, s, valid, u, tk.src, cnt, dsum, dmax);
+            }
         }
+        __syncthreads();
+        if (s.qn >= kQCap / 2) flush_queue<BLOCK>(a, s);
     }
-    block_flush<BLOCK>(c, cnt, dsum, 0, dmax);
 }
 
-// Bottom-up step (hybrid_bfs.cc:111-143).  One wave per 64-vertex word.
-template <int BLOCK>
-__global__ __launch_bounds__(BLOCK) void k_bottom_up(bu_args a) {
-    ctl_block *c = a.c;
-    if (a.gate && step_gated_out(c, a.level, BD_DIR_BU)) return;
-    const int lane = threadIdx.x & 63;
-    const int64_t wave0 = (int64_t)blockIdx.x * (BLOCK / 64) + (threadIdx.x >> 6);
-    const int64_t nwaves = (int64_t)gridDim.x * (BLOCK / 64);
-    unsigned long long cnt = 0, dsum = 0, exam = 0, dmax = 0;
-    for (int64_t w = wave0; w < a.nwords; w += nwaves) {
-        const uint64_t vw = a.vis[w];
-        const uint64_t unv = ~vw;
-        uint64_t found = 0;
-        if (unv != 0) {
-            const int64_t v = w * 64 + lane;
-            int32_t par = -1;
-            int64_t dg = 0;
-            if ((unv >> lane) & 1) {
-                const int64_t b = a.off[v], e = a.off[v + 1];
-                dg = e - b;
-                for (int64_t k = b; k < e; ++k) {
-                    const int32_t u = a.cols[k];
-                    ++exam;
-                    if ((a.front[u >> 6] >> (u & 63)) & 1) {
-                        par = u;
-                        break;
-                    }
+// --------------------------------------------------------------- bottom-up
+
+// Scan up to `cap` neighbours of v from position k in CSR order; returns the
+// first one in the frontier or -1, and the position reached in *k.
+__device__ __forceinline__ int32_t bu_scan(const step_args &a, int64_t *kp, int64_t e,
+                                           int64_t cap, unsigned long long &exam) {
+    int64_t k = *kp;
+    const int64_t stop = (k + cap < e) ? k + cap : e;
+    while (k < stop) {
+        int32_t u[4];
+        uint64_t w[4];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) u[j] = (k + j < stop) ? a.cols[k + j] : -1;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) w[j] = (u[j] >= 0) ? a.front[u[j] >> 6] : 0;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            if (u[j] >= 0) {
+                ++exam;
+                if ((w[j] >> (u[j] & 63)) & 1) {
+                    *kp = k + j + 1;
+                    return u[j];
                 }
             }
-            found = __ballot(par >= 0);
-            if (par >= 0) {
-                a.depth[v] = a.level + 1;
-                a.parent[v] = par;
-                ++cnt;
-                dsum += (unsigned long long)dg;
-                dmax = (unsigned long long)dg > dmax ? (unsigned long long)dg : dmax;
+        }
+        k += 4;
+    }
+    *kp = stop;
+    return -1;
+}
+
+template <int BLOCK>
+__device__ void bu_body(const step_args &a, step_smem &s, unsigned long long &cnt,
+                        unsigned long long &dsum, unsigned long long &dmax,
+                        unsigned long long &exam) {
+    const int lane = lane_id(), wid = threadIdx.x >> 6;
+    const int CW = a.bu_words;
+    const int64_t nchunks = (a.nwords + CW - 1) / CW;
+    for (int64_t c0 = (int64_t)blockIdx.x * (BLOCK / 64); c0 < nchunks;
+         c0 += (int64_t)gridDim.x * (BLOCK / 64)) {
+        const int64_t chunk = c0 + wid;
+        if (chunk < nchunks) {
+            const int64_t w0 = chunk * CW;
+            const int64_t wi = w0 + lane;
+            const bool own = lane < CW && wi < a.nwords;
+            const uint64_t vw = own ? a.vis[wi] : ~0ULL;
+            if (own) a.clr[wi] = 0;
+            const uint64_t unv = ~vw;
+            const int cntw = __popcll(unv);
+            int excl = cntw;
+#pragma unroll
+            for (int o = 1; o < 64; o <<= 1) {
+                const int y = __shfl_up(excl, o);
+                if (lane >= o) excl += y;
+            }
+            const int total = __shfl(excl, 63);
+            excl -= cntw;
+            s.b_found[wid][lane] = 0;
+            wave_lds_sync();
+            for (int r = 0; r < total; r += 64) {
+                const int g = r + lane;
+                const bool valid = g < total;
+                const int gg = valid ? g : total - 1;
+                // word j holding the gg-th unvisited vertex: largest j with excl_j <= gg
+                int j = 0;
+#pragma unroll
+                for (int st = 32; st > 0; st >>= 1) {
+                    const int mid = j + st;
+                    const int em = __shfl(excl, mid < 64 ? mid : 63);
+                    if (mid < 64 && em <= gg) j = mid;
+                }
+                const uint64_t bits = (uint64_t)__shfl((long long)unv, j);
+                const int ej = __shfl(excl, j);
+                const int bpos = select_bit(bits, gg - ej);
+                const int64_t v = (w0 + j) * 64 + bpos;
+                int32_t par = -1;
+                int64_t b = 0, e = 0, k = 0;
+                if (valid) {
+                    b = a.off[v];
+                    e = a.off[v + 1];
+                    k = b;
+                    par = bu_scan(a, &k, e, a.bu_cap, exam);
+                }
+                // lists longer than bu_cap: the whole wave finishes them
+                const bool lng = valid && par < 0 && k < e;
+                uint64_t lm = __ballot(lng);
+                while (lm) {
+                    const int src_lane = __ffsll((unsigned long long)lm) - 1;
+                    lm &= lm - 1;
+                    const int64_t kb = __shfl(k, src_lane), ke = __shfl(e, src_lane);
+                    int32_t hit = -1;
+                    for (int64_t k0 = kb; k0 < ke && hit < 0; k0 += 64) {
+                        const int64_t kk = k0 + lane;
+                        const int32_t u = kk < ke ? a.cols[kk] : -1;
+                        const bool h = u >= 0 && ((a.front[u >> 6] >> (u & 63)) & 1);
+                        const uint64_t hm = __ballot(h);
+                        const int seen = hm ? (__ffsll((unsigned long long)hm)) : (int)(ke - k0 < 64 ? ke - k0 : 64);
+                        if (lane == src_lane) exam += (unsigned long long)seen;
+                        if (hm) hit = __shfl(u, __ffsll((unsigned long long)hm) - 1);
+                    }
+                    if (lane == src_lane) par = hit;
+                }
+                const bool found = par >= 0;
+                if (found) {
+                    a.depth[v] = a.level + 1;
+                    a.parent[v] = par;
+                    const unsigned long long d = (unsigned long long)(e - b);
+                    ++cnt;
+                    dsum += d;
+                    dmax = d > dmax ? d : dmax;
+                    atomicOr((unsigned long long *)&s.b_found[wid][j],
+                             (unsigned long long)(1ULL << bpos));
+                }
+                emit_next(a, s, found, (int32_t)v, b, e);
+            }
+            wave_lds_sync();
+            if (own) {
+                const uint64_t f = s.b_found[wid][lane];
+                a.next_bm[wi] = f;
+                if (f) a.vis[wi] = vw | f;
             }
         }
-        if (lane == 0) {
-            a.next[w] = found;
-            if (found) a.vis[w] = vw | found;
-            a.clr[w] = 0;
-        }
+        __syncthreads();
+        if (s.qn >= kQCap / 2) flush_queue<BLOCK>(a, s);
     }
-    block_flush<BLOCK>(c, cnt, dsum, exam, dmax);
 }
+
+// --------------------------------------------------------- direction policy
 
 // hybrid_bfs.cc:162-190 as a per-step state machine.  After a TD step the
 // scout count is sum deg(N) (the -deg encoding of clear(), :93, :438); after
@@ -386,38 +496,14 @@ __device__ void beamer_next(ctl_block *c, int32_t last_dir, int64_t nnew, int64_
     }
 }
 
-struct decide_args {
-    ctl_block *c;
-    bd_level_rec *log;
-    const int32_t *script;
-    const float *mlp;
-    uint64_t *vis;
-    uint64_t *bm0;
-    int32_t *q0;
-    int32_t *depth;
-    int32_t *parent;
-    const int64_t *off;
-    int64_t source;      // >= 0: initialise a search from this vertex
-    int64_t nv, m, nnz;
-    int64_t alpha, beta;
-    int32_t level;
-    int32_t gate;
-    int32_t policy;
-    int32_t script_len;
-    int32_t log_cap;
-    int32_t mlp_h1, mlp_h2;
-    int32_t has_mlp;
-};
-
-// Choose the direction of step c->level.  Runs on one wave: lane 0 owns the
-// control block; the learned policy evaluates its network on the wave.
-__device__ void choose_dir(const decide_args &a, int32_t last_dir, int64_t nnew,
+// Choose the direction of step c->level on one wave (no workgroup barrier):
+// lane 0 owns the control block; the learned policy evaluates its network on
+// the wave.
+__device__ void choose_dir(const step_args &a, step_smem &s, int32_t last_dir, int64_t nnew,
                            int64_t nnew_deg) {
     ctl_block *c = a.c;
-    __shared__ float s_x[8];
-    __shared__ int s_use_mlp;
-    if (threadIdx.x == 0) {
-        s_use_mlp = 0;
+    if (lane_id() == 0) {
+        s.use_mlp = 0;
         switch (a.policy) {
         case BD_POLICY_TD_ONLY:
             c->dir = BD_DIR_TD;
@@ -426,83 +512,42 @@ __device__ void choose_dir(const decide_args &a, int32_t last_dir, int64_t nnew,
             c->dir = BD_DIR_BU;
             break;
         case BD_POLICY_SCRIPT: {
-            int32_t i = c->level < a.script_len ? c->level : a.script_len - 1;
+            const int32_t i = c->level < a.script_len ? c->level : a.script_len - 1;
             c->dir = (a.script_len > 0 && a.script[i] == BD_DIR_BU) ? BD_DIR_BU : BD_DIR_TD;
             break;
         }
         case BD_POLICY_LEARNED:
             if (a.has_mlp) {
-                s_use_mlp = 1;
-                mlp_features_dev(c->frontier, c->frontier_deg, c->max_frontier_deg,
-                                 c->unvisited, c->unvisited_deg, c->level, c->nv, a.m,
-                                 last_dir, s_x);
+                s.use_mlp = 1;
+                mlp_features_dev(c->frontier, c->frontier_deg, c->max_frontier_deg, c->unvisited,
+                                 c->unvisited_deg, c->level, c->nv, a.m, last_dir, s.x);
                 break;
             }
-            // no network loaded: Beamer is the fallback
-            beamer_next(c, last_dir, nnew, nnew_deg);
+            beamer_next(c, last_dir, nnew, nnew_deg);  // no network: Beamer
             break;
         default:
             beamer_next(c, last_dir, nnew, nnew_deg);
         }
     }
-    __syncthreads();
-    if (s_use_mlp) {
+    wave_lds_sync();
+    if (s.use_mlp) {
         float y[2];
-        mlp_forward_wave(a.mlp, a.mlp_h1, a.mlp_h2, s_x, y);
-        if (threadIdx.x == 0) {
-            // predicted log-costs; NaN falls back to top-down
-            c->dir = (y[1] < y[0]) ? BD_DIR_BU : BD_DIR_TD;
-        }
+        mlp_forward_wave(a.mlp, a.mlp_h1, a.mlp_h2, s.x, y, &s.mlp);
+        // predicted log-costs; a NaN prediction falls back to top-down
+        if (lane_id() == 0) c->dir = (y[1] < y[0]) ? BD_DIR_BU : BD_DIR_TD;
     }
 }
 
-__global__ __launch_bounds__(64) void k_decide(decide_args a) {
+// Close step a.level: record, totals, reset accumulators, next direction.
+// Runs on one wave.  Accumulators are read with device-scope atomic loads
+// because, in the fused path, other workgroups updated them in this kernel.
+__device__ void close_step(const step_args &a, step_smem &s) {
     ctl_block *c = a.c;
-    if (a.source >= 0) {
-        // clear() (hybrid_bfs.cc:433-443) is done by k_clear; here the source
-        // is pushed and marked (hybrid_bfs.cc:157-163).
-        const int64_t s = a.source;
-        if (threadIdx.x == 0) {
-            const int64_t dsrc = a.off[s + 1] - a.off[s];
-            memset(c, 0, sizeof(ctl_block));
-            c->nv = a.nv;
-            c->m2 = 2 * a.m;
-            c->alpha = a.alpha;
-            c->beta = a.beta;
-            c->policy = a.policy;
-            c->edges_to_check = 2 * a.m;
-            c->scout = dsrc;
-            c->prev_dir = -1;
-            c->frontier = 1;
-            c->frontier_deg = dsrc;
-            c->max_frontier_deg = dsrc;
-            c->unvisited = a.nv - 1;
-            c->unvisited_deg = a.nnz - dsrc;
-            c->qlen[0] = 1;
-            c->reached = 1;
-            c->reached_deg = dsrc;
-            a.vis[s >> 6] |= 1ULL << (s & 63);
-            a.bm0[s >> 6] = 1ULL << (s & 63);
-            a.q0[0] = (int32_t)s;
-            a.depth[s] = 0;
-            a.parent[s] = (int32_t)s;
-        }
-        __syncthreads();
-        // level 0: with no previous step, Beamer applies only the outer test
-        // to scout = deg(source) (hybrid_bfs.cc:163-167)
-        choose_dir(a, -1, 0, 0);
-        if (threadIdx.x == 0) {
-            __threadfence();
-            c->t_start = wall_clock64();
-        }
-        return;
-    }
-    if (a.gate && (c->done || c->level != a.level)) return;
     __shared__ long long s_new, s_new_deg;
     __shared__ int s_dir;
-    if (threadIdx.x == 0) {
-        const int64_t nnew = (int64_t)c->acc_new;
-        const int64_t nnew_deg = (int64_t)c->acc_new_deg;
+    if (lane_id() == 0) {
+        const int64_t nnew = (int64_t)ld_acquire(&c->acc_new);
+        const int64_t nnew_deg = (int64_t)ld_acquire(&c->acc_new_deg);
         const int32_t dir = c->dir;
         const int32_t L = c->level;
         bd_level_rec r;
@@ -516,10 +561,11 @@ __global__ __launch_bounds__(64) void k_decide(decide_args a) {
         r.result = dir == BD_DIR_TD ? nnew_deg : nnew;
         r.next = nnew;
         r.next_deg = nnew_deg;
-        r.edges_examined = dir == BD_DIR_TD ? c->frontier_deg : (int64_t)c->acc_examined;
+        r.edges_examined =
+            dir == BD_DIR_TD ? c->frontier_deg : (int64_t)ld_acquire(&c->acc_examined);
         r.max_frontier_deg = c->max_frontier_deg;
-        r.td_bitmap_input = (dir == BD_DIR_TD && c->td_bitmap) ? 1 : 0;
-        r.reserved = (int64_t)c->task_overflow;
+        r.tasks = (int64_t)c->ntasks[L & 1];
+        r.queued = (int64_t)c->qlen[L & 1];
         r.ms = -1.0;
         r.ms_other = -1.0;
         if (L < a.log_cap) a.log[L] = r;
@@ -534,14 +580,13 @@ __global__ __launch_bounds__(64) void k_decide(decide_args a) {
         c->unvisited_deg -= nnew_deg;
         c->frontier = nnew;
         c->frontier_deg = nnew_deg;
-        c->max_frontier_deg = (int64_t)c->acc_max_deg;
+        c->max_frontier_deg = (int64_t)ld_acquire(&c->acc_max_deg);
         c->acc_new = 0;
         c->acc_new_deg = 0;
         c->acc_examined = 0;
         c->acc_max_deg = 0;
-        c->ntasks = 0;
+        c->ntasks[L & 1] = 0;
         c->qlen[L & 1] = 0;
-        c->td_bitmap = dir == BD_DIR_BU ? 1 : 0;
         c->prev_dir = dir;
         c->level = L + 1;
         if (nnew == 0) c->done = 1;
@@ -549,12 +594,137 @@ __global__ __launch_bounds__(64) void k_decide(decide_args a) {
         s_new_deg = nnew_deg;
         s_dir = dir;
     }
+    wave_lds_sync();
+    if (s_new > 0) choose_dir(a, s, s_dir, s_new, s_new_deg);
+    if (lane_id() == 0) c->t_end = wall_clock64();
+}
+
+// ---------------------------------------------------------------- kernels
+
+template <int BLOCK>
+__device__ void block_flush_counters(ctl_block *c, step_smem &s, unsigned long long cnt,
+                                     unsigned long long dsum, unsigned long long exam,
+                                     unsigned long long dmax) {
+    const int lane = lane_id(), wid = threadIdx.x >> 6;
+    cnt = wave_sum(cnt);
+    dsum = wave_sum(dsum);
+    exam = wave_sum(exam);
+    dmax = wave_max(dmax);
+    if (lane == 0) {
+        s.red[0][wid] = cnt;
+        s.red[1][wid] = dsum;
+        s.red[2][wid] = exam;
+        s.red[3][wid] = dmax;
+    }
     __syncthreads();
-    if (s_new > 0) choose_dir(a, s_dir, s_new, s_new_deg);
+    if (threadIdx.x == 0) {
+        for (int w = 1; w < BLOCK / 64; ++w) {
+            cnt += s.red[0][w];
+            dsum += s.red[1][w];
+            exam += s.red[2][w];
+            dmax = s.red[3][w] > dmax ? s.red[3][w] : dmax;
+        }
+        if (cnt) atomicAdd(&c->acc_new, cnt);
+        if (dsum) atomicAdd(&c->acc_new_deg, dsum);
+        if (exam) atomicAdd(&c->acc_examined, exam);
+        if (dmax) atomicMax(&c->acc_max_deg, dmax);
+    }
+}
+
+template <int BLOCK>
+__global__ __launch_bounds__(BLOCK) void k_step(step_args a) {
+    ctl_block *c = a.c;
+    if (a.gate && (c->done || c->level != a.level)) return;
+    const int dir = a.force_dir >= 0 ? a.force_dir : c->dir;
+    __shared__ step_smem s;
+    if (threadIdx.x == 0) s.qn = 0;
+    __syncthreads();
+    unsigned long long cnt = 0, dsum = 0, exam = 0, dmax = 0;
+    if (dir == BD_DIR_TD) {
+        for (int64_t i = (int64_t)blockIdx.x * BLOCK + threadIdx.x; i < a.nwords;
+             i += (int64_t)gridDim.x * BLOCK)
+            a.clr[i] = 0;
+        td_body<BLOCK>(a, s, cnt, dsum, dmax);
+    } else {
+        bu_body<BLOCK>(a, s, cnt, dsum, dmax, exam);
+    }
+    flush_queue<BLOCK>(a, s);
+    block_flush_counters<BLOCK>(c, s, cnt, dsum, exam, dmax);
+    if (!a.fused_decide) return;
+    // The last workgroup to arrive closes the step.  The fence orders this
+    // workgroup's counter and queue atomics before its arrival.
+    __syncthreads();
     if (threadIdx.x == 0) {
         __threadfence();
-        c->t_end = wall_clock64();
+        const unsigned t = atomicAdd(&c->arrive, 1u);
+        s.last = (t == gridDim.x - 1);
     }
+    __syncthreads();
+    if (!s.last) return;
+    __threadfence();
+    if (threadIdx.x < 64) close_step(a, s);
+    if (threadIdx.x == 0) c->arrive = 0;
+}
+
+// Close a step run without the fused decision (host-driven and profiling
+// runs), or start a search when source >= 0: clear() (hybrid_bfs.cc:433-443)
+// has been done by k_clear; here the source is pushed and marked
+// (hybrid_bfs.cc:157-163) and the first direction chosen.
+__global__ __launch_bounds__(64) void k_decide(step_args a, int64_t source) {
+    ctl_block *c = a.c;
+    __shared__ step_smem s;
+    if (source >= 0) {
+        if (threadIdx.x == 0) {
+            const int64_t b = a.off[source], e = a.off[source + 1];
+            const int64_t dsrc = e - b;
+            memset(c, 0, sizeof(ctl_block));
+            c->nv = a.nv;
+            c->m2 = 2 * a.m;
+            c->alpha = a.alpha;
+            c->beta = a.beta;
+            c->policy = a.policy;
+            c->edges_to_check = 2 * a.m;
+            c->scout = dsrc;
+            c->prev_dir = -1;
+            c->frontier = 1;
+            c->frontier_deg = dsrc;
+            c->max_frontier_deg = dsrc;
+            c->unvisited = a.nv - 1;
+            c->unvisited_deg = a.nnz - dsrc;
+            c->reached = 1;
+            c->reached_deg = dsrc;
+            a.vis[source >> 6] |= 1ULL << (source & 63);
+            a.next_bm[source >> 6] = 1ULL << (source & 63);  // bm[0]
+            a.depth[source] = 0;
+            a.parent[source] = (int32_t)source;
+            if (dsrc >= a.task_t) {
+                const int64_t nt = (dsrc + a.task_edges - 1) / a.task_edges;
+                for (int64_t t = 0; t < nt && t < a.task_cap; ++t) {
+                    td_task tk;
+                    tk.beg = b + t * a.task_edges;
+                    tk.end = tk.beg + a.task_edges < e ? tk.beg + a.task_edges : e;
+                    tk.src = (int32_t)source;
+                    tk.pad = 0;
+                    a.tout[t] = tk;
+                }
+                c->ntasks[0] = nt;
+            } else {
+                a.qout[0] = (int32_t)source;
+                c->qlen[0] = 1;
+            }
+        }
+        wave_lds_sync();
+        // level 0: with no previous step Beamer applies only the outer test
+        // to scout = deg(source) (hybrid_bfs.cc:163-167)
+        choose_dir(a, s, -1, 0, 0);
+        if (threadIdx.x == 0) {
+            __threadfence();
+            c->t_start = wall_clock64();
+        }
+        return;
+    }
+    if (a.gate && (c->done || c->level != a.level)) return;
+    close_step(a, s);
 }
 
 // clear() (hybrid_bfs.cc:433-443) for the device layout.
@@ -575,18 +745,21 @@ __global__ void k_clear(int32_t *__restrict__ depth, int32_t *__restrict__ paren
     }
 }
 
-// Reset the per-step accumulators (profile runs time both directions).
-__global__ void k_reset_acc(ctl_block *c) {
+// Profiling: undo the first direction's step before timing the other one.
+__global__ void k_reset_step(ctl_block *c, int32_t level) {
     c->acc_new = 0;
     c->acc_new_deg = 0;
     c->acc_examined = 0;
     c->acc_max_deg = 0;
+    c->qlen[(level + 1) & 1] = 0;
+    c->ntasks[(level + 1) & 1] = 0;
 }
 
 // count_num_traversed_edges (hybrid_bfs.cc:401-414): sum of degrees over
 // reached vertices.
-__global__ void k_count_traversed(const int32_t *__restrict__ depth, const int64_t *__restrict__ off,
-                                  int64_t nv, unsigned long long *__restrict__ out) {
+__global__ void k_count_traversed(const int32_t *__restrict__ depth,
+                                  const int64_t *__restrict__ off, int64_t nv,
+                                  unsigned long long *__restrict__ out) {
     unsigned long long s = 0;
     for (int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v < nv;
          v += (int64_t)gridDim.x * blockDim.x)
@@ -608,112 +781,72 @@ __global__ void k_scrub(const uint4 *__restrict__ p, int64_t n, uint4 *__restric
     if (acc.x == 0x9E3779B9u && acc.y == 0x7F4A7C15u) sink[0] = acc;
 }
 
-// Launch kernel template K<block> for block in {128, 256, 512}.
-#define BD_LAUNCH_BLOCKED(K, block, grid, stream, arg)                                 \
-    do {                                                                               \
-        switch (block) {                                                               \
-        case 128:                                                                      \
-            hipLaunchKernelGGL(K<128>, dim3(grid), dim3(128), 0, stream, arg);         \
-            break;                                                                     \
-        case 512:                                                                      \
-            hipLaunchKernelGGL(K<512>, dim3(grid), dim3(512), 0, stream, arg);         \
-            break;                                                                     \
-        default:                                                                       \
-            hipLaunchKernelGGL(K<256>, dim3(grid), dim3(256), 0, stream, arg);         \
-        }                                                                              \
-    } while (0)
-
 int norm_block(int b) { return (b == 128 || b == 512) ? b : 256; }
 
-struct level_ptrs {
-    td_args td;
-    bu_args bu;
-};
-
-level_ptrs make_level(bd_ctx *c, int32_t level, int32_t gate) {
-    level_ptrs p;
-    td_args &t = p.td;
-    t.off = c->d_off;
-    t.cols = c->d_cols;
-    t.vis = c->d_vis;
-    t.qin = c->d_q[level & 1];
-    t.front = c->d_bm[level % 3];
-    t.next_bm = c->d_bm[(level + 1) % 3];
-    t.clr = c->d_bm[(level + 2) % 3];
-    t.qout = c->d_q[(level + 1) & 1];
-    t.depth = c->d_depth;
-    t.parent = c->d_parent;
-    t.tasks = c->d_tasks;
-    t.c = c->d_ctl;
-    t.nwords = c->nwords;
-    t.task_cap = c->task_cap;
-    t.level = level;
-    t.gate = gate;
-    t.wave_t = std::max(1, c->tune.wave_threshold);
-    t.task_t = std::max(64, c->tune.task_threshold);
-    t.task_edges = std::max(64, c->tune.task_edges);
-    t.pad = 0;
-    bu_args &b = p.bu;
-    b.off = c->d_off;
-    b.cols = c->d_cols;
-    b.vis = c->d_vis;
-    b.front = c->d_bm[level % 3];
-    b.next = c->d_bm[(level + 1) % 3];
-    b.clr = c->d_bm[(level + 2) % 3];
-    b.depth = c->d_depth;
-    b.parent = c->d_parent;
-    b.c = c->d_ctl;
-    b.nwords = c->nwords;
-    b.level = level;
-    b.gate = gate;
-    return p;
+step_args make_args(bd_ctx *c, int32_t level, int64_t alpha, int64_t beta, int policy) {
+    step_args a;
+    a.off = c->d_off;
+    a.cols = c->d_cols;
+    a.vis = c->d_vis;
+    a.front = c->d_bm[level % 3];
+    a.next_bm = c->d_bm[(level + 1) % 3];
+    a.clr = c->d_bm[(level + 2) % 3];
+    a.qin = c->d_q[level & 1];
+    a.qout = c->d_q[(level + 1) & 1];
+    a.tin = c->d_tasks[level & 1];
+    a.tout = c->d_tasks[(level + 1) & 1];
+    a.depth = c->d_depth;
+    a.parent = c->d_parent;
+    a.c = c->d_ctl;
+    a.log = c->d_log;
+    a.script = c->d_script;
+    a.mlp = c->d_mlp;
+    a.nwords = c->nwords;
+    a.task_cap = c->task_cap;
+    a.nv = c->nv;
+    a.m = c->m;
+    a.nnz = c->nnz;
+    a.alpha = alpha;
+    a.beta = beta;
+    a.level = level;
+    a.gate = 1;
+    a.force_dir = -1;
+    a.fused_decide = 1;
+    a.wave_t = std::max(1, c->tune.wave_threshold);
+    a.task_t = std::max(64, c->tune.task_threshold);
+    a.task_edges = std::max(64, c->tune.task_edges);
+    a.bu_words = c->tune.bu_words > 0 ? std::min(64, c->tune.bu_words) : 32;
+    a.bu_cap = c->tune.bu_cap > 0 ? c->tune.bu_cap : 32;
+    a.policy = policy;
+    a.script_len = (int32_t)c->script_len;
+    a.log_cap = (int32_t)c->log_cap;
+    a.mlp_h1 = c->mlp_h1;
+    a.mlp_h2 = c->mlp_h2;
+    a.has_mlp = c->has_mlp ? 1 : 0;
+    return a;
 }
 
-int launch_td(bd_ctx *c, const td_args &t) {
-    const int blk = norm_block(c->tune.td_block);
+int launch_step(bd_ctx *c, const step_args &a) {
+    const int blk = norm_block(c->tune.block);
     const int64_t grid = (int64_t)c->cu_count * std::max(1, c->tune.grid_per_cu);
-    BD_LAUNCH_BLOCKED(k_top_down, blk, grid, c->stream, t);
-    BD_LAUNCH_BLOCKED(k_top_down_tasks, blk, grid, c->stream, t);
+    switch (blk) {
+    case 128:
+        hipLaunchKernelGGL(k_step<128>, dim3(grid), dim3(128), 0, c->stream, a);
+        break;
+    case 512:
+        hipLaunchKernelGGL(k_step<512>, dim3(grid), dim3(512), 0, c->stream, a);
+        break;
+    default:
+        hipLaunchKernelGGL(k_step<256>, dim3(grid), dim3(256), 0, c->stream, a);
+    }
     BD_HIP(c, hipGetLastError());
     return BD_OK;
 }
 
-int launch_bu(bd_ctx *c, const bu_args &b) {
-    const int blk = norm_block(c->tune.bu_block);
-    const int64_t grid = grid_for(c, c->nwords * 64, blk);
-    BD_LAUNCH_BLOCKED(k_bottom_up, blk, grid, c->stream, b);
+int launch_decide(bd_ctx *c, const step_args &a, int64_t source) {
+    hipLaunchKernelGGL(k_decide, dim3(1), dim3(64), 0, c->stream, a, source);
     BD_HIP(c, hipGetLastError());
     return BD_OK;
-}
-
-decide_args make_decide(bd_ctx *c, int32_t level, int64_t source, int64_t alpha, int64_t beta,
-                        int policy) {
-    decide_args d;
-    d.c = c->d_ctl;
-    d.log = c->d_log;
-    d.script = c->d_script;
-    d.mlp = c->d_mlp;
-    d.vis = c->d_vis;
-    d.bm0 = c->d_bm[0];
-    d.q0 = c->d_q[0];
-    d.depth = c->d_depth;
-    d.parent = c->d_parent;
-    d.off = c->d_off;
-    d.source = source;
-    d.nv = c->nv;
-    d.m = c->m;
-    d.nnz = c->nnz;
-    d.alpha = alpha;
-    d.beta = beta;
-    d.level = level;
-    d.gate = 1;
-    d.policy = policy;
-    d.script_len = (int32_t)c->script_len;
-    d.log_cap = (int32_t)c->log_cap;
-    d.mlp_h1 = c->mlp_h1;
-    d.mlp_h2 = c->mlp_h2;
-    d.has_mlp = c->has_mlp ? 1 : 0;
-    return d;
 }
 
 int check_run_args(bd_ctx *c, int64_t source, int64_t alpha, int64_t beta, int policy) {
@@ -733,10 +866,13 @@ int start_search(bd_ctx *c, int64_t source, int64_t alpha, int64_t beta, int pol
     hipLaunchKernelGGL(k_clear, dim3(grid_for(c, n, 256)), dim3(256), 0, c->stream, c->d_depth,
                        c->d_parent, c->nv, c->d_vis, c->d_dead, c->d_bm[0], c->d_bm[1],
                        c->d_bm[2], c->nwords);
-    decide_args d = make_decide(c, 0, source, alpha, beta, policy);
-    hipLaunchKernelGGL(k_decide, dim3(1), dim3(64), 0, c->stream, d);
     BD_HIP(c, hipGetLastError());
-    return BD_OK;
+    // level "-1" pointers: next_bm = bm[0], qout = q[0], tout = tasks[0]
+    step_args a = make_args(c, 0, alpha, beta, policy);
+    a.next_bm = c->d_bm[0];
+    a.qout = c->d_q[0];
+    a.tout = c->d_tasks[0];
+    return launch_decide(c, a, source);
 }
 
 int read_ctl(bd_ctx *c) {
@@ -748,7 +884,7 @@ int read_ctl(bd_ctx *c) {
 
 int finish_search(bd_ctx *c, int64_t source, bd_run_stats *out, double event_ms) {
     const ctl_block &h = *c->h_ctl;
-    if (h.task_overflow) return set_err(c, BD_EHIP, "top-down task list overflow");
+    if (h.task_overflow) return set_err(c, BD_EHIP, "task list overflow");
     c->last_steps = h.steps;
     c->last_source = source;
     c->have_result = true;
@@ -786,17 +922,12 @@ int run_search(bd_ctx *c, int64_t source, int64_t alpha, int64_t beta, int polic
     if (c->tune.level_driver == 2 || timed) {
         if ((rc = read_ctl(c))) return rc;
         while (!c->h_ctl->done) {
-            level_ptrs p = make_level(c, L, 1);
+            step_args a = make_args(c, L, alpha, beta, policy);
+            a.fused_decide = 0;
             if (timed) BD_HIP(c, hipEventRecord(c->ev[2], c->stream));
-            if (c->h_ctl->dir == BD_DIR_TD)
-                rc = launch_td(c, p.td);
-            else
-                rc = launch_bu(c, p.bu);
-            if (rc) return rc;
+            if ((rc = launch_step(c, a))) return rc;
             if (timed) BD_HIP(c, hipEventRecord(c->ev[3], c->stream));
-            hipLaunchKernelGGL(k_decide, dim3(1), dim3(64), 0, c->stream,
-                               make_decide(c, L, -1, alpha, beta, policy));
-            BD_HIP(c, hipGetLastError());
+            if ((rc = launch_decide(c, a, -1))) return rc;
             if ((rc = read_ctl(c))) return rc;
             if (timed) {
                 float ms = 0;
@@ -808,14 +939,8 @@ int run_search(bd_ctx *c, int64_t source, int64_t alpha, int64_t beta, int polic
     } else {
         int chunk = 8;
         for (;;) {
-            for (int k = 0; k < chunk; ++k, ++L) {
-                level_ptrs p = make_level(c, L, 1);
-                if ((rc = launch_td(c, p.td))) return rc;
-                if ((rc = launch_bu(c, p.bu))) return rc;
-                hipLaunchKernelGGL(k_decide, dim3(1), dim3(64), 0, c->stream,
-                                   make_decide(c, L, -1, alpha, beta, policy));
-            }
-            BD_HIP(c, hipGetLastError());
+            for (int k = 0; k < chunk; ++k, ++L)
+                if ((rc = launch_step(c, make_args(c, L, alpha, beta, policy)))) return rc;
             if ((rc = read_ctl(c))) return rc;
             if (c->h_ctl->done) break;
             if (L > max_levels) return set_err(c, BD_EHIP, "search did not terminate");
@@ -956,16 +1081,16 @@ extern "C" int bd_bfs_check(bd_ctx *c, int64_t source, int *ok) {
 }
 
 // Time both directions at every step from the same state; the search follows
-// `policy`.  The direction not taken runs first, then the state it changed is
-// restored (vis) and the chosen direction runs, so the search ends exactly as
-// a normal run would.  Both steps must find the same number of vertices.
+// `policy`.  The direction not taken runs first; then the state it changed
+// (vis, next-frontier counts) is restored and the chosen direction runs, so
+// the search ends exactly as a normal run would.  Both directions must find
+// the same number of vertices.
 extern "C" int bd_bfs_profile(bd_ctx *c, int64_t source, int64_t alpha, int64_t beta, int policy,
                               bd_run_stats *out) {
     if (!c) return BD_EINVAL;
     int rc;
     if ((rc = check_run_args(c, source, alpha, beta, policy))) return rc;
     BD_HIP(c, hipSetDevice(c->device));
-    // scrub buffer: 512 MiB, larger than the last-level cache
     const int64_t scrub_n = ((int64_t)512 << 20) / 16;
     uint4 *scrub = nullptr;
     if ((rc = dev_alloc(c, (void **)&scrub, sizeof(uint4) * (scrub_n + 1), "scrub"))) return rc;
@@ -974,10 +1099,6 @@ extern "C" int bd_bfs_profile(bd_ctx *c, int64_t source, int64_t alpha, int64_t 
         ~guard() { dev_free(p); }
     } g{scrub};
     BD_HIP(c, hipMemsetAsync(scrub, 0, sizeof(uint4) * (scrub_n + 1), c->stream));
-    auto do_scrub = [&]() {
-        hipLaunchKernelGGL(k_scrub, dim3(c->cu_count * 4), dim3(256), 0, c->stream, scrub,
-                           scrub_n, scrub + scrub_n);
-    };
     if ((rc = start_search(c, source, alpha, beta, policy))) return rc;
     if ((rc = read_ctl(c))) return rc;
     std::vector<std::pair<double, double>> times;
@@ -986,23 +1107,25 @@ extern "C" int bd_bfs_profile(bd_ctx *c, int64_t source, int64_t alpha, int64_t 
     while (!c->h_ctl->done) {
         const int32_t chosen = c->h_ctl->dir;
         const int32_t other = 1 - chosen;
-        level_ptrs p = make_level(c, L, 0);
         float ms[2] = {0, 0};
         unsigned long long found[2] = {0, 0};
         for (int pass = 0; pass < 2; ++pass) {
-            const int32_t d = pass == 0 ? other : chosen;
+            step_args a = make_args(c, L, alpha, beta, policy);
+            a.fused_decide = 0;
+            a.gate = 0;
+            a.force_dir = pass == 0 ? other : chosen;
             if (pass == 1) {
                 BD_HIP(c, hipMemcpyAsync(c->d_vis, c->d_vis_save, sizeof(uint64_t) * c->nwords,
                                          hipMemcpyDeviceToDevice, c->stream));
-                hipLaunchKernelGGL(k_reset_acc, dim3(1), dim3(1), 0, c->stream, c->d_ctl);
+                hipLaunchKernelGGL(k_reset_step, dim3(1), dim3(1), 0, c->stream, c->d_ctl, L);
             } else {
                 BD_HIP(c, hipMemcpyAsync(c->d_vis_save, c->d_vis, sizeof(uint64_t) * c->nwords,
                                          hipMemcpyDeviceToDevice, c->stream));
             }
-            do_scrub();
+            hipLaunchKernelGGL(k_scrub, dim3(c->cu_count * 4), dim3(256), 0, c->stream, scrub,
+                               scrub_n, scrub + scrub_n);
             BD_HIP(c, hipEventRecord(c->ev[2], c->stream));
-            rc = d == BD_DIR_TD ? launch_td(c, p.td) : launch_bu(c, p.bu);
-            if (rc) return rc;
+            if ((rc = launch_step(c, a))) return rc;
             BD_HIP(c, hipEventRecord(c->ev[3], c->stream));
             BD_HIP(c, hipEventSynchronize(c->ev[3]));
             BD_HIP(c, hipEventElapsedTime(&ms[pass], c->ev[2], c->ev[3]));
@@ -1013,9 +1136,8 @@ extern "C" int bd_bfs_profile(bd_ctx *c, int64_t source, int64_t alpha, int64_t 
             return set_err(c, BD_EHIP, "profile: directions disagree on the next frontier");
         times.push_back({ms[1], ms[0]});
         total += ms[1];
-        hipLaunchKernelGGL(k_decide, dim3(1), dim3(64), 0, c->stream,
-                           make_decide(c, L, -1, alpha, beta, policy));
-        BD_HIP(c, hipGetLastError());
+        step_args a = make_args(c, L, alpha, beta, policy);
+        if ((rc = launch_decide(c, a, -1))) return rc;
         if ((rc = read_ctl(c))) return rc;
         if (++L > c->nv + 2) return set_err(c, BD_EHIP, "search did not terminate");
     }

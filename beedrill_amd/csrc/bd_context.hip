@@ -43,14 +43,14 @@ static void free_search(bd_ctx *c) {
     dev_free(c->d_vis_save);
     for (auto &b : c->d_bm) dev_free(b);
     for (auto &q : c->d_q) dev_free(q);
-    dev_free(c->d_tasks);
+    for (auto &t : c->d_tasks) dev_free(t);
     for (auto &b : c->d_ms) dev_free(b);
     dev_free(c->d_ms_depth);
     c->d_depth = c->d_parent = nullptr;
     c->d_vis = c->d_vis_save = nullptr;
     for (auto &b : c->d_bm) b = nullptr;
     for (auto &q : c->d_q) q = nullptr;
-    c->d_tasks = nullptr;
+    for (auto &t : c->d_tasks) t = nullptr;
     for (auto &b : c->d_ms) b = nullptr;
     c->d_ms_depth = nullptr;
     c->ms_depth_rows = 0;
@@ -97,8 +97,8 @@ int alloc_search(bd_ctx *c) {
     const int64_t te = std::max(1, c->tune.task_edges);
     const int64_t tt = std::max(1, c->tune.task_threshold);
     c->task_cap = c->nnz / te + c->nnz / tt + 16;
-    if ((rc = dev_alloc(c, (void **)&c->d_tasks, sizeof(td_task) * c->task_cap, "tasks")))
-        return rc;
+    for (auto &t : c->d_tasks)
+        if ((rc = dev_alloc(c, (void **)&t, sizeof(td_task) * c->task_cap, "tasks"))) return rc;
     return BD_OK;
 }
 
@@ -123,9 +123,10 @@ extern "C" int bd_create(bd_ctx **out, int device) {
     if (device < 0 || device >= n) return BD_EINVAL;
     bd_ctx *c = new bd_ctx();
     c->device = device;
-    c->tune.bu_block = 256;
-    c->tune.td_block = 256;
+    c->tune.block = 256;
     c->tune.grid_per_cu = 8;
+    c->tune.bu_words = 32;
+    c->tune.bu_cap = 32;
     c->tune.wave_threshold = 32;
     c->tune.task_threshold = 4096;
     c->tune.task_edges = 1024;
@@ -220,14 +221,13 @@ extern "C" int bd_pick_roots(bd_ctx *c, int64_t n, int64_t *roots) {
 extern "C" int bd_set_tuning(bd_ctx *c, const bd_tuning *t) {
     if (!c || !t) return BD_EINVAL;
     bd_tuning n = c->tune;
-    if (t->bu_block > 0) {
-        if (t->bu_block % 64 || t->bu_block > 1024) return set_err(c, BD_EINVAL, "bu_block");
-        n.bu_block = t->bu_block;
+    if (t->block > 0) {
+        if (t->block != 128 && t->block != 256 && t->block != 512)
+            return set_err(c, BD_EINVAL, "block must be 128, 256 or 512");
+        n.block = t->block;
     }
-    if (t->td_block > 0) {
-        if (t->td_block % 64 || t->td_block > 1024) return set_err(c, BD_EINVAL, "td_block");
-        n.td_block = t->td_block;
-    }
+    if (t->bu_words > 0) n.bu_words = std::min(t->bu_words, 64);
+    if (t->bu_cap > 0) n.bu_cap = t->bu_cap;
     if (t->grid_per_cu > 0) n.grid_per_cu = std::min(t->grid_per_cu, 64);
     if (t->wave_threshold > 0) n.wave_threshold = t->wave_threshold;
     if (t->task_threshold > 0) n.task_threshold = std::max(t->task_threshold, 64);

@@ -23,9 +23,12 @@ struct td_task {
     int32_t pad;
 };
 
-// Device-resident control block.  The step kernels accumulate into acc_*;
-// the decide kernel closes a step, writes its record and picks the next
-// direction, so the level loop can run without the host.
+// Device-resident control block.  The step kernel accumulates into acc_*;
+// the last workgroup of the step (or the decide kernel) closes the step,
+// writes its record and picks the next direction, so the level loop runs
+// without the host.  qlen[L&1]/ntasks[L&1] describe frontier L (queue of
+// ordinary vertices and edge slices of high-degree vertices); step L fills
+// the other half.
 struct ctl_block {
     // run constants
     int64_t nv;
@@ -45,7 +48,7 @@ struct ctl_block {
     int32_t level;
     int32_t dir;
     int32_t done;
-    int32_t td_bitmap;   // TD reads F from the bitmap (previous step was BU)
+    uint32_t arrive;     // workgroups finished in the running step
     int32_t prev_dir;    // -1 before the first step
     int64_t frontier;
     int64_t frontier_deg;
@@ -58,7 +61,7 @@ struct ctl_block {
     unsigned long long acc_examined;
     unsigned long long acc_max_deg;
     unsigned long long qlen[2];
-    unsigned long long ntasks;
+    unsigned long long ntasks[2];
     unsigned long long task_overflow;
     // totals
     int64_t steps;
@@ -97,7 +100,7 @@ struct bd_ctx {
     uint64_t *d_vis_save = nullptr;
     uint64_t *d_bm[3] = {nullptr, nullptr, nullptr};
     int32_t *d_q[2] = {nullptr, nullptr};
-    bd::td_task *d_tasks = nullptr;
+    bd::td_task *d_tasks[2] = {nullptr, nullptr};
     int64_t task_cap = 0;
     bd::ctl_block *d_ctl = nullptr;
     bd::ctl_block *h_ctl = nullptr;  // pinned mirror

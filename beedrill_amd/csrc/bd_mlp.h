@@ -41,6 +41,15 @@ __host__ __device__ inline void mlp_features_dev(int64_t frontier, int64_t front
     x[7] = prev_dir == 1 ? 1.0f : 0.0f;
 }
 
+// Orders LDS accesses among the lanes of one wave.  The network code below
+// runs on a single wave, so it never needs a workgroup barrier and may be
+// called by one wave of a larger workgroup.
+__device__ __forceinline__ void wave_lds_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+}
+
 typedef short bf16x4 __attribute__((ext_vector_type(4)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
@@ -93,29 +102,33 @@ __device__ __forceinline__ void mlp_forward_rows_mfma(const float *p, int h1, in
     const float *w1 = p, *b1 = w1 + h1 * kMlpIn, *w2 = b1 + h1, *b2 = w2 + h2 * h1;
     const float *w3 = b2 + h2, *b3 = w3 + kMlpOut * h2;
     mlp_layer_mfma(w1, b1, kMlpIn, h1, X, H, true);
-    __syncthreads();
+    wave_lds_sync();
     mlp_layer_mfma(w2, b2, h1, h2, H, X, true);
-    __syncthreads();
+    wave_lds_sync();
     mlp_layer_mfma(w3, b3, h2, kMlpOut, X, Y, false);
-    __syncthreads();
+    wave_lds_sync();
 }
 
-// Single-sample forward on one wave (used by the decide kernel).  x is in
-// LDS; every lane returns y.
+// LDS scratch of one single-wave forward pass.
+struct mlp_scratch {
+    float X[16][32];
+    float H[16][32];
+    float Y[16][32];
+};
+
+// Single-sample forward on one wave (used when a step picks its direction).
+// x is in LDS; every lane returns y.
 __device__ inline void mlp_forward_wave(const float *p, int h1, int h2, const float *x,
-                                        float *y) {
-    __shared__ float X[16][32];
-    __shared__ float H[16][32];
-    __shared__ float Y[16][32];
+                                        float *y, mlp_scratch *s) {
     const int l = threadIdx.x & 63;
     for (int i = l; i < 16 * 32; i += 64) {
         const int r = i / 32, cidx = i % 32;
-        X[r][cidx] = (r == 0 && cidx < kMlpIn) ? x[cidx] : 0.f;
+        s->X[r][cidx] = (r == 0 && cidx < kMlpIn) ? x[cidx] : 0.f;
     }
-    __syncthreads();
-    mlp_forward_rows_mfma(p, h1, h2, X, H, Y);
-    y[0] = Y[0][0];
-    y[1] = Y[0][1];
+    wave_lds_sync();
+    mlp_forward_rows_mfma(p, h1, h2, s->X, s->H, s->Y);
+    y[0] = s->Y[0][0];
+    y[1] = s->Y[0][1];
 }
 
 }  // namespace bd

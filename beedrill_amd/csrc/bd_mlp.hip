@@ -82,13 +82,13 @@ __global__ __launch_bounds__(64) void k_mlp_infer_mfma(const float *__restrict__
             const int64_t s = base + r;
             X[r][cidx] = (s < n && cidx < kMlpIn) ? x[s * kMlpIn + cidx] : 0.f;
         }
-        __syncthreads();
+        wave_lds_sync();
         mlp_forward_rows_mfma(p, h1, h2, X, H, Y);
         if (l < 32) {
             const int r = l >> 1, o = l & 1;
             if (base + r < n) y[(base + r) * 2 + o] = Y[r][o];
         }
-        __syncthreads();
+        wave_lds_sync();
     }
 }
 

@@ -73,8 +73,8 @@ typedef struct {
     int64_t next_deg;         /* sum deg(N)                                */
     int64_t edges_examined;   /* TD: sum deg(F); BU: neighbour ids read    */
     int64_t max_frontier_deg; /* max deg over F                            */
-    int64_t td_bitmap_input;  /* 1 when a TD step read F from the bitmap   */
-    int64_t reserved;
+    int64_t tasks;            /* edge slices of F's high-degree vertices   */
+    int64_t queued;           /* F's other vertices, read from the queue   */
     double ms;                /* device time of the step (timed runs only) */
     double ms_other;          /* profile runs: time of the other direction */
 } bd_level_rec;
@@ -178,17 +178,24 @@ int bd_bfs_profile(bd_ctx *ctx, int64_t source, int64_t alpha, int64_t beta, int
 
 /* Launch tuning (autotuner output).  Any field <= 0 keeps the default. */
 typedef struct {
-    int32_t bu_block;        /* threads per bottom-up workgroup             */
-    int32_t td_block;        /* threads per top-down workgroup              */
-    int32_t grid_per_cu;     /* workgroups per CU for grid-stride kernels   */
-    int32_t wave_threshold;  /* degree at which a vertex gets a whole wave   */
-    int32_t task_threshold;  /* degree at which a vertex is split into tasks */
-    int32_t task_edges;      /* edges per task                             */
+    int32_t block;           /* threads per workgroup of the step kernel:
+                                128, 256 or 512                           */
+    int32_t grid_per_cu;     /* workgroups per CU (the grid is persistent) */
+    int32_t wave_threshold;  /* top-down: degree at which a vertex gets a
+                                whole wave                                */
+    int32_t task_threshold;  /* degree at which a frontier vertex is split
+                                into task slices                          */
+    int32_t task_edges;      /* edges per task slice                       */
+    int32_t bu_words;        /* bottom-up: visited words per wave chunk
+                                (1..64)                                   */
+    int32_t bu_cap;          /* bottom-up: neighbours a lane scans before
+                                its wave finishes the list together       */
     int32_t level_driver;    /* 1 = direction chosen on device and steps
                                 enqueued without a per-step host sync;
                                 2 = host reads the counters after each step */
     int32_t time_steps;      /* 1 = time each step's kernels with HIP events
                                 (implies the host-driven loop), 2 = off */
+    int32_t reserved;
 } bd_tuning;
 int bd_set_tuning(bd_ctx *ctx, const bd_tuning *t);
 int bd_get_tuning(bd_ctx *ctx, bd_tuning *t);

@@ -12,7 +12,8 @@ from conftest import complete_graph, path_graph, star_graph, two_components
 pytestmark = pytest.mark.gpu
 
 LEVEL_KEYS = ("level", "dir", "frontier", "frontier_deg", "unvisited", "unvisited_deg",
-              "edges_to_check", "result", "next", "next_deg", "max_frontier_deg")
+              "edges_to_check", "result", "next", "next_deg", "max_frontier_deg",
+              "edges_examined")
 
 
 def oracle_csr(orc, nv, s, d):
@@ -156,8 +157,11 @@ def test_s16_host_driven_levels_identical(ctx, orc, s16):
     try:
         ctx.set_tuning(level_driver=2)
         run_and_compare(ctx, orc, off, adj, m, root)
-        ctx.set_tuning(level_driver=1, bu_block=128, td_block=512, wave_threshold=8,
-                       task_threshold=256, task_edges=128)
+        ctx.set_tuning(level_driver=1, block=512, wave_threshold=8, task_threshold=256,
+                       task_edges=128, bu_words=8, bu_cap=4)
+        run_and_compare(ctx, orc, off, adj, m, root)
+        ctx.set_tuning(block=128, wave_threshold=1000, task_threshold=64, task_edges=64,
+                       bu_words=64, bu_cap=1, grid_per_cu=1)
         run_and_compare(ctx, orc, off, adj, m, root)
     finally:
         ctx.set_tuning(**t0)
