@@ -7,13 +7,14 @@
 //   bm[3][W]             bitmaps   bm[L%3] = depth-L set (bottom-up input),
 //                                  bm[(L+1)%3] = set found by step L,
 //                                  bm[(L+2)%3] zeroed during step L
-//   q[2][V]              int32     frontier vertices of degree < task_t
-//   tasks[2][..]                   edge slices of frontier vertices of degree
+//   q[2]                 int32     frontier vertices of degree < task_t
+//   tasks[2]                       edge slices of frontier vertices of degree
 //                                  >= task_t (the reference's 64-edge
 //                                  worklist granules, worklist.h:96-117)
-//   ctl                            counters and direction-policy state
+//   seg                            segment lengths and per-slot counters
+//   ctl                            step state and direction-policy state
 //
-// One BFS level is one launch of k_step.  Its direction is read from ctl.
+// One BFS level is one launch of k_step followed by a one-wave k_decide.
 //   Top-down (hybrid_bfs.cc:71-100): blocks take tiles of the queue; per
 //   tile, vertices of degree >= wave_t get one wave each, the rest are
 //   concatenated by a block-wide scan and every thread takes one edge.
@@ -26,10 +27,10 @@
 //   Reading the depth-L bitmap, not vis, keeps the reference's two-phase rule
 //   (:117-141): a vertex found in this step is never a parent in this step.
 // Both directions emit the next frontier the same way: queue entries staged
-// in LDS (one atomic per block flush) and task slices for high-degree
-// vertices, plus the depth-(L+1) bitmap.  The last workgroup to finish
-// closes the step (record, counters) and picks the next direction, so a
-// whole search is a stream of k_step launches with no host round trip.
+// in LDS and flushed once per block into the block's segment, task slices
+// for high-degree vertices, and the depth-(L+1) bitmap.  k_decide closes the
+// step (record, counters, segment prefix sums) and picks the next direction
+// on the device, so a search is a stream of launches with no host round trip.
 #include <math.h>
 #include <stdio.h>
 #include <string.h>
@@ -46,6 +47,7 @@ namespace {
 
 constexpr int kMaxLevelsPerSync = 64;
 constexpr int kQCap = 2048;  // LDS-staged queue entries per workgroup
+constexpr int kMaxBlock = 512;
 
 struct step_args {
     const int64_t *off;
@@ -61,17 +63,18 @@ struct step_args {
     int32_t *depth;
     int32_t *parent;
     ctl_block *c;
+    seg_counters *seg;
     bd_level_rec *log;
     const int32_t *script;
     const float *mlp;
     int64_t nwords;
     int64_t task_cap;
+    int64_t qseg_cap, tseg_cap;
     int64_t nv, m, nnz;
     int64_t alpha, beta;
     int32_t level;
     int32_t gate;          // skip when the search is done or at another level
     int32_t force_dir;     // -1: ctl->dir; 0/1: run this direction (profiling)
-    int32_t fused_decide;  // the last workgroup closes the step
     int32_t wave_t, task_t, task_edges;
     int32_t bu_words, bu_cap;
     int32_t policy, script_len, log_cap;
@@ -95,11 +98,17 @@ __device__ __forceinline__ unsigned long long wave_max(unsigned long long x) {
     return x;
 }
 
-__device__ __forceinline__ int lane_id() { return threadIdx.x & 63; }
-
-__device__ __forceinline__ unsigned long long ld_acquire(const unsigned long long *p) {
-    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+__device__ __forceinline__ long long wave_incl_scan(long long x) {
+    const int lane = threadIdx.x & 63;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const long long y = __shfl_up(x, o);
+        if (lane >= o) x += y;
+    }
+    return x;
 }
+
+__device__ __forceinline__ int lane_id() { return threadIdx.x & 63; }
 
 // Position of the k-th set bit (k from 0) of x.
 __device__ __forceinline__ int select_bit(uint64_t x, int k) {
@@ -117,21 +126,32 @@ __device__ __forceinline__ int select_bit(uint64_t x, int k) {
     return pos;
 }
 
-// Per-workgroup LDS state shared by both directions (about 19 KB at 256
-// threads, so eight workgroups fit a CU).  The network scratch of the
-// direction decision reuses the queue staging area, which is empty by then.
-constexpr int kMaxBlock = 512;
+// Index of the segment holding item i: the largest s with pref[s] <= i.
+__device__ __forceinline__ int seg_of(const int64_t *pref, int64_t i) {
+    int s = 0;
+#pragma unroll
+    for (int st = 64; st > 0; st >>= 1) {
+        const int mid = s + st;
+        if (mid <= kSeg && pref[mid] <= i) s = mid;
+    }
+    return s;
+}
+
+// Per-workgroup LDS state shared by both directions.  The network scratch of
+// the direction decision reuses the queue staging area.
 struct step_smem {
     union {
         int32_t q[kQCap];
         mlp_scratch mlp;
     };
     int32_t qn;
-    int32_t qbase;
-    int last;
+    int64_t qpos, qopos;
+    int32_t qfit;
     int use_mlp;
     float x[8];
     unsigned long long red[4][kMaxBlock / 64];
+    int64_t qpref[kSeg + 2];
+    int64_t tpref[kSeg + 2];
     // top-down tile
     int64_t t_beg[kMaxBlock];
     int64_t t_end[kMaxBlock];
@@ -144,31 +164,55 @@ struct step_smem {
     uint64_t b_found[kMaxBlock / 64][64];
 };
 
+// Reserve n slots of the output list for this workgroup's segment; returns
+// how many fit there (at *pos) and where the rest goes in the overflow
+// segment (at *opos).
+__device__ __forceinline__ int64_t seg_reserve(unsigned long long (*len)[kLine], int64_t seg_cap,
+                                               int64_t n, int64_t *pos, int64_t *opos) {
+    const int sidx = blockIdx.x % kSeg;
+    *pos = (int64_t)atomicAdd(&len[sidx][0], (unsigned long long)n);
+    int64_t fit = seg_cap - *pos;
+    fit = fit < 0 ? 0 : (fit > n ? n : fit);
+    *opos = 0;
+    if (fit < n) *opos = (int64_t)atomicAdd(&len[kSeg][0], (unsigned long long)(n - fit));
+    return fit;
+}
+
+__device__ __forceinline__ int64_t seg_base(int64_t seg_cap) {
+    return (int64_t)(blockIdx.x % kSeg) * seg_cap;
+}
+
+// Emit task slices of high-degree vertex u (one lane).
+__device__ void emit_tasks(const step_args &a, int32_t u, int64_t b, int64_t e) {
+    const int64_t dg = e - b;
+    const int64_t nt = (dg + a.task_edges - 1) / a.task_edges;
+    int64_t pos, opos;
+    const int64_t fit = seg_reserve(a.seg->tlen[(a.level + 1) & 1], a.tseg_cap, nt, &pos, &opos);
+    for (int64_t t = 0; t < nt; ++t) {
+        td_task tk;
+        tk.beg = b + t * a.task_edges;
+        tk.end = tk.beg + a.task_edges < e ? tk.beg + a.task_edges : e;
+        tk.src = u;
+        tk.pad = 0;
+        if (t < fit) {
+            a.tout[seg_base(a.tseg_cap) + pos + t] = tk;
+        } else {
+            const int64_t o = opos + (t - fit);
+            if (o < a.task_cap)
+                a.tout[kSeg * a.tseg_cap + o] = tk;
+            else
+                atomicAdd(&a.c->task_overflow, 1ULL);
+        }
+    }
+}
+
 // Emit vertex u of the next frontier: high-degree vertices become task
 // slices, the others are staged in LDS.  Every lane of the wave must call
 // this together (emit = false for idle lanes).
 __device__ __forceinline__ void emit_next(const step_args &a, step_smem &s, bool emit, int32_t u,
                                           int64_t b, int64_t e) {
-    const int64_t dg = e - b;
-    const bool huge = emit && dg >= a.task_t;
-    if (huge) {
-        const int64_t nt = (dg + a.task_edges - 1) / a.task_edges;
-        const int64_t base =
-            (int64_t)atomicAdd(&a.c->ntasks[(a.level + 1) & 1], (unsigned long long)nt);
-        for (int64_t t = 0; t < nt; ++t) {
-            const int64_t idx = base + t;
-            if (idx < a.task_cap) {
-                td_task tk;
-                tk.beg = b + t * a.task_edges;
-                tk.end = tk.beg + a.task_edges < e ? tk.beg + a.task_edges : e;
-                tk.src = u;
-                tk.pad = 0;
-                a.tout[idx] = tk;
-            } else {
-                atomicAdd(&a.c->task_overflow, 1ULL);
-            }
-        }
-    }
+    const bool huge = emit && (e - b) >= a.task_t;
+    if (huge) emit_tasks(a, u, b, e);
     const bool q = emit && !huge;
     const uint64_t m = __ballot(q);
     if (m == 0) return;
@@ -182,32 +226,55 @@ __device__ __forceinline__ void emit_next(const step_args &a, step_smem &s, bool
     // Slots below kQCap are staged in LDS (the flush writes exactly those);
     // the part of the batch past the end goes straight to HBM.
     const int in_lds = p >= kQCap ? 0 : (cnt < kQCap - p ? cnt : kQCap - p);
-    long long g = 0;
+    long long pos = 0, opos = 0, fit = 0;
     if (in_lds < cnt) {
-        if (lane == leader)
-            g = (long long)atomicAdd(&a.c->qlen[(a.level + 1) & 1],
-                                     (unsigned long long)(cnt - in_lds));
-        g = __shfl(g, leader);
+        if (lane == leader) {
+            int64_t ps, ops;
+            fit = seg_reserve(a.seg->qlen[(a.level + 1) & 1], a.qseg_cap, cnt - in_lds, &ps, &ops);
+            pos = ps;
+            opos = ops;
+        }
+        pos = __shfl(pos, leader);
+        opos = __shfl(opos, leader);
+        fit = __shfl(fit, leader);
     }
     if (q) {
-        if (rank < in_lds)
+        if (rank < in_lds) {
             s.q[p + rank] = u;
-        else
-            a.qout[g + (rank - in_lds)] = u;
+        } else {
+            const int64_t r = rank - in_lds;
+            if (r < fit)
+                a.qout[seg_base(a.qseg_cap) + pos + r] = u;
+            else
+                a.qout[kSeg * a.qseg_cap + opos + (r - fit)] = u;
+        }
     }
 }
 
-// Write the staged queue entries to HBM (whole workgroup).
+// Write the staged queue entries to this workgroup's segment (whole
+// workgroup).
 template <int BLOCK>
 __device__ void flush_queue(const step_args &a, step_smem &s) {
     __syncthreads();
     const int n = s.qn < kQCap ? s.qn : kQCap;
     if (n > 0) {
-        if (threadIdx.x == 0)
-            s.qbase = (int32_t)atomicAdd(&a.c->qlen[(a.level + 1) & 1], (unsigned long long)n);
+        if (threadIdx.x == 0) {
+            int64_t pos, opos;
+            s.qfit = (int32_t)seg_reserve(a.seg->qlen[(a.level + 1) & 1], a.qseg_cap, n, &pos,
+                                          &opos);
+            s.qpos = pos;
+            s.qopos = opos;
+        }
         __syncthreads();
-        const int64_t base = s.qbase;
-        for (int i = threadIdx.x; i < n; i += BLOCK) a.qout[base + i] = s.q[i];
+        const int fit = s.qfit;
+        const int64_t base = seg_base(a.qseg_cap) + s.qpos;
+        const int64_t obase = kSeg * a.qseg_cap + s.qopos;
+        for (int i = threadIdx.x; i < n; i += BLOCK) {
+            if (i < fit)
+                a.qout[base + i] = s.q[i];
+            else
+                a.qout[obase + (i - fit)] = s.q[i];
+        }
     }
     __syncthreads();
     if (threadIdx.x == 0) s.qn = 0;
@@ -216,7 +283,10 @@ __device__ void flush_queue(const step_args &a, step_smem &s) {
 
 // ---------------------------------------------------------------- top-down
 
-// Visit edge (src -> u).  Wave-collective.
+// Visit edge (src -> u).  Wave-collective.  The visited word is first read
+// with a device-scope load, which bypasses the CU's L1: a stale L1 copy would
+// send every later edge into a popular vertex to the atomic, and atomics on
+// one word serialize (~12 ns each).
 __device__ __forceinline__ void td_visit(const step_args &a, step_smem &s, bool valid, int32_t u,
                                          int32_t src, unsigned long long &cnt,
                                          unsigned long long &dsum, unsigned long long &dmax) {
@@ -224,7 +294,9 @@ __device__ __forceinline__ void td_visit(const step_args &a, step_smem &s, bool 
     const uint64_t bit = 1ULL << (u & 63);
     if (valid) {
         uint64_t *wp = a.vis + (u >> 6);
-        if (!(*wp & bit)) {
+        const uint64_t seen = __hip_atomic_load((unsigned long long *)wp, __ATOMIC_RELAXED,
+                                                __HIP_MEMORY_SCOPE_AGENT);
+        if (!(seen & bit)) {
             const unsigned long long old =
                 atomicOr((unsigned long long *)wp, (unsigned long long)bit);
             won = !(old & bit);
@@ -248,17 +320,24 @@ __device__ __forceinline__ void td_visit(const step_args &a, step_smem &s, bool 
 template <int BLOCK>
 __device__ void td_body(const step_args &a, step_smem &s, unsigned long long &cnt,
                         unsigned long long &dsum, unsigned long long &dmax) {
-    ctl_block *c = a.c;
     const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
-    const int64_t nitems = (int64_t)c->qlen[a.level & 1];
-    const int64_t nt_all = (int64_t)c->ntasks[a.level & 1];
-    const int64_t ntasks = nt_all < a.task_cap ? nt_all : a.task_cap;
+    for (int i = tid; i < kSeg + 2; i += BLOCK) {
+        s.qpref[i] = a.c->qpref[i];
+        s.tpref[i] = a.c->tpref[i];
+    }
+    __syncthreads();
+    const int64_t nitems = s.qpref[kSeg + 1];
+    const int64_t ntasks = s.tpref[kSeg + 1];
 
     for (int64_t tile = blockIdx.x; tile * BLOCK < nitems; tile += gridDim.x) {
         if (tid == 0) s.t_nmed = 0;
         __syncthreads();
         const int64_t slot = tile * BLOCK + tid;
-        const int32_t v = slot < nitems ? a.qin[slot] : -1;
+        int32_t v = -1;
+        if (slot < nitems) {
+            const int sg = seg_of(s.qpref, slot);
+            v = a.qin[(int64_t)sg * a.qseg_cap + (slot - s.qpref[sg])];
+        }
         int64_t b = 0, e = 0;
         if (v >= 0) {
             b = a.off[v];
@@ -327,7 +406,8 @@ __device__ void td_body(const step_args &a, step_smem &s, unsigned long long &cn
          t0 += (int64_t)gridDim.x * (BLOCK / 64)) {
         const int64_t t = t0 + wid;
         if (t < ntasks) {
-            const td_task tk = a.tin[t];
+            const int sg = seg_of(s.tpref, t);
+            const td_task tk = a.tin[(int64_t)sg * a.tseg_cap + (t - s.tpref[sg])];
             for (int64_t k0 = tk.beg; k0 < tk.end; k0 += 64) {
                 const int64_t k = k0 + lane;
                 const bool valid = k < tk.end;
@@ -342,12 +422,23 @@ __device__ void td_body(const step_args &a, step_smem &s, unsigned long long &cn
 
 // --------------------------------------------------------------- bottom-up
 
-// Scan up to `cap` neighbours of v from position k in CSR order; returns the
-// first one in the frontier or -1, and the position reached in *k.
+// Scan up to `cap` neighbours from position *kp in CSR order; returns the
+// first one in the frontier or -1, and the position reached in *kp.  The
+// first neighbour is tested alone (most vertices stop there), then four at a
+// time.
 __device__ __forceinline__ int32_t bu_scan(const step_args &a, int64_t *kp, int64_t e,
                                            int64_t cap, unsigned long long &exam) {
     int64_t k = *kp;
     const int64_t stop = (k + cap < e) ? k + cap : e;
+    if (k < stop) {
+        const int32_t u = a.cols[k];
+        ++exam;
+        ++k;
+        if ((a.front[u >> 6] >> (u & 63)) & 1) {
+            *kp = k;
+            return u;
+        }
+    }
     while (k < stop) {
         int32_t u[4];
         uint64_t w[4];
@@ -389,12 +480,7 @@ __device__ void bu_body(const step_args &a, step_smem &s, unsigned long long &cn
             if (own) a.clr[wi] = 0;
             const uint64_t unv = ~vw;
             const int cntw = __popcll(unv);
-            int excl = cntw;
-#pragma unroll
-            for (int o = 1; o < 64; o <<= 1) {
-                const int y = __shfl_up(excl, o);
-                if (lane >= o) excl += y;
-            }
+            int excl = (int)wave_incl_scan(cntw);
             const int total = __shfl(excl, 63);
             excl -= cntw;
             s.b_found[wid][lane] = 0;
@@ -424,8 +510,7 @@ __device__ void bu_body(const step_args &a, step_smem &s, unsigned long long &cn
                     par = bu_scan(a, &k, e, a.bu_cap, exam);
                 }
                 // lists longer than bu_cap: the whole wave finishes them
-                const bool lng = valid && par < 0 && k < e;
-                uint64_t lm = __ballot(lng);
+                uint64_t lm = __ballot(valid && par < 0 && k < e);
                 while (lm) {
                     const int src_lane = __ffsll((unsigned long long)lm) - 1;
                     lm &= lm - 1;
@@ -436,7 +521,8 @@ __device__ void bu_body(const step_args &a, step_smem &s, unsigned long long &cn
                         const int32_t u = kk < ke ? a.cols[kk] : -1;
                         const bool h = u >= 0 && ((a.front[u >> 6] >> (u & 63)) & 1);
                         const uint64_t hm = __ballot(h);
-                        const int seen = hm ? (__ffsll((unsigned long long)hm)) : (int)(ke - k0 < 64 ? ke - k0 : 64);
+                        const int64_t rest = ke - k0 < 64 ? ke - k0 : 64;
+                        const int64_t seen = hm ? (int64_t)__ffsll((unsigned long long)hm) : rest;
                         if (lane == src_lane) exam += (unsigned long long)seen;
                         if (hm) hit = __shfl(u, __ffsll((unsigned long long)hm) - 1);
                     }
@@ -538,18 +624,47 @@ __device__ void choose_dir(const step_args &a, step_smem &s, int32_t last_dir, i
     }
 }
 
-// Close step a.level: record, totals, reset accumulators, next direction.
-// Runs on one wave.  Accumulators are read with device-scope atomic loads
-// because, in the fused path, other workgroups updated them in this kernel.
+// Prefix sums of the segment lengths of list `len` (segment capacities
+// seg_cap, overflow capacity ovf_cap) into pref[0..kSeg+1], then zero
+// `clear_len`.  One wave; lane s handles segment s (lane 0 also the
+// overflow segment).
+__device__ void seg_prefix(unsigned long long (*len)[kLine], int64_t seg_cap, int64_t ovf_cap,
+                           int64_t *pref, unsigned long long (*clear_len)[kLine]) {
+    const int l = lane_id();
+    int64_t n = (int64_t)len[l][0];
+    n = n < seg_cap ? n : seg_cap;
+    int64_t no = (int64_t)len[kSeg][0];
+    no = no < ovf_cap ? no : ovf_cap;
+    const int64_t incl = wave_incl_scan(n);
+    pref[l + 1] = incl;
+    if (l == 0) pref[0] = 0;
+    if (l == 63) pref[kSeg + 1] = incl + no;
+    clear_len[l][0] = 0;
+    if (l == 0) clear_len[kSeg][0] = 0;
+}
+
+// Close step a.level: sum the slot counters, write the record, update the
+// totals, publish the next frontier's segment prefix sums, clear the lists
+// the next step fills, and pick the next direction.  Runs on one wave.
 __device__ void close_step(const step_args &a, step_smem &s) {
     ctl_block *c = a.c;
+    seg_counters *g = a.seg;
+    const int l = lane_id();
     __shared__ long long s_new, s_new_deg;
     __shared__ int s_dir;
-    if (lane_id() == 0) {
-        const int64_t nnew = (int64_t)ld_acquire(&c->acc_new);
-        const int64_t nnew_deg = (int64_t)ld_acquire(&c->acc_new_deg);
+    unsigned long long nn = wave_sum(g->acc[l][0]);
+    unsigned long long nd = wave_sum(g->acc[l][1]);
+    unsigned long long ex = wave_sum(g->acc[l][2]);
+    unsigned long long mx = wave_max(g->acc[l][3]);
+    g->acc[l][0] = 0;
+    g->acc[l][1] = 0;
+    g->acc[l][2] = 0;
+    g->acc[l][3] = 0;
+    const int32_t L = c->level;
+    if (l == 0) {
+        const int64_t nnew = (int64_t)nn;
+        const int64_t nnew_deg = (int64_t)nd;
         const int32_t dir = c->dir;
-        const int32_t L = c->level;
         bd_level_rec r;
         r.level = L;
         r.dir = dir;
@@ -561,11 +676,10 @@ __device__ void close_step(const step_args &a, step_smem &s) {
         r.result = dir == BD_DIR_TD ? nnew_deg : nnew;
         r.next = nnew;
         r.next_deg = nnew_deg;
-        r.edges_examined =
-            dir == BD_DIR_TD ? c->frontier_deg : (int64_t)ld_acquire(&c->acc_examined);
+        r.edges_examined = dir == BD_DIR_TD ? c->frontier_deg : (int64_t)ex;
         r.max_frontier_deg = c->max_frontier_deg;
-        r.tasks = (int64_t)c->ntasks[L & 1];
-        r.queued = (int64_t)c->qlen[L & 1];
+        r.tasks = c->tpref[kSeg + 1];
+        r.queued = c->qpref[kSeg + 1];
         r.ms = -1.0;
         r.ms_other = -1.0;
         if (L < a.log_cap) a.log[L] = r;
@@ -580,13 +694,7 @@ __device__ void close_step(const step_args &a, step_smem &s) {
         c->unvisited_deg -= nnew_deg;
         c->frontier = nnew;
         c->frontier_deg = nnew_deg;
-        c->max_frontier_deg = (int64_t)ld_acquire(&c->acc_max_deg);
-        c->acc_new = 0;
-        c->acc_new_deg = 0;
-        c->acc_examined = 0;
-        c->acc_max_deg = 0;
-        c->ntasks[L & 1] = 0;
-        c->qlen[L & 1] = 0;
+        c->max_frontier_deg = (int64_t)mx;
         c->prev_dir = dir;
         c->level = L + 1;
         if (nnew == 0) c->done = 1;
@@ -595,14 +703,20 @@ __device__ void close_step(const step_args &a, step_smem &s) {
         s_dir = dir;
     }
     wave_lds_sync();
+    const int o = (L + 1) & 1, i = L & 1;
+    seg_prefix(g->qlen[o], a.qseg_cap, a.nv, c->qpref, g->qlen[i]);
+    seg_prefix(g->tlen[o], a.tseg_cap, a.task_cap, c->tpref, g->tlen[i]);
+    wave_lds_sync();
     if (s_new > 0) choose_dir(a, s, s_dir, s_new, s_new_deg);
-    if (lane_id() == 0) c->t_end = wall_clock64();
+    if (l == 0) c->t_end = wall_clock64();
 }
 
 // ---------------------------------------------------------------- kernels
 
+// Sum the block's counters and add them to its slot (spread over kSlots
+// cache lines so no line sees more than a few dozen atomics per step).
 template <int BLOCK>
-__device__ void block_flush_counters(ctl_block *c, step_smem &s, unsigned long long cnt,
+__device__ void block_flush_counters(const step_args &a, step_smem &s, unsigned long long cnt,
                                      unsigned long long dsum, unsigned long long exam,
                                      unsigned long long dmax) {
     const int lane = lane_id(), wid = threadIdx.x >> 6;
@@ -624,10 +738,11 @@ __device__ void block_flush_counters(ctl_block *c, step_smem &s, unsigned long l
             exam += s.red[2][w];
             dmax = s.red[3][w] > dmax ? s.red[3][w] : dmax;
         }
-        if (cnt) atomicAdd(&c->acc_new, cnt);
-        if (dsum) atomicAdd(&c->acc_new_deg, dsum);
-        if (exam) atomicAdd(&c->acc_examined, exam);
-        if (dmax) atomicMax(&c->acc_max_deg, dmax);
+        unsigned long long *slot = a.seg->acc[blockIdx.x % kSlots];
+        if (cnt) atomicAdd(&slot[0], cnt);
+        if (dsum) atomicAdd(&slot[1], dsum);
+        if (exam) atomicAdd(&slot[2], exam);
+        if (dmax) atomicMax(&slot[3], dmax);
     }
 }
 
@@ -649,32 +764,26 @@ __global__ __launch_bounds__(BLOCK) void k_step(step_args a) {
         bu_body<BLOCK>(a, s, cnt, dsum, dmax, exam);
     }
     flush_queue<BLOCK>(a, s);
-    block_flush_counters<BLOCK>(c, s, cnt, dsum, exam, dmax);
-    if (!a.fused_decide) return;
-    // The last workgroup to arrive closes the step.  The fence orders this
-    // workgroup's counter and queue atomics before its arrival.
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        __threadfence();
-        const unsigned t = atomicAdd(&c->arrive, 1u);
-        s.last = (t == gridDim.x - 1);
-    }
-    __syncthreads();
-    if (!s.last) return;
-    __threadfence();
-    if (threadIdx.x < 64) close_step(a, s);
-    if (threadIdx.x == 0) c->arrive = 0;
+    block_flush_counters<BLOCK>(a, s, cnt, dsum, exam, dmax);
 }
 
-// Close a step run without the fused decision (host-driven and profiling
-// runs), or start a search when source >= 0: clear() (hybrid_bfs.cc:433-443)
-// has been done by k_clear; here the source is pushed and marked
-// (hybrid_bfs.cc:157-163) and the first direction chosen.
+// Close step a.level (source < 0), or start a search (source >= 0):
+// clear() (hybrid_bfs.cc:433-443) has been done by k_clear; here the source
+// is pushed and marked (hybrid_bfs.cc:157-163) and the first direction
+// chosen.
 __global__ __launch_bounds__(64) void k_decide(step_args a, int64_t source) {
     ctl_block *c = a.c;
+    seg_counters *g = a.seg;
     __shared__ step_smem s;
+    const int l = lane_id();
     if (source >= 0) {
-        if (threadIdx.x == 0) {
+        for (int w = 0; w < 2; ++w)
+            for (int i = l; i <= kSeg; i += 64) {
+                g->qlen[w][i][0] = 0;
+                g->tlen[w][i][0] = 0;
+            }
+        for (int k = 0; k < 4; ++k) g->acc[l][k] = 0;
+        if (l == 0) {
             const int64_t b = a.off[source], e = a.off[source + 1];
             const int64_t dsrc = e - b;
             memset(c, 0, sizeof(ctl_block));
@@ -697,27 +806,38 @@ __global__ __launch_bounds__(64) void k_decide(step_args a, int64_t source) {
             a.next_bm[source >> 6] = 1ULL << (source & 63);  // bm[0]
             a.depth[source] = 0;
             a.parent[source] = (int32_t)source;
+            // frontier 0 lives in segment 0 of list 0
             if (dsrc >= a.task_t) {
                 const int64_t nt = (dsrc + a.task_edges - 1) / a.task_edges;
-                for (int64_t t = 0; t < nt && t < a.task_cap; ++t) {
+                for (int64_t t = 0; t < nt; ++t) {
                     td_task tk;
                     tk.beg = b + t * a.task_edges;
                     tk.end = tk.beg + a.task_edges < e ? tk.beg + a.task_edges : e;
                     tk.src = (int32_t)source;
                     tk.pad = 0;
-                    a.tout[t] = tk;
+                    if (t < a.tseg_cap)
+                        a.tout[t] = tk;
+                    else
+                        a.tout[kSeg * a.tseg_cap + (t - a.tseg_cap)] = tk;
                 }
-                c->ntasks[0] = nt;
+                for (int k = 0; k < kSeg + 2; ++k) {
+                    c->qpref[k] = 0;
+                    c->tpref[k] = k == 0 ? 0 : (nt < a.tseg_cap ? nt : a.tseg_cap);
+                }
+                c->tpref[kSeg + 1] = nt;
             } else {
                 a.qout[0] = (int32_t)source;
-                c->qlen[0] = 1;
+                for (int k = 0; k < kSeg + 2; ++k) {
+                    c->qpref[k] = k == 0 ? 0 : 1;
+                    c->tpref[k] = 0;
+                }
             }
         }
         wave_lds_sync();
         // level 0: with no previous step Beamer applies only the outer test
         // to scout = deg(source) (hybrid_bfs.cc:163-167)
         choose_dir(a, s, -1, 0, 0);
-        if (threadIdx.x == 0) {
+        if (l == 0) {
             __threadfence();
             c->t_start = wall_clock64();
         }
@@ -746,17 +866,20 @@ __global__ void k_clear(int32_t *__restrict__ depth, int32_t *__restrict__ paren
 }
 
 // Profiling: undo the first direction's step before timing the other one.
-__global__ void k_reset_step(ctl_block *c, int32_t level) {
-    c->acc_new = 0;
-    c->acc_new_deg = 0;
-    c->acc_examined = 0;
-    c->acc_max_deg = 0;
-    c->qlen[(level + 1) & 1] = 0;
-    c->ntasks[(level + 1) & 1] = 0;
+__global__ __launch_bounds__(64) void k_reset_step(seg_counters *g, int32_t level) {
+    const int l = threadIdx.x;
+    const int o = (level + 1) & 1;
+    for (int k = 0; k < 4; ++k) g->acc[l][k] = 0;
+    g->qlen[o][l][0] = 0;
+    g->tlen[o][l][0] = 0;
+    if (l == 0) {
+        g->qlen[o][kSeg][0] = 0;
+        g->tlen[o][kSeg][0] = 0;
+    }
 }
 
 // count_num_traversed_edges (hybrid_bfs.cc:401-414): sum of degrees over
-// reached vertices.
+// reached vertices (one atomic per wave, spread over kSlots lines).
 __global__ void k_count_traversed(const int32_t *__restrict__ depth,
                                   const int64_t *__restrict__ off, int64_t nv,
                                   unsigned long long *__restrict__ out) {
@@ -765,7 +888,7 @@ __global__ void k_count_traversed(const int32_t *__restrict__ depth,
          v += (int64_t)gridDim.x * blockDim.x)
         if (depth[v] >= 0) s += (unsigned long long)(off[v + 1] - off[v]);
     s = wave_sum(s);
-    if ((threadIdx.x & 63) == 0 && s) atomicAdd(out, s);
+    if ((threadIdx.x & 63) == 0 && s) atomicAdd(&out[(blockIdx.x % kSlots) * kLine], s);
 }
 
 // Streams a buffer larger than the last-level cache so a timed step starts
@@ -798,11 +921,14 @@ step_args make_args(bd_ctx *c, int32_t level, int64_t alpha, int64_t beta, int p
     a.depth = c->d_depth;
     a.parent = c->d_parent;
     a.c = c->d_ctl;
+    a.seg = c->d_seg;
     a.log = c->d_log;
     a.script = c->d_script;
     a.mlp = c->d_mlp;
     a.nwords = c->nwords;
     a.task_cap = c->task_cap;
+    a.qseg_cap = c->qseg_cap;
+    a.tseg_cap = c->tseg_cap;
     a.nv = c->nv;
     a.m = c->m;
     a.nnz = c->nnz;
@@ -811,7 +937,6 @@ step_args make_args(bd_ctx *c, int32_t level, int64_t alpha, int64_t beta, int p
     a.level = level;
     a.gate = 1;
     a.force_dir = -1;
-    a.fused_decide = 1;
     a.wave_t = std::max(1, c->tune.wave_threshold);
     a.task_t = std::max(64, c->tune.task_threshold);
     a.task_edges = std::max(64, c->tune.task_edges);
@@ -867,7 +992,7 @@ int start_search(bd_ctx *c, int64_t source, int64_t alpha, int64_t beta, int pol
                        c->d_parent, c->nv, c->d_vis, c->d_dead, c->d_bm[0], c->d_bm[1],
                        c->d_bm[2], c->nwords);
     BD_HIP(c, hipGetLastError());
-    // level "-1" pointers: next_bm = bm[0], qout = q[0], tout = tasks[0]
+    // frontier 0: bitmap bm[0], queue q[0], tasks tasks[0]
     step_args a = make_args(c, 0, alpha, beta, policy);
     a.next_bm = c->d_bm[0];
     a.qout = c->d_q[0];
@@ -923,7 +1048,6 @@ int run_search(bd_ctx *c, int64_t source, int64_t alpha, int64_t beta, int polic
         if ((rc = read_ctl(c))) return rc;
         while (!c->h_ctl->done) {
             step_args a = make_args(c, L, alpha, beta, policy);
-            a.fused_decide = 0;
             if (timed) BD_HIP(c, hipEventRecord(c->ev[2], c->stream));
             if ((rc = launch_step(c, a))) return rc;
             if (timed) BD_HIP(c, hipEventRecord(c->ev[3], c->stream));
@@ -939,8 +1063,11 @@ int run_search(bd_ctx *c, int64_t source, int64_t alpha, int64_t beta, int polic
     } else {
         int chunk = 8;
         for (;;) {
-            for (int k = 0; k < chunk; ++k, ++L)
-                if ((rc = launch_step(c, make_args(c, L, alpha, beta, policy)))) return rc;
+            for (int k = 0; k < chunk; ++k, ++L) {
+                const step_args a = make_args(c, L, alpha, beta, policy);
+                if ((rc = launch_step(c, a))) return rc;
+                if ((rc = launch_decide(c, a, -1))) return rc;
+            }
             if ((rc = read_ctl(c))) return rc;
             if (c->h_ctl->done) break;
             if (L > max_levels) return set_err(c, BD_EHIP, "search did not terminate");
@@ -1010,20 +1137,23 @@ extern "C" int bd_bfs_count_traversed(bd_ctx *c, int64_t *out) {
     if (!c->have_result) return set_err(c, BD_ESTATE, "no search has run");
     BD_HIP(c, hipSetDevice(c->device));
     unsigned long long *d = nullptr;
-    int rc = dev_alloc(c, (void **)&d, sizeof(unsigned long long), "count");
+    const size_t bytes = sizeof(unsigned long long) * kSlots * kLine;
+    int rc = dev_alloc(c, (void **)&d, bytes, "count");
     if (rc) return rc;
-    unsigned long long h = 0;
-    hipError_t e = hipMemsetAsync(d, 0, sizeof(h), c->stream);
+    std::vector<unsigned long long> h(kSlots * kLine, 0);
+    hipError_t e = hipMemsetAsync(d, 0, bytes, c->stream);
     if (e == hipSuccess) {
         hipLaunchKernelGGL(k_count_traversed, dim3(grid_for(c, c->nv, 256)), dim3(256), 0,
                            c->stream, c->d_depth, c->d_off, c->nv, d);
         e = hipGetLastError();
     }
-    if (e == hipSuccess) e = hipMemcpyAsync(&h, d, sizeof(h), hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(h.data(), d, bytes, hipMemcpyDeviceToHost, c->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
     dev_free(d);
     if (e != hipSuccess) return hip_err(c, e, "count_traversed");
-    *out = (int64_t)(h / 2);
+    unsigned long long sum = 0;
+    for (int s = 0; s < kSlots; ++s) sum += h[s * kLine];
+    *out = (int64_t)(sum / 2);
     return BD_OK;
 }
 
@@ -1102,6 +1232,7 @@ extern "C" int bd_bfs_profile(bd_ctx *c, int64_t source, int64_t alpha, int64_t 
     if ((rc = start_search(c, source, alpha, beta, policy))) return rc;
     if ((rc = read_ctl(c))) return rc;
     std::vector<std::pair<double, double>> times;
+    std::vector<unsigned long long> acc(kSlots * kLine);
     double total = 0;
     int32_t L = 0;
     while (!c->h_ctl->done) {
@@ -1111,13 +1242,12 @@ extern "C" int bd_bfs_profile(bd_ctx *c, int64_t source, int64_t alpha, int64_t 
         unsigned long long found[2] = {0, 0};
         for (int pass = 0; pass < 2; ++pass) {
             step_args a = make_args(c, L, alpha, beta, policy);
-            a.fused_decide = 0;
             a.gate = 0;
             a.force_dir = pass == 0 ? other : chosen;
             if (pass == 1) {
                 BD_HIP(c, hipMemcpyAsync(c->d_vis, c->d_vis_save, sizeof(uint64_t) * c->nwords,
                                          hipMemcpyDeviceToDevice, c->stream));
-                hipLaunchKernelGGL(k_reset_step, dim3(1), dim3(1), 0, c->stream, c->d_ctl, L);
+                hipLaunchKernelGGL(k_reset_step, dim3(1), dim3(64), 0, c->stream, c->d_seg, L);
             } else {
                 BD_HIP(c, hipMemcpyAsync(c->d_vis_save, c->d_vis, sizeof(uint64_t) * c->nwords,
                                          hipMemcpyDeviceToDevice, c->stream));
@@ -1129,8 +1259,9 @@ extern "C" int bd_bfs_profile(bd_ctx *c, int64_t source, int64_t alpha, int64_t 
             BD_HIP(c, hipEventRecord(c->ev[3], c->stream));
             BD_HIP(c, hipEventSynchronize(c->ev[3]));
             BD_HIP(c, hipEventElapsedTime(&ms[pass], c->ev[2], c->ev[3]));
-            if ((rc = read_ctl(c))) return rc;
-            found[pass] = c->h_ctl->acc_new;
+            BD_HIP(c, hipMemcpy(acc.data(), c->d_seg->acc, sizeof(unsigned long long) * acc.size(),
+                                hipMemcpyDeviceToHost));
+            for (int s = 0; s < kSlots; ++s) found[pass] += acc[s * kLine];
         }
         if (found[0] != found[1])
             return set_err(c, BD_EHIP, "profile: directions disagree on the next frontier");

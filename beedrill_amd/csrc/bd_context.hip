@@ -90,15 +90,23 @@ int alloc_search(bd_ctx *c) {
         return rc;
     for (auto &b : c->d_bm)
         if ((rc = dev_alloc(c, (void **)&b, sizeof(uint64_t) * nw, "frontier bitmap"))) return rc;
+    // Segmented lists: kSeg segments of twice the even share, then an
+    // overflow segment as large as the whole list can be.
+    c->qseg_cap = 2 * ((nv + kSeg - 1) / kSeg) + 4096;
     for (auto &q : c->d_q)
-        if ((rc = dev_alloc(c, (void **)&q, sizeof(int32_t) * nv, "frontier queue"))) return rc;
+        if ((rc = dev_alloc(c, (void **)&q, sizeof(int32_t) * (kSeg * c->qseg_cap + nv),
+                            "frontier queue")))
+            return rc;
     // Each vertex of degree >= task_threshold yields <= deg/task_edges + 1
     // tasks and appears in at most one frontier, so this bound is exact.
-    const int64_t te = std::max(1, c->tune.task_edges);
-    const int64_t tt = std::max(1, c->tune.task_threshold);
+    const int64_t te = std::max(64, c->tune.task_edges);
+    const int64_t tt = std::max(64, c->tune.task_threshold);
     c->task_cap = c->nnz / te + c->nnz / tt + 16;
+    c->tseg_cap = 2 * ((c->task_cap + kSeg - 1) / kSeg) + 1024;
     for (auto &t : c->d_tasks)
-        if ((rc = dev_alloc(c, (void **)&t, sizeof(td_task) * c->task_cap, "tasks"))) return rc;
+        if ((rc = dev_alloc(c, (void **)&t, sizeof(td_task) * (kSeg * c->tseg_cap + c->task_cap),
+                            "tasks")))
+            return rc;
     return BD_OK;
 }
 
@@ -128,8 +136,8 @@ extern "C" int bd_create(bd_ctx **out, int device) {
     c->tune.bu_words = 32;
     c->tune.bu_cap = 32;
     c->tune.wave_threshold = 32;
-    c->tune.task_threshold = 4096;
-    c->tune.task_edges = 1024;
+    c->tune.task_threshold = 256;
+    c->tune.task_edges = 128;
     c->tune.level_driver = 1;
     c->tune.time_steps = 2;
     hipError_t e = hipSetDevice(device);
@@ -144,6 +152,7 @@ extern "C" int bd_create(bd_ctx **out, int device) {
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
     for (int i = 0; i < 4 && e == hipSuccess; ++i) e = hipEventCreate(&c->ev[i]);
     if (e == hipSuccess) e = hipMalloc((void **)&c->d_ctl, sizeof(ctl_block));
+    if (e == hipSuccess) e = hipMalloc((void **)&c->d_seg, sizeof(seg_counters));
     if (e == hipSuccess) e = hipHostMalloc((void **)&c->h_ctl, sizeof(ctl_block), hipHostMallocDefault);
     if (e == hipSuccess) {
         c->log_cap = 1 << 16;
@@ -164,6 +173,7 @@ extern "C" int bd_destroy(bd_ctx *c) {
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     free_graph(c);
     dev_free(c->d_ctl);
+    dev_free(c->d_seg);
     dev_free(c->d_log);
     dev_free(c->d_script);
     dev_free(c->d_mlp);

@@ -23,12 +23,27 @@ struct td_task {
     int32_t pad;
 };
 
-// Device-resident control block.  The step kernel accumulates into acc_*;
-// the last workgroup of the step (or the decide kernel) closes the step,
-// writes its record and picks the next direction, so the level loop runs
-// without the host.  qlen[L&1]/ntasks[L&1] describe frontier L (queue of
-// ordinary vertices and edge slices of high-degree vertices); step L fills
-// the other half.
+// Frontier lists (queue of ordinary vertices, task slices of high-degree
+// vertices) are segmented: workgroup b appends to segment b % kSeg, so no
+// address is shared by more than a few dozen workgroups -- on MI355X a
+// same-address atomic costs ~12 ns and they serialize, so one counter hit by
+// 2048 workgroups costs ~25 us.  Segment kSeg is an overflow segment that
+// takes what does not fit.  The same spreading applies to the step's
+// counters (kSlots partial accumulators, each on its own cache line).
+constexpr int kSeg = 64;
+constexpr int kSlots = 64;
+constexpr int kLine = 16;  // 64-bit words per 128-byte line
+
+struct seg_counters {
+    unsigned long long qlen[2][kSeg + 1][kLine];
+    unsigned long long tlen[2][kSeg + 1][kLine];
+    // per-slot partial accumulators: new, new_deg, examined, max_deg
+    unsigned long long acc[kSlots][kLine];
+};
+
+// Device-resident control block, written by the decide kernel between
+// steps (and read by every workgroup of a step).  qpref/tpref are the
+// prefix sums of the segment lengths of frontier L (index NSEG+1 = total).
 struct ctl_block {
     // run constants
     int64_t nv;
@@ -48,20 +63,15 @@ struct ctl_block {
     int32_t level;
     int32_t dir;
     int32_t done;
-    uint32_t arrive;     // workgroups finished in the running step
+    int32_t pad0;
     int32_t prev_dir;    // -1 before the first step
     int64_t frontier;
     int64_t frontier_deg;
     int64_t max_frontier_deg;
     int64_t unvisited;
     int64_t unvisited_deg;
-    // accumulators of the running step
-    unsigned long long acc_new;
-    unsigned long long acc_new_deg;
-    unsigned long long acc_examined;
-    unsigned long long acc_max_deg;
-    unsigned long long qlen[2];
-    unsigned long long ntasks[2];
+    int64_t qpref[kSeg + 2];
+    int64_t tpref[kSeg + 2];
     unsigned long long task_overflow;
     // totals
     int64_t steps;
@@ -101,7 +111,10 @@ struct bd_ctx {
     uint64_t *d_bm[3] = {nullptr, nullptr, nullptr};
     int32_t *d_q[2] = {nullptr, nullptr};
     bd::td_task *d_tasks[2] = {nullptr, nullptr};
-    int64_t task_cap = 0;
+    int64_t task_cap = 0;    // most task slices one frontier can need
+    int64_t qseg_cap = 0;    // entries per queue segment (overflow holds nv)
+    int64_t tseg_cap = 0;    // entries per task segment (overflow holds task_cap)
+    bd::seg_counters *d_seg = nullptr;
     bd::ctl_block *d_ctl = nullptr;
     bd::ctl_block *h_ctl = nullptr;  // pinned mirror
     bd_level_rec *d_log = nullptr;
