@@ -182,26 +182,54 @@ __device__ __forceinline__ int64_t seg_base(int64_t seg_cap) {
     return (int64_t)(blockIdx.x % kSeg) * seg_cap;
 }
 
-// Emit task slices of high-degree vertex u (one lane).
-__device__ void emit_tasks(const step_args &a, int32_t u, int64_t b, int64_t e) {
-    const int64_t dg = e - b;
-    const int64_t nt = (dg + a.task_edges - 1) / a.task_edges;
-    int64_t pos, opos;
-    const int64_t fit = seg_reserve(a.seg->tlen[(a.level + 1) & 1], a.tseg_cap, nt, &pos, &opos);
-    for (int64_t t = 0; t < nt; ++t) {
-        td_task tk;
-        tk.beg = b + t * a.task_edges;
-        tk.end = tk.beg + a.task_edges < e ? tk.beg + a.task_edges : e;
-        tk.src = u;
-        tk.pad = 0;
-        if (t < fit) {
-            a.tout[seg_base(a.tseg_cap) + pos + t] = tk;
-        } else {
-            const int64_t o = opos + (t - fit);
-            if (o < a.task_cap)
-                a.tout[kSeg * a.tseg_cap + o] = tk;
-            else
-                atomicAdd(&a.c->task_overflow, 1ULL);
+// Emit the task slices of the lanes' high-degree vertices (wave-collective):
+// one reservation for the whole wave, then the slices are written 64 at a
+// time, lane j taking slice j of the concatenation (its owner found by a
+// binary search over the lanes' exclusive prefix).
+__device__ void emit_tasks(const step_args &a, bool huge, int32_t u, int64_t b, int64_t e) {
+    const int lane = lane_id();
+    const long long nt = huge ? (e - b + a.task_edges - 1) / a.task_edges : 0;
+    const long long incl = wave_incl_scan(nt);
+    const long long total = __shfl(incl, 63);
+    const long long excl = incl - nt;
+    long long pos = 0, opos = 0, fit = 0;
+    if (lane == 0) {
+        int64_t ps, ops;
+        fit = seg_reserve(a.seg->tlen[(a.level + 1) & 1], a.tseg_cap, total, &ps, &ops);
+        pos = ps;
+        opos = ops;
+    }
+    pos = __shfl(pos, 0);
+    opos = __shfl(opos, 0);
+    fit = __shfl(fit, 0);
+    for (long long j0 = 0; j0 < total; j0 += 64) {
+        const long long j = j0 + lane;
+        const long long jj = j < total ? j : total - 1;
+        int o = 0;
+#pragma unroll
+        for (int st = 32; st > 0; st >>= 1) {
+            const int mid = o + st;
+            const long long em = __shfl(excl, mid < 64 ? mid : 63);
+            if (mid < 64 && em <= jj) o = mid;
+        }
+        const long long t = jj - __shfl(excl, o);
+        const int64_t bo = __shfl(b, o), eo = __shfl(e, o);
+        const int32_t uo = __shfl(u, o);
+        if (j < total) {
+            td_task tk;
+            tk.beg = bo + t * a.task_edges;
+            tk.end = tk.beg + a.task_edges < eo ? tk.beg + a.task_edges : eo;
+            tk.src = uo;
+            tk.pad = 0;
+            if (j < fit) {
+                a.tout[seg_base(a.tseg_cap) + pos + j] = tk;
+            } else {
+                const int64_t ov = opos + (j - fit);
+                if (ov < a.task_cap)
+                    a.tout[kSeg * a.tseg_cap + ov] = tk;
+                else
+                    atomicAdd(&a.c->task_overflow, 1ULL);
+            }
         }
     }
 }
@@ -212,7 +240,7 @@ __device__ void emit_tasks(const step_args &a, int32_t u, int64_t b, int64_t e) 
 __device__ __forceinline__ void emit_next(const step_args &a, step_smem &s, bool emit, int32_t u,
                                           int64_t b, int64_t e) {
     const bool huge = emit && (e - b) >= a.task_t;
-    if (huge) emit_tasks(a, u, b, e);
+    if (__ballot(huge)) emit_tasks(a, huge, u, b, e);
     const bool q = emit && !huge;
     const uint64_t m = __ballot(q);
     if (m == 0) return;
@@ -420,6 +448,74 @@ __device__ void td_body(const step_args &a, step_smem &s, unsigned long long &cn
     }
 }
 
+// Top-down step whose frontier is only in the depth-L bitmap (the previous
+// step was bottom-up, which does not build lists).  Each wave takes chunks of
+// bu_words frontier words, ranks their set bits, and takes 64 frontier
+// vertices at a time: short lists are walked by their own lane in lockstep,
+// lists of degree >= wave_t by the whole wave.
+template <int BLOCK>
+__device__ void td_bitmap_body(const step_args &a, step_smem &s, unsigned long long &cnt,
+                               unsigned long long &dsum, unsigned long long &dmax) {
+    const int lane = lane_id(), wid = threadIdx.x >> 6;
+    const int CW = a.bu_words;
+    const int64_t nchunks = (a.nwords + CW - 1) / CW;
+    for (int64_t c0 = (int64_t)blockIdx.x * (BLOCK / 64); c0 < nchunks;
+         c0 += (int64_t)gridDim.x * (BLOCK / 64)) {
+        const int64_t chunk = c0 + wid;
+        if (chunk < nchunks) {
+            const int64_t w0 = chunk * CW;
+            const int64_t wi = w0 + lane;
+            const uint64_t fw = (lane < CW && wi < a.nwords) ? a.front[wi] : 0ULL;
+            const int cntw = __popcll(fw);
+            int excl = (int)wave_incl_scan(cntw);
+            const int total = __shfl(excl, 63);
+            excl -= cntw;
+            for (int r = 0; r < total; r += 64) {
+                const int g = r + lane;
+                const bool valid = g < total;
+                const int gg = valid ? g : total - 1;
+                int j = 0;
+#pragma unroll
+                for (int st = 32; st > 0; st >>= 1) {
+                    const int mid = j + st;
+                    const int em = __shfl(excl, mid < 64 ? mid : 63);
+                    if (mid < 64 && em <= gg) j = mid;
+                }
+                const uint64_t bits = (uint64_t)__shfl((long long)fw, j);
+                const int bpos = select_bit(bits, gg - __shfl(excl, j));
+                const int32_t v = (int32_t)((w0 + j) * 64 + bpos);
+                int64_t b = 0, e = 0;
+                if (valid) {
+                    b = a.off[v];
+                    e = a.off[v + 1];
+                }
+                const bool small = valid && (e - b) < a.wave_t;
+                const int64_t maxd = (int64_t)wave_max(small ? (unsigned long long)(e - b) : 0ULL);
+                for (int64_t k = 0; k < maxd; ++k) {
+                    const bool vv = small && k < e - b;
+                    const int32_t u = vv ? a.cols[b + k] : 0;
+                    td_visit(a, s, vv, u, v, cnt, dsum, dmax);
+                }
+                uint64_t big = __ballot(valid && !small);
+                while (big) {
+                    const int o = __ffsll((unsigned long long)big) - 1;
+                    big &= big - 1;
+                    const int64_t bo = __shfl(b, o), eo = __shfl(e, o);
+                    const int32_t vo = __shfl(v, o);
+                    for (int64_t k0 = bo; k0 < eo; k0 += 64) {
+                        const int64_t k = k0 + lane;
+                        const bool vv = k < eo;
+                        const int32_t u = vv ? a.cols[k] : 0;
+                        td_visit(a, s, vv, u, vo, cnt, dsum, dmax);
+                    }
+                }
+            }
+        }
+        __syncthreads();
+        if (s.qn >= kQCap / 2) flush_queue<BLOCK>(a, s);
+    }
+}
+
 // --------------------------------------------------------------- bottom-up
 
 // Scan up to `cap` neighbours from position *kp in CSR order; returns the
@@ -539,7 +635,6 @@ __device__ void bu_body(const step_args &a, step_smem &s, unsigned long long &cn
                     atomicOr((unsigned long long *)&s.b_found[wid][j],
                              (unsigned long long)(1ULL << bpos));
                 }
-                emit_next(a, s, found, (int32_t)v, b, e);
             }
             wave_lds_sync();
             if (own) {
@@ -696,6 +791,7 @@ __device__ void close_step(const step_args &a, step_smem &s) {
         c->frontier_deg = nnew_deg;
         c->max_frontier_deg = (int64_t)mx;
         c->prev_dir = dir;
+        c->td_bitmap = dir == BD_DIR_BU ? 1 : 0;
         c->level = L + 1;
         if (nnew == 0) c->done = 1;
         s_new = nnew;
@@ -759,7 +855,10 @@ __global__ __launch_bounds__(BLOCK) void k_step(step_args a) {
         for (int64_t i = (int64_t)blockIdx.x * BLOCK + threadIdx.x; i < a.nwords;
              i += (int64_t)gridDim.x * BLOCK)
             a.clr[i] = 0;
-        td_body<BLOCK>(a, s, cnt, dsum, dmax);
+        if (c->td_bitmap)
+            td_bitmap_body<BLOCK>(a, s, cnt, dsum, dmax);
+        else
+            td_body<BLOCK>(a, s, cnt, dsum, dmax);
     } else {
         bu_body<BLOCK>(a, s, cnt, dsum, dmax, exam);
     }

@@ -63,7 +63,7 @@ struct ctl_block {
     int32_t level;
     int32_t dir;
     int32_t done;
-    int32_t pad0;
+    int32_t td_bitmap;   // frontier L is only in bm[L%3] (step L-1 was BU)
     int32_t prev_dir;    // -1 before the first step
     int64_t frontier;
     int64_t frontier_deg;

@@ -1,0 +1,122 @@
+"""Train the learned direction policy on the GPU.
+
+The network predicts, for the state before a BFS step, the log2 cost in
+microseconds of a top-down step and of a bottom-up step; the learned policy
+takes the cheaper one (it replaces the predicates at hybrid_bfs.cc:167 and
+:178-179).  Training data are per-step timings the search logs itself
+(bd_bfs_profile times both directions from the same state at every step) on
+seeded RMAT graphs of scales 18-22 with A in [0.45, 0.60] (B = C =
+(0.95 - A) / 2, D = 0.05), on generator seeds disjoint from the benchmark's
+(seed 0).  Training is full-batch Adam on the device (bd_mlp_train).
+
+    python -m beedrill_amd.train_policy --out beedrill_amd/models/direction_mlp_32x32.f32
+
+Writes the float32 parameters (layout in include/beedrill_amd.h) and a JSON
+file with the data set summary next to them.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+if os.path.dirname(HERE) not in sys.path:
+    sys.path.insert(0, os.path.dirname(HERE))
+
+import beedrill_amd as bd  # noqa: E402
+
+DEFAULT_OUT = os.path.join(HERE, "models", "direction_mlp_32x32.f32")
+
+
+def training_graphs(scales=(18, 19, 20, 21, 22), a_values=(0.45, 0.50, 0.55, 0.60),
+                    edge_factors=(16,), seeds=(101, 202)):
+    for scale in scales:
+        for a in a_values:
+            for ef in edge_factors:
+                for seed in seeds:
+                    bc = (0.95 - a) / 2
+                    yield dict(scale=scale, a=a, b=bc, c=bc, d=0.05, edge_factor=ef, seed=seed)
+
+
+def log_steps(ctx, graph, roots_per_graph):
+    """Profile `roots_per_graph` lcg(0) roots; one sample per step."""
+    ctx.generate_rmat(graph["scale"], graph["edge_factor"], graph["a"], graph["b"], graph["c"],
+                      graph["d"], seed=graph["seed"])
+    nv, _, m = ctx.info()
+    xs, ys, meta = [], [], []
+    for root in ctx.pick_roots(roots_per_graph):
+        ctx.profile(int(root), 15, 18, "beamer")
+        prev = -1
+        for rec in ctx.levels():
+            t_chosen, t_other = rec["ms"] * 1e3, rec["ms_other"] * 1e3
+            if rec["dir"] == bd.DIR_TD:
+                t_td, t_bu = t_chosen, t_other
+            else:
+                t_td, t_bu = t_other, t_chosen
+            xs.append(bd.features(rec, nv, m, prev))
+            ys.append([math.log2(max(t_td, 1.0)), math.log2(max(t_bu, 1.0))])
+            meta.append(dict(root=int(root), level=rec["level"], dir=rec["dir"], td_us=t_td,
+                             bu_us=t_bu, **{k: graph[k] for k in ("scale", "a", "seed")}))
+            prev = rec["dir"]
+    return xs, ys, meta
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--out", default=DEFAULT_OUT)
+    ap.add_argument("--roots", type=int, default=6)
+    ap.add_argument("--epochs", type=int, default=3000)
+    ap.add_argument("--lr", type=float, default=3e-3)
+    ap.add_argument("--scales", default="18,19,20,21,22")
+    ap.add_argument("--seed", type=int, default=7)
+    args = ap.parse_args()
+    scales = tuple(int(s) for s in args.scales.split(","))
+
+    ctx = bd.Context(0)
+    t0 = time.time()
+    X, Y, META = [], [], []
+    for g in training_graphs(scales=scales):
+        x, y, meta = log_steps(ctx, g, args.roots)
+        X += x
+        Y += y
+        META += meta
+        print(f"[train] scale {g['scale']} A {g['a']:.2f} seed {g['seed']}: "
+              f"{len(X)} samples ({time.time() - t0:.0f} s)", flush=True)
+    X = np.asarray(X, np.float32)
+    Y = np.asarray(Y, np.float32)
+    loss = ctx.mlp_train(X, Y, epochs=args.epochs, lr=args.lr, seed=args.seed)
+    params, h1, h2 = ctx.mlp_get()
+    pred = ctx.mlp_infer(X, mfma=True)
+    agree = float(np.mean((pred[:, 1] < pred[:, 0]) == (Y[:, 1] < Y[:, 0])))
+    td, bu = np.exp2(Y[:, 0]), np.exp2(Y[:, 1])
+    chosen = np.where(pred[:, 1] < pred[:, 0], bu, td)
+    beamer = np.array([m["td_us"] if m["dir"] == bd.DIR_TD else m["bu_us"] for m in META])
+    best = np.minimum(td, bu)
+    os.makedirs(os.path.dirname(os.path.abspath(args.out)), exist_ok=True)
+    params.astype(np.float32).tofile(args.out)
+    summary = {
+        "samples": int(len(X)), "h1": h1, "h2": h2, "epochs": args.epochs, "lr": args.lr,
+        "final_mse_log2us": loss, "direction_agreement": agree,
+        "train_cost_us": {"learned": float(chosen.sum()), "beamer": float(beamer.sum()),
+                          "best": float(best.sum())},
+        "graphs": {"scales": list(scales), "a": [0.45, 0.50, 0.55, 0.60], "seeds": [101, 202],
+                   "edge_factor": 16, "roots_per_graph": args.roots},
+        "features": ["log2(1+|F|)/log2(1+V)", "log2(1+sum deg F)/log2(1+2M)",
+                     "log2(1+unvisited)/log2(1+V)", "log2(1+sum deg unvisited)/log2(1+2M)",
+                     "min(level,64)/64", "log2(1+max deg F)/log2(1+V)",
+                     "log2(1+2M/V)/8", "previous step bottom-up"],
+        "targets": ["log2 us top-down step", "log2 us bottom-up step"],
+    }
+    with open(os.path.splitext(args.out)[0] + ".json", "w") as f:
+        json.dump(summary, f, indent=1)
+    print(json.dumps(summary))
+
+
+if __name__ == "__main__":
+    main()
