@@ -14,7 +14,10 @@
 //   seg                            segment lengths and per-slot counters
 //   ctl                            step state and direction-policy state
 //
-// One BFS level is one launch of k_step followed by a one-wave k_decide.
+// One BFS level is one launch of k_step.  The first wave of every workgroup
+// rebuilds the state before the step from the previous step's counters and
+// published state (the same result in every workgroup, so none waits for
+// another), and workgroup 0 publishes it with the previous step's record.
 //   Top-down (hybrid_bfs.cc:71-100): blocks take tiles of the queue; per
 //   tile, vertices of degree >= wave_t get one wave each, the rest are
 //   concatenated by a block-wide scan and every thread takes one edge.
@@ -26,11 +29,13 @@
 //   in bm[L%3].  Lists longer than bu_cap are finished by the whole wave.
 //   Reading the depth-L bitmap, not vis, keeps the reference's two-phase rule
 //   (:117-141): a vertex found in this step is never a parent in this step.
-// Both directions emit the next frontier the same way: queue entries staged
-// in LDS and flushed once per block into the block's segment, task slices
-// for high-degree vertices, and the depth-(L+1) bitmap.  k_decide closes the
-// step (record, counters, segment prefix sums) and picks the next direction
-// on the device, so a search is a stream of launches with no host round trip.
+// A top-down step emits the next frontier as queue entries (staged in LDS and
+// flushed once per block into the block's segment), task slices for
+// high-degree vertices, and the depth-(L+1) bitmap; a bottom-up step writes
+// only the bitmap, and a top-down step after it reads the bitmap.  The
+// direction is
+// picked on the device, so a search is a stream of launches with no host
+// round trip.
 #include <math.h>
 #include <stdio.h>
 #include <string.h>
@@ -73,7 +78,7 @@ struct step_args {
     int64_t nv, m, nnz;
     int64_t alpha, beta;
     int32_t level;
-    int32_t gate;          // skip when the search is done or at another level
+    int32_t close_only;    // only publish the state before this step
     int32_t force_dir;     // -1: ctl->dir; 0/1: run this direction (profiling)
     int32_t wave_t, task_t, task_edges;
     int32_t bu_words, bu_cap;
@@ -150,8 +155,9 @@ struct step_smem {
     int use_mlp;
     float x[8];
     unsigned long long red[4][kMaxBlock / 64];
-    int64_t qpref[kSeg + 2];
-    int64_t tpref[kSeg + 2];
+    ctl_block st;  // the search state before this step
+    long long s_new, s_new_deg;
+    int s_dir;
     // top-down tile
     int64_t t_beg[kMaxBlock];
     int64_t t_end[kMaxBlock];
@@ -195,7 +201,7 @@ __device__ void emit_tasks(const step_args &a, bool huge, int32_t u, int64_t b, 
     long long pos = 0, opos = 0, fit = 0;
     if (lane == 0) {
         int64_t ps, ops;
-        fit = seg_reserve(a.seg->tlen[(a.level + 1) & 1], a.tseg_cap, total, &ps, &ops);
+        fit = seg_reserve(a.seg->tlen[(a.level + 1) % 3], a.tseg_cap, total, &ps, &ops);
         pos = ps;
         opos = ops;
     }
@@ -228,7 +234,7 @@ __device__ void emit_tasks(const step_args &a, bool huge, int32_t u, int64_t b, 
                 if (ov < a.task_cap)
                     a.tout[kSeg * a.tseg_cap + ov] = tk;
                 else
-                    atomicAdd(&a.c->task_overflow, 1ULL);
+                    atomicAdd(&a.seg->task_overflow, 1ULL);
             }
         }
     }
@@ -258,7 +264,7 @@ __device__ __forceinline__ void emit_next(const step_args &a, step_smem &s, bool
     if (in_lds < cnt) {
         if (lane == leader) {
             int64_t ps, ops;
-            fit = seg_reserve(a.seg->qlen[(a.level + 1) & 1], a.qseg_cap, cnt - in_lds, &ps, &ops);
+            fit = seg_reserve(a.seg->qlen[(a.level + 1) % 3], a.qseg_cap, cnt - in_lds, &ps, &ops);
             pos = ps;
             opos = ops;
         }
@@ -288,7 +294,7 @@ __device__ void flush_queue(const step_args &a, step_smem &s) {
     if (n > 0) {
         if (threadIdx.x == 0) {
             int64_t pos, opos;
-            s.qfit = (int32_t)seg_reserve(a.seg->qlen[(a.level + 1) & 1], a.qseg_cap, n, &pos,
+            s.qfit = (int32_t)seg_reserve(a.seg->qlen[(a.level + 1) % 3], a.qseg_cap, n, &pos,
                                           &opos);
             s.qpos = pos;
             s.qopos = opos;
@@ -349,13 +355,8 @@ template <int BLOCK>
 __device__ void td_body(const step_args &a, step_smem &s, unsigned long long &cnt,
                         unsigned long long &dsum, unsigned long long &dmax) {
     const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
-    for (int i = tid; i < kSeg + 2; i += BLOCK) {
-        s.qpref[i] = a.c->qpref[i];
-        s.tpref[i] = a.c->tpref[i];
-    }
-    __syncthreads();
-    const int64_t nitems = s.qpref[kSeg + 1];
-    const int64_t ntasks = s.tpref[kSeg + 1];
+    const int64_t nitems = s.st.qpref[kSeg + 1];
+    const int64_t ntasks = s.st.tpref[kSeg + 1];
 
     for (int64_t tile = blockIdx.x; tile * BLOCK < nitems; tile += gridDim.x) {
         if (tid == 0) s.t_nmed = 0;
@@ -363,8 +364,8 @@ __device__ void td_body(const step_args &a, step_smem &s, unsigned long long &cn
         const int64_t slot = tile * BLOCK + tid;
         int32_t v = -1;
         if (slot < nitems) {
-            const int sg = seg_of(s.qpref, slot);
-            v = a.qin[(int64_t)sg * a.qseg_cap + (slot - s.qpref[sg])];
+            const int sg = seg_of(s.st.qpref, slot);
+            v = a.qin[(int64_t)sg * a.qseg_cap + (slot - s.st.qpref[sg])];
         }
         int64_t b = 0, e = 0;
         if (v >= 0) {
@@ -434,8 +435,8 @@ __device__ void td_body(const step_args &a, step_smem &s, unsigned long long &cn
          t0 += (int64_t)gridDim.x * (BLOCK / 64)) {
         const int64_t t = t0 + wid;
         if (t < ntasks) {
-            const int sg = seg_of(s.tpref, t);
-            const td_task tk = a.tin[(int64_t)sg * a.tseg_cap + (t - s.tpref[sg])];
+            const int sg = seg_of(s.st.tpref, t);
+            const td_task tk = a.tin[(int64_t)sg * a.tseg_cap + (t - s.st.tpref[sg])];
             for (int64_t k0 = tk.beg; k0 < tk.end; k0 += 64) {
                 const int64_t k = k0 + lane;
                 const bool valid = k < tk.end;
@@ -677,12 +678,11 @@ __device__ void beamer_next(ctl_block *c, int32_t last_dir, int64_t nnew, int64_
     }
 }
 
-// Choose the direction of step c->level on one wave (no workgroup barrier):
-// lane 0 owns the control block; the learned policy evaluates its network on
-// the wave.
-__device__ void choose_dir(const step_args &a, step_smem &s, int32_t last_dir, int64_t nnew,
-                           int64_t nnew_deg) {
-    ctl_block *c = a.c;
+// Choose the direction of the step described by *c on one wave (no
+// workgroup barrier): lane 0 updates the state; the learned policy evaluates
+// its network on the wave.
+__device__ void choose_dir(const step_args &a, step_smem &s, ctl_block *c, int32_t last_dir,
+                           int64_t nnew, int64_t nnew_deg) {
     if (lane_id() == 0) {
         s.use_mlp = 0;
         switch (a.policy) {
@@ -717,14 +717,14 @@ __device__ void choose_dir(const step_args &a, step_smem &s, int32_t last_dir, i
         // predicted log-costs; a NaN prediction falls back to top-down
         if (lane_id() == 0) c->dir = (y[1] < y[0]) ? BD_DIR_BU : BD_DIR_TD;
     }
+    wave_lds_sync();
 }
 
-// Prefix sums of the segment lengths of list `len` (segment capacities
-// seg_cap, overflow capacity ovf_cap) into pref[0..kSeg+1], then zero
-// `clear_len`.  One wave; lane s handles segment s (lane 0 also the
-// overflow segment).
+// Prefix sums of the segment lengths of a list (segment capacity seg_cap,
+// overflow capacity ovf_cap) into pref[0..kSeg+1].  One wave: lane s takes
+// segment s, lane 63 adds the overflow segment.
 __device__ void seg_prefix(unsigned long long (*len)[kLine], int64_t seg_cap, int64_t ovf_cap,
-                           int64_t *pref, unsigned long long (*clear_len)[kLine]) {
+                           int64_t *pref) {
     const int l = lane_id();
     int64_t n = (int64_t)len[l][0];
     n = n < seg_cap ? n : seg_cap;
@@ -734,50 +734,54 @@ __device__ void seg_prefix(unsigned long long (*len)[kLine], int64_t seg_cap, in
     pref[l + 1] = incl;
     if (l == 0) pref[0] = 0;
     if (l == 63) pref[kSeg + 1] = incl + no;
-    clear_len[l][0] = 0;
-    if (l == 0) clear_len[kSeg][0] = 0;
 }
 
-// Close step a.level: sum the slot counters, write the record, update the
-// totals, publish the next frontier's segment prefix sums, clear the lists
-// the next step fills, and pick the next direction.  Runs on one wave.
-__device__ void close_step(const step_args &a, step_smem &s) {
-    ctl_block *c = a.c;
-    seg_counters *g = a.seg;
+// Rebuild, in s.st, the state before step L = a.level from the published
+// state before step L-1 and the counters of step L-1: close step L-1
+// (record, totals), describe frontier L (segment prefix sums) and choose
+// its direction.  Every workgroup of the step runs this on its first wave
+// and gets the same result, so no workgroup waits for another; `publish`
+// (workgroup 0) also writes the step record.
+__device__ void rebuild_state(const step_args &a, step_smem &s, bool publish) {
+    const int L = a.level;
     const int l = lane_id();
-    __shared__ long long s_new, s_new_deg;
-    __shared__ int s_dir;
-    unsigned long long nn = wave_sum(g->acc[l][0]);
-    unsigned long long nd = wave_sum(g->acc[l][1]);
-    unsigned long long ex = wave_sum(g->acc[l][2]);
-    unsigned long long mx = wave_max(g->acc[l][3]);
-    g->acc[l][0] = 0;
-    g->acc[l][1] = 0;
-    g->acc[l][2] = 0;
-    g->acc[l][3] = 0;
-    const int32_t L = c->level;
+    seg_counters *g = a.seg;
+    {
+        const uint64_t *src = (const uint64_t *)&a.c[(L - 1) & 1];
+        uint64_t *dst = (uint64_t *)&s.st;
+        for (int i = l; i < (int)(sizeof(ctl_block) / 8); i += 64) dst[i] = src[i];
+    }
+    const int pa = (L - 1) % 3;
+    const unsigned long long nn = wave_sum(g->acc[pa][l][0]);
+    const unsigned long long nd = wave_sum(g->acc[pa][l][1]);
+    const unsigned long long ex = wave_sum(g->acc[pa][l][2]);
+    const unsigned long long mx = wave_max(g->acc[pa][l][3]);
+    wave_lds_sync();
+    ctl_block *c = &s.st;
     if (l == 0) {
         const int64_t nnew = (int64_t)nn;
         const int64_t nnew_deg = (int64_t)nd;
         const int32_t dir = c->dir;
-        bd_level_rec r;
-        r.level = L;
-        r.dir = dir;
-        r.frontier = c->frontier;
-        r.frontier_deg = c->frontier_deg;
-        r.unvisited = c->unvisited;
-        r.unvisited_deg = c->unvisited_deg;
-        r.edges_to_check = c->edges_to_check;
-        r.result = dir == BD_DIR_TD ? nnew_deg : nnew;
-        r.next = nnew;
-        r.next_deg = nnew_deg;
-        r.edges_examined = dir == BD_DIR_TD ? c->frontier_deg : (int64_t)ex;
-        r.max_frontier_deg = c->max_frontier_deg;
-        r.tasks = c->tpref[kSeg + 1];
-        r.queued = c->qpref[kSeg + 1];
-        r.ms = -1.0;
-        r.ms_other = -1.0;
-        if (L < a.log_cap) a.log[L] = r;
+        if (publish && L - 1 < a.log_cap) {
+            bd_level_rec r;
+            r.level = L - 1;
+            r.dir = dir;
+            r.frontier = c->frontier;
+            r.frontier_deg = c->frontier_deg;
+            r.unvisited = c->unvisited;
+            r.unvisited_deg = c->unvisited_deg;
+            r.edges_to_check = c->edges_to_check;
+            r.result = dir == BD_DIR_TD ? nnew_deg : nnew;
+            r.next = nnew;
+            r.next_deg = nnew_deg;
+            r.edges_examined = dir == BD_DIR_TD ? c->frontier_deg : (int64_t)ex;
+            r.max_frontier_deg = c->max_frontier_deg;
+            r.tasks = c->tpref[kSeg + 1];
+            r.queued = c->qpref[kSeg + 1];
+            r.ms = -1.0;
+            r.ms_other = -1.0;
+            a.log[L - 1] = r;
+        }
         c->steps += 1;
         if (dir == BD_DIR_TD)
             c->td_steps += 1;
@@ -792,19 +796,45 @@ __device__ void close_step(const step_args &a, step_smem &s) {
         c->max_frontier_deg = (int64_t)mx;
         c->prev_dir = dir;
         c->td_bitmap = dir == BD_DIR_BU ? 1 : 0;
-        c->level = L + 1;
+        c->level = L;
         if (nnew == 0) c->done = 1;
-        s_new = nnew;
-        s_new_deg = nnew_deg;
-        s_dir = dir;
+        s.s_new = nnew;
+        s.s_new_deg = nnew_deg;
+        s.s_dir = dir;
     }
     wave_lds_sync();
-    const int o = (L + 1) & 1, i = L & 1;
-    seg_prefix(g->qlen[o], a.qseg_cap, a.nv, c->qpref, g->qlen[i]);
-    seg_prefix(g->tlen[o], a.tseg_cap, a.task_cap, c->tpref, g->tlen[i]);
+    seg_prefix(g->qlen[L % 3], a.qseg_cap, a.nv, c->qpref);
+    seg_prefix(g->tlen[L % 3], a.tseg_cap, a.task_cap, c->tpref);
     wave_lds_sync();
-    if (s_new > 0) choose_dir(a, s, s_dir, s_new, s_new_deg);
-    if (l == 0) c->t_end = wall_clock64();
+    if (s.s_new > 0) choose_dir(a, s, c, s.s_dir, s.s_new, s.s_new_deg);
+}
+
+// Workgroup 0, first wave: publish the state before step L as copy L&1 (both
+// copies once the search is done, so later launches see it), and clear the
+// counters the next step fills: acc[(L+1)%3] and the segment lengths
+// [(L+2)%3], neither of which anyone reads or writes during step L.
+__device__ void publish_state(const step_args &a, step_smem &s) {
+    const int L = a.level;
+    const int l = lane_id();
+    seg_counters *g = a.seg;
+    const bool done = s.st.done != 0;
+    {
+        const uint64_t *src = (const uint64_t *)&s.st;
+        uint64_t *d0 = (uint64_t *)&a.c[L & 1];
+        uint64_t *d1 = (uint64_t *)&a.c[(L + 1) & 1];
+        for (int i = l; i < (int)(sizeof(ctl_block) / 8); i += 64) {
+            d0[i] = src[i];
+            if (done) d1[i] = src[i];
+        }
+    }
+    for (int k = 0; k < 4; ++k) g->acc[(L + 1) % 3][l][k] = 0;
+    g->qlen[(L + 2) % 3][l][0] = 0;
+    g->tlen[(L + 2) % 3][l][0] = 0;
+    if (l == 0) {
+        g->qlen[(L + 2) % 3][kSeg][0] = 0;
+        g->tlen[(L + 2) % 3][kSeg][0] = 0;
+        if (done) g->t_end = wall_clock64();
+    }
 }
 
 // ---------------------------------------------------------------- kernels
@@ -834,7 +864,7 @@ __device__ void block_flush_counters(const step_args &a, step_smem &s, unsigned 
             exam += s.red[2][w];
             dmax = s.red[3][w] > dmax ? s.red[3][w] : dmax;
         }
-        unsigned long long *slot = a.seg->acc[blockIdx.x % kSlots];
+        unsigned long long *slot = a.seg->acc[a.level % 3][blockIdx.x % kSlots];
         if (cnt) atomicAdd(&slot[0], cnt);
         if (dsum) atomicAdd(&slot[1], dsum);
         if (exam) atomicAdd(&slot[2], exam);
@@ -842,20 +872,35 @@ __device__ void block_flush_counters(const step_args &a, step_smem &s, unsigned 
     }
 }
 
+// One BFS level.  The first wave of every workgroup rebuilds the state
+// before the step (closing the previous step and choosing this step's
+// direction); then the workgroup runs its share of the step.  close_only
+// launches (one workgroup) only publish the state.
 template <int BLOCK>
 __global__ __launch_bounds__(BLOCK) void k_step(step_args a) {
-    ctl_block *c = a.c;
-    if (a.gate && (c->done || c->level != a.level)) return;
-    const int dir = a.force_dir >= 0 ? a.force_dir : c->dir;
+    const int L = a.level;
+    if (L > 0 && a.c[(L - 1) & 1].done) return;
     __shared__ step_smem s;
+    if (threadIdx.x < 64) {
+        if (L == 0) {
+            const uint64_t *src = (const uint64_t *)&a.c[0];
+            uint64_t *dst = (uint64_t *)&s.st;
+            for (int i = threadIdx.x; i < (int)(sizeof(ctl_block) / 8); i += 64) dst[i] = src[i];
+        } else {
+            rebuild_state(a, s, blockIdx.x == 0);
+            if (blockIdx.x == 0) publish_state(a, s);
+        }
+    }
     if (threadIdx.x == 0) s.qn = 0;
     __syncthreads();
+    if (s.st.done || a.close_only) return;
+    const int dir = a.force_dir >= 0 ? a.force_dir : s.st.dir;
     unsigned long long cnt = 0, dsum = 0, exam = 0, dmax = 0;
     if (dir == BD_DIR_TD) {
         for (int64_t i = (int64_t)blockIdx.x * BLOCK + threadIdx.x; i < a.nwords;
              i += (int64_t)gridDim.x * BLOCK)
             a.clr[i] = 0;
-        if (c->td_bitmap)
+        if (s.st.td_bitmap)
             td_bitmap_body<BLOCK>(a, s, cnt, dsum, dmax);
         else
             td_body<BLOCK>(a, s, cnt, dsum, dmax);
@@ -866,84 +911,81 @@ __global__ __launch_bounds__(BLOCK) void k_step(step_args a) {
     block_flush_counters<BLOCK>(a, s, cnt, dsum, exam, dmax);
 }
 
-// Close step a.level (source < 0), or start a search (source >= 0):
-// clear() (hybrid_bfs.cc:433-443) has been done by k_clear; here the source
-// is pushed and marked (hybrid_bfs.cc:157-163) and the first direction
-// chosen.
-__global__ __launch_bounds__(64) void k_decide(step_args a, int64_t source) {
-    ctl_block *c = a.c;
+// Start a search: clear() (hybrid_bfs.cc:433-443) has been done by k_clear;
+// here the counters are zeroed, the source is pushed and marked
+// (hybrid_bfs.cc:157-163), and the first direction chosen.
+__global__ __launch_bounds__(64) void k_init(step_args a, int64_t source) {
     seg_counters *g = a.seg;
     __shared__ step_smem s;
     const int l = lane_id();
-    if (source >= 0) {
-        for (int w = 0; w < 2; ++w)
-            for (int i = l; i <= kSeg; i += 64) {
-                g->qlen[w][i][0] = 0;
-                g->tlen[w][i][0] = 0;
-            }
-        for (int k = 0; k < 4; ++k) g->acc[l][k] = 0;
-        if (l == 0) {
-            const int64_t b = a.off[source], e = a.off[source + 1];
-            const int64_t dsrc = e - b;
-            memset(c, 0, sizeof(ctl_block));
-            c->nv = a.nv;
-            c->m2 = 2 * a.m;
-            c->alpha = a.alpha;
-            c->beta = a.beta;
-            c->policy = a.policy;
-            c->edges_to_check = 2 * a.m;
-            c->scout = dsrc;
-            c->prev_dir = -1;
-            c->frontier = 1;
-            c->frontier_deg = dsrc;
-            c->max_frontier_deg = dsrc;
-            c->unvisited = a.nv - 1;
-            c->unvisited_deg = a.nnz - dsrc;
-            c->reached = 1;
-            c->reached_deg = dsrc;
-            a.vis[source >> 6] |= 1ULL << (source & 63);
-            a.next_bm[source >> 6] = 1ULL << (source & 63);  // bm[0]
-            a.depth[source] = 0;
-            a.parent[source] = (int32_t)source;
-            // frontier 0 lives in segment 0 of list 0
-            if (dsrc >= a.task_t) {
-                const int64_t nt = (dsrc + a.task_edges - 1) / a.task_edges;
-                for (int64_t t = 0; t < nt; ++t) {
-                    td_task tk;
-                    tk.beg = b + t * a.task_edges;
-                    tk.end = tk.beg + a.task_edges < e ? tk.beg + a.task_edges : e;
-                    tk.src = (int32_t)source;
-                    tk.pad = 0;
-                    if (t < a.tseg_cap)
-                        a.tout[t] = tk;
-                    else
-                        a.tout[kSeg * a.tseg_cap + (t - a.tseg_cap)] = tk;
-                }
-                for (int k = 0; k < kSeg + 2; ++k) {
-                    c->qpref[k] = 0;
-                    c->tpref[k] = k == 0 ? 0 : (nt < a.tseg_cap ? nt : a.tseg_cap);
-                }
-                c->tpref[kSeg + 1] = nt;
-            } else {
-                a.qout[0] = (int32_t)source;
-                for (int k = 0; k < kSeg + 2; ++k) {
-                    c->qpref[k] = k == 0 ? 0 : 1;
-                    c->tpref[k] = 0;
-                }
-            }
+    for (int w = 0; w < 3; ++w) {
+        for (int i = l; i <= kSeg; i += 64) {
+            g->qlen[w][i][0] = 0;
+            g->tlen[w][i][0] = 0;
         }
-        wave_lds_sync();
-        // level 0: with no previous step Beamer applies only the outer test
-        // to scout = deg(source) (hybrid_bfs.cc:163-167)
-        choose_dir(a, s, -1, 0, 0);
-        if (l == 0) {
-            __threadfence();
-            c->t_start = wall_clock64();
-        }
-        return;
+        for (int k = 0; k < 4; ++k) g->acc[w][l][k] = 0;
     }
-    if (a.gate && (c->done || c->level != a.level)) return;
-    close_step(a, s);
+    ctl_block *c = &a.c[0];
+    if (l == 0) {
+        g->task_overflow = 0;
+        g->t_end = 0;
+        const int64_t b = a.off[source], e = a.off[source + 1];
+        const int64_t dsrc = e - b;
+        memset(&a.c[0], 0, 2 * sizeof(ctl_block));
+        c->nv = a.nv;
+        c->m2 = 2 * a.m;
+        c->alpha = a.alpha;
+        c->beta = a.beta;
+        c->policy = a.policy;
+        c->edges_to_check = 2 * a.m;
+        c->scout = dsrc;
+        c->prev_dir = -1;
+        c->frontier = 1;
+        c->frontier_deg = dsrc;
+        c->max_frontier_deg = dsrc;
+        c->unvisited = a.nv - 1;
+        c->unvisited_deg = a.nnz - dsrc;
+        c->reached = 1;
+        c->reached_deg = dsrc;
+        a.vis[source >> 6] |= 1ULL << (source & 63);
+        a.next_bm[source >> 6] = 1ULL << (source & 63);  // bm[0]
+        a.depth[source] = 0;
+        a.parent[source] = (int32_t)source;
+        // frontier 0 lives in segment 0 of list 0
+        if (dsrc >= a.task_t) {
+            const int64_t nt = (dsrc + a.task_edges - 1) / a.task_edges;
+            for (int64_t t = 0; t < nt; ++t) {
+                td_task tk;
+                tk.beg = b + t * a.task_edges;
+                tk.end = tk.beg + a.task_edges < e ? tk.beg + a.task_edges : e;
+                tk.src = (int32_t)source;
+                tk.pad = 0;
+                if (t < a.tseg_cap)
+                    a.tout[t] = tk;
+                else
+                    a.tout[kSeg * a.tseg_cap + (t - a.tseg_cap)] = tk;
+            }
+            for (int k = 0; k < kSeg + 2; ++k) {
+                c->qpref[k] = 0;
+                c->tpref[k] = k == 0 ? 0 : (nt < a.tseg_cap ? nt : a.tseg_cap);
+            }
+            c->tpref[kSeg + 1] = nt;
+        } else {
+            a.qout[0] = (int32_t)source;
+            for (int k = 0; k < kSeg + 2; ++k) {
+                c->qpref[k] = k == 0 ? 0 : 1;
+                c->tpref[k] = 0;
+            }
+        }
+    }
+    wave_lds_sync();
+    // level 0: with no previous step Beamer applies only the outer test to
+    // scout = deg(source) (hybrid_bfs.cc:163-167)
+    choose_dir(a, s, c, -1, 0, 0);
+    if (l == 0) {
+        __threadfence();
+        g->t_start = wall_clock64();
+    }
 }
 
 // clear() (hybrid_bfs.cc:433-443) for the device layout.
@@ -964,16 +1006,16 @@ __global__ void k_clear(int32_t *__restrict__ depth, int32_t *__restrict__ paren
     }
 }
 
-// Profiling: undo the first direction's step before timing the other one.
+// Profiling: undo the first direction's step at level L (its counters and
+// the lengths of the list it filled) before timing the other direction.
 __global__ __launch_bounds__(64) void k_reset_step(seg_counters *g, int32_t level) {
     const int l = threadIdx.x;
-    const int o = (level + 1) & 1;
-    for (int k = 0; k < 4; ++k) g->acc[l][k] = 0;
-    g->qlen[o][l][0] = 0;
-    g->tlen[o][l][0] = 0;
+    for (int k = 0; k < 4; ++k) g->acc[level % 3][l][k] = 0;
+    g->qlen[(level + 1) % 3][l][0] = 0;
+    g->tlen[(level + 1) % 3][l][0] = 0;
     if (l == 0) {
-        g->qlen[o][kSeg][0] = 0;
-        g->tlen[o][kSeg][0] = 0;
+        g->qlen[(level + 1) % 3][kSeg][0] = 0;
+        g->tlen[(level + 1) % 3][kSeg][0] = 0;
     }
 }
 
@@ -1034,7 +1076,7 @@ step_args make_args(bd_ctx *c, int32_t level, int64_t alpha, int64_t beta, int p
     a.alpha = alpha;
     a.beta = beta;
     a.level = level;
-    a.gate = 1;
+    a.close_only = 0;
     a.force_dir = -1;
     a.wave_t = std::max(1, c->tune.wave_threshold);
     a.task_t = std::max(64, c->tune.task_threshold);
@@ -1050,9 +1092,9 @@ step_args make_args(bd_ctx *c, int32_t level, int64_t alpha, int64_t beta, int p
     return a;
 }
 
-int launch_step(bd_ctx *c, const step_args &a) {
+int launch_step(bd_ctx *c, const step_args &a, int64_t grid = 0) {
     const int blk = norm_block(c->tune.block);
-    const int64_t grid = (int64_t)c->cu_count * std::max(1, c->tune.grid_per_cu);
+    if (grid <= 0) grid = (int64_t)c->cu_count * std::max(1, c->tune.grid_per_cu);
     switch (blk) {
     case 128:
         hipLaunchKernelGGL(k_step<128>, dim3(grid), dim3(128), 0, c->stream, a);
@@ -1067,8 +1109,8 @@ int launch_step(bd_ctx *c, const step_args &a) {
     return BD_OK;
 }
 
-int launch_decide(bd_ctx *c, const step_args &a, int64_t source) {
-    hipLaunchKernelGGL(k_decide, dim3(1), dim3(64), 0, c->stream, a, source);
+int launch_init(bd_ctx *c, const step_args &a, int64_t source) {
+    hipLaunchKernelGGL(k_init, dim3(1), dim3(64), 0, c->stream, a, source);
     BD_HIP(c, hipGetLastError());
     return BD_OK;
 }
@@ -1096,19 +1138,32 @@ int start_search(bd_ctx *c, int64_t source, int64_t alpha, int64_t beta, int pol
     a.next_bm = c->d_bm[0];
     a.qout = c->d_q[0];
     a.tout = c->d_tasks[0];
-    return launch_decide(c, a, source);
+    return launch_init(c, a, source);
 }
 
+// Copy both published states to the pinned mirror and wait.
 int read_ctl(bd_ctx *c) {
-    BD_HIP(c, hipMemcpyAsync(c->h_ctl, c->d_ctl, sizeof(ctl_block), hipMemcpyDeviceToHost,
+    BD_HIP(c, hipMemcpyAsync(c->h_ctl, c->d_ctl, 2 * sizeof(ctl_block), hipMemcpyDeviceToHost,
                              c->stream));
     BD_HIP(c, hipStreamSynchronize(c->stream));
     return BD_OK;
 }
 
+// The final state: once the search is done both copies hold it.
+const ctl_block *final_ctl(const bd_ctx *c) {
+    return c->h_ctl[0].done ? &c->h_ctl[0] : (c->h_ctl[1].done ? &c->h_ctl[1] : nullptr);
+}
+
 int finish_search(bd_ctx *c, int64_t source, bd_run_stats *out, double event_ms) {
-    const ctl_block &h = *c->h_ctl;
-    if (h.task_overflow) return set_err(c, BD_EHIP, "task list overflow");
+    const ctl_block *hp = final_ctl(c);
+    if (!hp) return set_err(c, BD_EHIP, "search state not final");
+    const ctl_block &h = *hp;
+    struct {
+        unsigned long long task_overflow;
+        uint64_t t_start, t_end;
+    } tail;
+    BD_HIP(c, hipMemcpy(&tail, &c->d_seg->task_overflow, sizeof(tail), hipMemcpyDeviceToHost));
+    if (tail.task_overflow) return set_err(c, BD_EHIP, "task list overflow");
     c->last_steps = h.steps;
     c->last_source = source;
     c->have_result = true;
@@ -1124,14 +1179,17 @@ int finish_search(bd_ctx *c, int64_t source, bd_run_stats *out, double event_ms)
         out->bu_steps = h.bu_steps;
         out->reached = h.reached;
         out->traversed_edges = h.reached_deg / 2;
-        const double dev_ms = (double)(h.t_end - h.t_start) / (double)c->wall_khz;
-        out->ms = (h.t_end >= h.t_start && dev_ms > 0) ? dev_ms : event_ms;
+        const double dev_ms = (double)(tail.t_end - tail.t_start) / (double)c->wall_khz;
+        out->ms = (tail.t_end >= tail.t_start && dev_ms > 0) ? dev_ms : event_ms;
     }
     return BD_OK;
 }
 
 }  // namespace
 
+// A search is k_init, then one k_step launch per level.  Step L publishes
+// the state before it (closing step L-1), so the launch that finds the
+// frontier empty marks the search done and later launches return at once.
 int run_search(bd_ctx *c, int64_t source, int64_t alpha, int64_t beta, int policy,
                bd_run_stats *out) {
     int rc;
@@ -1144,20 +1202,19 @@ int run_search(bd_ctx *c, int64_t source, int64_t alpha, int64_t beta, int polic
     const bool timed = c->tune.time_steps == 1;
     std::vector<double> step_ms;
     if (c->tune.level_driver == 2 || timed) {
-        if ((rc = read_ctl(c))) return rc;
-        while (!c->h_ctl->done) {
-            step_args a = make_args(c, L, alpha, beta, policy);
+        for (;; ++L) {
+            if (L > max_levels) return set_err(c, BD_EHIP, "search did not terminate");
+            const step_args a = make_args(c, L, alpha, beta, policy);
             if (timed) BD_HIP(c, hipEventRecord(c->ev[2], c->stream));
             if ((rc = launch_step(c, a))) return rc;
             if (timed) BD_HIP(c, hipEventRecord(c->ev[3], c->stream));
-            if ((rc = launch_decide(c, a, -1))) return rc;
             if ((rc = read_ctl(c))) return rc;
+            if (c->h_ctl[L & 1].done) break;
             if (timed) {
                 float ms = 0;
                 BD_HIP(c, hipEventElapsedTime(&ms, c->ev[2], c->ev[3]));
                 step_ms.push_back(ms);
             }
-            if (++L > max_levels) return set_err(c, BD_EHIP, "search did not terminate");
         }
     } else {
         int chunk = 8;
@@ -1165,10 +1222,9 @@ int run_search(bd_ctx *c, int64_t source, int64_t alpha, int64_t beta, int polic
             for (int k = 0; k < chunk; ++k, ++L) {
                 const step_args a = make_args(c, L, alpha, beta, policy);
                 if ((rc = launch_step(c, a))) return rc;
-                if ((rc = launch_decide(c, a, -1))) return rc;
             }
             if ((rc = read_ctl(c))) return rc;
-            if (c->h_ctl->done) break;
+            if (final_ctl(c)) break;
             if (L > max_levels) return set_err(c, BD_EHIP, "search did not terminate");
             chunk = std::min(chunk * 2, kMaxLevelsPerSync);
         }
@@ -1333,20 +1389,28 @@ extern "C" int bd_bfs_profile(bd_ctx *c, int64_t source, int64_t alpha, int64_t 
     std::vector<std::pair<double, double>> times;
     std::vector<unsigned long long> acc(kSlots * kLine);
     double total = 0;
-    int32_t L = 0;
-    while (!c->h_ctl->done) {
-        const int32_t chosen = c->h_ctl->dir;
+    for (int32_t L = 0;; ++L) {
+        if (L > c->nv + 2) return set_err(c, BD_EHIP, "search did not terminate");
+        if (L > 0) {
+            // one workgroup closes step L-1 and publishes the state before L
+            step_args a = make_args(c, L, alpha, beta, policy);
+            a.close_only = 1;
+            if ((rc = launch_step(c, a, 1))) return rc;
+            if ((rc = read_ctl(c))) return rc;
+            if (c->h_ctl[L & 1].done) break;
+        }
+        const int32_t chosen = c->h_ctl[L & 1].dir;
         const int32_t other = 1 - chosen;
         float ms[2] = {0, 0};
         unsigned long long found[2] = {0, 0};
         for (int pass = 0; pass < 2; ++pass) {
             step_args a = make_args(c, L, alpha, beta, policy);
-            a.gate = 0;
             a.force_dir = pass == 0 ? other : chosen;
             if (pass == 1) {
                 BD_HIP(c, hipMemcpyAsync(c->d_vis, c->d_vis_save, sizeof(uint64_t) * c->nwords,
                                          hipMemcpyDeviceToDevice, c->stream));
                 hipLaunchKernelGGL(k_reset_step, dim3(1), dim3(64), 0, c->stream, c->d_seg, L);
+                BD_HIP(c, hipGetLastError());
             } else {
                 BD_HIP(c, hipMemcpyAsync(c->d_vis_save, c->d_vis, sizeof(uint64_t) * c->nwords,
                                          hipMemcpyDeviceToDevice, c->stream));
@@ -1358,18 +1422,14 @@ extern "C" int bd_bfs_profile(bd_ctx *c, int64_t source, int64_t alpha, int64_t 
             BD_HIP(c, hipEventRecord(c->ev[3], c->stream));
             BD_HIP(c, hipEventSynchronize(c->ev[3]));
             BD_HIP(c, hipEventElapsedTime(&ms[pass], c->ev[2], c->ev[3]));
-            BD_HIP(c, hipMemcpy(acc.data(), c->d_seg->acc, sizeof(unsigned long long) * acc.size(),
-                                hipMemcpyDeviceToHost));
+            BD_HIP(c, hipMemcpy(acc.data(), c->d_seg->acc[L % 3],
+                                sizeof(unsigned long long) * acc.size(), hipMemcpyDeviceToHost));
             for (int s = 0; s < kSlots; ++s) found[pass] += acc[s * kLine];
         }
         if (found[0] != found[1])
             return set_err(c, BD_EHIP, "profile: directions disagree on the next frontier");
         times.push_back({ms[1], ms[0]});
         total += ms[1];
-        step_args a = make_args(c, L, alpha, beta, policy);
-        if ((rc = launch_decide(c, a, -1))) return rc;
-        if ((rc = read_ctl(c))) return rc;
-        if (++L > c->nv + 2) return set_err(c, BD_EHIP, "search did not terminate");
     }
     if ((rc = finish_search(c, source, out, total))) return rc;
     for (size_t i = 0; i < times.size() && i < c->last_log.size(); ++i) {

@@ -151,9 +151,9 @@ extern "C" int bd_create(bd_ctx **out, int device) {
     if (e == hipSuccess && khz > 0) c->wall_khz = khz;
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
     for (int i = 0; i < 4 && e == hipSuccess; ++i) e = hipEventCreate(&c->ev[i]);
-    if (e == hipSuccess) e = hipMalloc((void **)&c->d_ctl, sizeof(ctl_block));
+    if (e == hipSuccess) e = hipMalloc((void **)&c->d_ctl, 2 * sizeof(ctl_block));
     if (e == hipSuccess) e = hipMalloc((void **)&c->d_seg, sizeof(seg_counters));
-    if (e == hipSuccess) e = hipHostMalloc((void **)&c->h_ctl, sizeof(ctl_block), hipHostMallocDefault);
+    if (e == hipSuccess) e = hipHostMalloc((void **)&c->h_ctl, 2 * sizeof(ctl_block), hipHostMallocDefault);
     if (e == hipSuccess) {
         c->log_cap = 1 << 16;
         e = hipMalloc((void **)&c->d_log, sizeof(bd_level_rec) * c->log_cap);

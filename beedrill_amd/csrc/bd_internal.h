@@ -34,16 +34,27 @@ constexpr int kSeg = 64;
 constexpr int kSlots = 64;
 constexpr int kLine = 16;  // 64-bit words per 128-byte line
 
+// Step L appends to list (L+1)&1 and counts its segments in qlen/tlen
+// [(L+1)%3]; its counters go to acc[L%3].  Three rotating copies of the
+// counters let the step that reads one copy clear another that nobody
+// touches during that step.
 struct seg_counters {
-    unsigned long long qlen[2][kSeg + 1][kLine];
-    unsigned long long tlen[2][kSeg + 1][kLine];
+    unsigned long long qlen[3][kSeg + 1][kLine];
+    unsigned long long tlen[3][kSeg + 1][kLine];
     // per-slot partial accumulators: new, new_deg, examined, max_deg
-    unsigned long long acc[kSlots][kLine];
+    unsigned long long acc[3][kSlots][kLine];
+    unsigned long long task_overflow;
+    uint64_t t_start;  // device timestamps (wall_clock64 ticks)
+    uint64_t t_end;
 };
 
-// Device-resident control block, written by the decide kernel between
-// steps (and read by every workgroup of a step).  qpref/tpref are the
-// prefix sums of the segment lengths of frontier L (index NSEG+1 = total).
+// State of the search before a step: the step's direction and frontier
+// description, the policy state and the totals so far.  Two copies: the
+// step kernel for level L rebuilds the state before step L from copy
+// (L-1)&1 and the counters of step L-1 (every workgroup does it, with the
+// same result) and workgroup 0 publishes it as copy L&1.  qpref/tpref are
+// the prefix sums of the segment lengths of frontier L (index kSeg+1 =
+// total).
 struct ctl_block {
     // run constants
     int64_t nv;
@@ -72,16 +83,12 @@ struct ctl_block {
     int64_t unvisited_deg;
     int64_t qpref[kSeg + 2];
     int64_t tpref[kSeg + 2];
-    unsigned long long task_overflow;
     // totals
     int64_t steps;
     int64_t td_steps;
     int64_t bu_steps;
     int64_t reached;
     int64_t reached_deg;
-    // device timestamps (wall_clock64 ticks)
-    uint64_t t_start;
-    uint64_t t_end;
 };
 
 struct ctx_state;
@@ -115,8 +122,8 @@ struct bd_ctx {
     int64_t qseg_cap = 0;    // entries per queue segment (overflow holds nv)
     int64_t tseg_cap = 0;    // entries per task segment (overflow holds task_cap)
     bd::seg_counters *d_seg = nullptr;
-    bd::ctl_block *d_ctl = nullptr;
-    bd::ctl_block *h_ctl = nullptr;  // pinned mirror
+    bd::ctl_block *d_ctl = nullptr;  // two published copies (see ctl_block)
+    bd::ctl_block *h_ctl = nullptr;  // pinned mirror of both copies
     bd_level_rec *d_log = nullptr;
     int64_t log_cap = 0;
     int32_t *d_script = nullptr;

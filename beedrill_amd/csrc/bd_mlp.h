@@ -19,25 +19,26 @@ __host__ __device__ inline int64_t mlp_nparams(int h1, int h2) {
     return (int64_t)h1 * kMlpIn + h1 + (int64_t)h2 * h1 + h2 + (int64_t)kMlpOut * h2 + kMlpOut;
 }
 
-__host__ __device__ inline float lg1p(double x) { return (float)log2(1.0 + (x > 0 ? x : 0)); }
+__host__ __device__ inline float lg1p(float x) { return log2f(1.0f + (x > 0.f ? x : 0.f)); }
 
 // Step features, all computable before the step from direction-independent
 // quantities: |F|, sum deg(F), max deg(F), unvisited count and degree sum,
 // level, average degree, and whether the previous step was bottom-up (a
-// top-down step then reads F from the bitmap).
+// top-down step then reads F from the bitmap).  Single precision: the
+// features feed a bf16 network and are evaluated once per step on the GPU.
 __host__ __device__ inline void mlp_features_dev(int64_t frontier, int64_t frontier_deg,
                                                  int64_t max_frontier_deg, int64_t unvisited,
                                                  int64_t unvisited_deg, int64_t level, int64_t nv,
                                                  int64_t m, int64_t prev_dir, float *x) {
-    const double lv = log2(1.0 + (double)nv);
-    const double le = log2(1.0 + 2.0 * (double)m);
-    x[0] = (float)(lg1p((double)frontier) / lv);
-    x[1] = (float)(lg1p((double)frontier_deg) / le);
-    x[2] = (float)(lg1p((double)unvisited) / lv);
-    x[3] = (float)(lg1p((double)unvisited_deg) / le);
-    x[4] = (float)((level < 64 ? level : 64) / 64.0);
-    x[5] = (float)(lg1p((double)max_frontier_deg) / lv);
-    x[6] = (float)(lg1p(2.0 * (double)m / (double)(nv > 0 ? nv : 1)) / 8.0);
+    const float lv = lg1p((float)nv);
+    const float le = lg1p(2.0f * (float)m);
+    x[0] = lg1p((float)frontier) / lv;
+    x[1] = lg1p((float)frontier_deg) / le;
+    x[2] = lg1p((float)unvisited) / lv;
+    x[3] = lg1p((float)unvisited_deg) / le;
+    x[4] = (float)(level < 64 ? level : 64) / 64.0f;
+    x[5] = lg1p((float)max_frontier_deg) / lv;
+    x[6] = lg1p(2.0f * (float)m / (float)(nv > 0 ? nv : 1)) / 8.0f;
     x[7] = prev_dir == 1 ? 1.0f : 0.0f;
 }
 

@@ -3,8 +3,9 @@
 The network predicts, for the state before a BFS step, the log2 cost in
 microseconds of a top-down step and of a bottom-up step; the learned policy
 takes the cheaper one (it replaces the predicates at hybrid_bfs.cc:167 and
-:178-179).  Training data are per-step timings the search logs itself
-(bd_bfs_profile times both directions from the same state at every step) on
+:178-179).  Training data are per-step timings the search logs itself (HIP
+events around each step; every step is also run once in the other direction
+inside an otherwise identical search, see log_steps) on
 seeded RMAT graphs of scales 18-22 with A in [0.45, 0.60] (B = C =
 (0.95 - A) / 2, D = 0.05), on generator seeds disjoint from the benchmark's
 (seed 0).  Training is full-batch Adam on the device (bd_mlp_train).
@@ -44,27 +45,58 @@ def training_graphs(scales=(18, 19, 20, 21, 22), a_values=(0.45, 0.50, 0.55, 0.6
                     yield dict(scale=scale, a=a, b=bc, c=bc, d=0.05, edge_factor=ef, seed=seed)
 
 
+def _log2us(ms):
+    return math.log2(max(ms * 1e3, 1.0))
+
+
 def log_steps(ctx, graph, roots_per_graph):
-    """Profile `roots_per_graph` lcg(0) roots; one sample per step."""
+    """Per-step costs of both directions, measured in ordinary searches.
+
+    Each root is searched once with Beamer's choices, then once per step with
+    only that step's direction flipped (the frontier sequence does not depend
+    on the directions, so every run visits the same states).  Step times come
+    from HIP events around the step kernel with the caches as the previous
+    step left them.  A flipped run also times the following step after a
+    step of the other direction: that sample has one known target (masked).
+    """
     ctx.generate_rmat(graph["scale"], graph["edge_factor"], graph["a"], graph["b"], graph["c"],
                       graph["d"], seed=graph["seed"])
     nv, _, m = ctx.info()
-    xs, ys, meta = [], [], []
-    for root in ctx.pick_roots(roots_per_graph):
-        ctx.profile(int(root), 15, 18, "beamer")
-        prev = -1
-        for rec in ctx.levels():
-            t_chosen, t_other = rec["ms"] * 1e3, rec["ms_other"] * 1e3
-            if rec["dir"] == bd.DIR_TD:
-                t_td, t_bu = t_chosen, t_other
-            else:
-                t_td, t_bu = t_other, t_chosen
-            xs.append(bd.features(rec, nv, m, prev))
-            ys.append([math.log2(max(t_td, 1.0)), math.log2(max(t_bu, 1.0))])
-            meta.append(dict(root=int(root), level=rec["level"], dir=rec["dir"], td_us=t_td,
-                             bu_us=t_bu, **{k: graph[k] for k in ("scale", "a", "seed")}))
-            prev = rec["dir"]
-    return xs, ys, meta
+    xs, ys, masks, meta = [], [], [], []
+    tune = ctx.get_tuning()
+    ctx.set_tuning(time_steps=1)
+    try:
+        for root in ctx.pick_roots(roots_per_graph):
+            root = int(root)
+            ctx.bfs(root, 15, 18, "beamer")
+            base = ctx.levels()
+            dirs = [r["dir"] for r in base]
+            for L, rec in enumerate(base):
+                script = list(dirs)
+                script[L] = 1 - script[L]
+                ctx.set_script(np.array(script, np.int32))
+                ctx.bfs(root, 15, 18, "script")
+                flip = ctx.levels()
+                t = {rec["dir"]: rec["ms"], 1 - rec["dir"]: flip[L]["ms"]}
+                prev = dirs[L - 1] if L > 0 else -1
+                xs.append(bd.features(rec, nv, m, prev))
+                ys.append([_log2us(t[bd.DIR_TD]), _log2us(t[bd.DIR_BU])])
+                masks.append([1.0, 1.0])
+                meta.append(dict(root=root, level=L, dir=rec["dir"], td_us=t[bd.DIR_TD] * 1e3,
+                                 bu_us=t[bd.DIR_BU] * 1e3,
+                                 **{k: graph[k] for k in ("scale", "a", "seed")}))
+                if L + 1 < len(flip):
+                    nxt = flip[L + 1]
+                    y = [0.0, 0.0]
+                    mk = [0.0, 0.0]
+                    y[nxt["dir"]] = _log2us(nxt["ms"])
+                    mk[nxt["dir"]] = 1.0
+                    xs.append(bd.features(base[L + 1], nv, m, script[L]))
+                    ys.append(y)
+                    masks.append(mk)
+    finally:
+        ctx.set_tuning(time_steps=tune["time_steps"])
+    return xs, ys, masks, meta
 
 
 def main():
@@ -80,28 +112,33 @@ def main():
 
     ctx = bd.Context(0)
     t0 = time.time()
-    X, Y, META = [], [], []
+    X, Y, MASK, META = [], [], [], []
     for g in training_graphs(scales=scales):
-        x, y, meta = log_steps(ctx, g, args.roots)
+        x, y, mk, meta = log_steps(ctx, g, args.roots)
         X += x
         Y += y
+        MASK += mk
         META += meta
         print(f"[train] scale {g['scale']} A {g['a']:.2f} seed {g['seed']}: "
               f"{len(X)} samples ({time.time() - t0:.0f} s)", flush=True)
     X = np.asarray(X, np.float32)
     Y = np.asarray(Y, np.float32)
-    loss = ctx.mlp_train(X, Y, epochs=args.epochs, lr=args.lr, seed=args.seed)
+    MASK = np.asarray(MASK, np.float32)
+    loss = ctx.mlp_train(X, Y, MASK, epochs=args.epochs, lr=args.lr, seed=args.seed)
     params, h1, h2 = ctx.mlp_get()
-    pred = ctx.mlp_infer(X, mfma=True)
-    agree = float(np.mean((pred[:, 1] < pred[:, 0]) == (Y[:, 1] < Y[:, 0])))
-    td, bu = np.exp2(Y[:, 0]), np.exp2(Y[:, 1])
+    full = MASK.min(axis=1) > 0
+    Xf, Yf = X[full], Y[full]
+    pred = ctx.mlp_infer(Xf, mfma=True)
+    agree = float(np.mean((pred[:, 1] < pred[:, 0]) == (Yf[:, 1] < Yf[:, 0])))
+    td, bu = np.exp2(Yf[:, 0]), np.exp2(Yf[:, 1])
     chosen = np.where(pred[:, 1] < pred[:, 0], bu, td)
     beamer = np.array([m["td_us"] if m["dir"] == bd.DIR_TD else m["bu_us"] for m in META])
     best = np.minimum(td, bu)
     os.makedirs(os.path.dirname(os.path.abspath(args.out)), exist_ok=True)
     params.astype(np.float32).tofile(args.out)
     summary = {
-        "samples": int(len(X)), "h1": h1, "h2": h2, "epochs": args.epochs, "lr": args.lr,
+        "samples": int(len(X)), "fully_labelled": int(full.sum()), "h1": h1, "h2": h2,
+        "epochs": args.epochs, "lr": args.lr,
         "final_mse_log2us": loss, "direction_agreement": agree,
         "train_cost_us": {"learned": float(chosen.sum()), "beamer": float(beamer.sum()),
                           "best": float(best.sum())},

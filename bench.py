@@ -123,6 +123,7 @@ def main():
     ap.add_argument("--roofline-roots", type=int, default=8)
     ap.add_argument("--cpu-budget", type=float, default=20.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--model", default=MODEL, help="direction network parameters (float32)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only; --gpus {args.gpus} is not supported "
@@ -144,9 +145,9 @@ def main():
 
     results = {"beamer": run_policy(ctx, roots, "beamer", args.warmup)}
     log(f"beamer: {results['beamer']['hmean_gteps']:.2f} GTEPS (harmonic mean)")
-    have_model = os.path.exists(MODEL)
+    have_model = os.path.exists(args.model)
     if have_model:
-        ctx.mlp_set(np.fromfile(MODEL, np.float32), 32, 32)
+        ctx.mlp_set(np.fromfile(args.model, np.float32), 32, 32)
         results["learned"] = run_policy(ctx, roots, "learned", args.warmup)
         log(f"learned: {results['learned']['hmean_gteps']:.2f} GTEPS (harmonic mean)")
     head = results["learned"] if have_model else results["beamer"]
