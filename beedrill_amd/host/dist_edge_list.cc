@@ -4,7 +4,7 @@
 dist_edge_list::handle dist_edge_list::load_binary(const char *filename, int device) {
     printf("Opening %s...\n", filename);
     fflush(stdout);
-    handle h(new dist_edge_list());
+    handle h = emu::make_repl_shallow<dist_edge_list>();
     h->ctx_ = bd_open(device);
     bd_check(h->ctx_.get(), bd_graph_load_el64(h->ctx_.get(), filename), "load_binary");
     int64_t nv = 0, nnz = 0, m = 0;

@@ -14,6 +14,7 @@
 #include <vector>
 
 #include "beedrill_amd.h"
+#include "emu_shallow.h"
 
 // Shared owner of one GPU context (device, stream, resident graph).
 using bd_handle = std::shared_ptr<bd_ctx>;
@@ -24,7 +25,11 @@ void bd_check(bd_ctx *c, int rc, const char *what);
 // dist_edge_list (dist_edge_list.h:15-89).  load_binary reads an el64 file
 // (dist_edge_list.cc:50-143); the edge list stays resident on the GPU.
 struct dist_edge_list {
-    using handle = std::unique_ptr<dist_edge_list>;
+    using handle = std::unique_ptr<emu::repl_shallow<dist_edge_list>>;
+    dist_edge_list() = default;
+    // dist_edge_list.h:40-45: shares the other's device edge list.
+    dist_edge_list(const dist_edge_list &other, emu::shallow_copy)
+        : ctx_(other.ctx_), nv_(other.nv_), m_(other.m_) {}
     static handle load_binary(const char *filename, int device = 0);
     long num_vertices() const { return nv_; }
     long num_edges() const { return m_; }
@@ -47,6 +52,8 @@ struct dist_edge_list {
 class graph {
    public:
     explicit graph(bd_handle ctx);
+    // graph_base.h:54-60: shares the other's device CSR.
+    graph(const graph &other, emu::shallow_copy) : graph(other.ctx_) {}
     long num_vertices() const { return nv_; }
     long num_edges() const { return m_; }
     long out_degree(long v) const;
@@ -72,6 +79,6 @@ class graph {
 // graph_base.h:335-436.  The device CSR was built when the edge list was
 // loaded; the graph shares that context.
 template <class Graph>
-std::unique_ptr<Graph> create_graph_from_edge_list(dist_edge_list &el) {
-    return std::unique_ptr<Graph>(new Graph(el.ctx()));
+std::unique_ptr<emu::repl_shallow<Graph>> create_graph_from_edge_list(dist_edge_list &el) {
+    return emu::make_repl_shallow<Graph>(el.ctx());
 }

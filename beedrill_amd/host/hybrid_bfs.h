@@ -11,6 +11,10 @@
 class hybrid_bfs {
    public:
     explicit hybrid_bfs(graph &g);
+    // hybrid_bfs.cc:24-32: a copy sharing the other's graph and device state
+    // (emu::make_repl_shallow<hybrid_bfs>(g), hybrid_bfs_main.cc:246).
+    hybrid_bfs(const hybrid_bfs &other, emu::shallow_copy)
+        : g_(other.g_), policy_(other.policy_), stats_(other.stats_), have_(other.have_) {}
 
     // hybrid_bfs.cc:197-213: top-down steps only.
     void run_with_migrating_threads(long source);

@@ -179,7 +179,7 @@ int main(int argc, char **argv) {
     const uint64_t lcg_mul = 6364136223846793005ULL;
     uint64_t lcg_state = 1;  // lcg rng(0) (hybrid_bfs_main.cc:159)
 
-    std::unique_ptr<graph> g;
+    std::unique_ptr<emu::repl_shallow<graph>> g;
     try {
         auto t0 = std::chrono::steady_clock::now();
         dist_edge_list::handle el;
@@ -196,7 +196,7 @@ int main(int argc, char **argv) {
             bd_check(ctx.get(), bd_graph_generate(ctx.get(), BD_GEN_RMAT, nv, ne, abcd[0], abcd[1],
                                                   abcd[2], abcd[3], args.seed, &m),
                      "generate");
-            g.reset(new graph(ctx));
+            g = emu::make_repl_shallow<graph>(ctx);
         } else {
             el = dist_edge_list::load_binary(args.graph_filename, args.device);
             g = create_graph_from_edge_list<graph>(*el);
@@ -244,8 +244,8 @@ int main(int argc, char **argv) {
             exit(1);
         }
         LOG("Initializing BFS data structures...\n");
-        hybrid_bfs bfs(*g);
-        bfs.set_policy(args.policy);
+        auto bfs = emu::make_repl_shallow<hybrid_bfs>(*g);
+        bfs->set_policy(args.policy);
         if (args.mlp_weights) {
             FILE *fp = fopen(args.mlp_weights, "rb");
             if (!fp) {
@@ -276,28 +276,28 @@ int main(int argc, char **argv) {
                     source = (long)(lcg_state % (uint64_t)g->num_vertices());
                 } while (g->out_degree(source) == 0);
             }
-            bfs.clear();
+            bfs->clear();
             LOG("Doing breadth-first search from vertex %li (sample %li of %li)\n", source, s + 1,
                 args.num_trials);
             switch (alg) {
-            case REMOTE_WRITES: bfs.run_with_remote_writes(source); break;
-            case MIGRATING_THREADS: bfs.run_with_migrating_threads(source); break;
+            case REMOTE_WRITES: bfs->run_with_remote_writes(source); break;
+            case MIGRATING_THREADS: bfs->run_with_migrating_threads(source); break;
             case REMOTE_WRITES_HYBRID:
-                bfs.run_with_remote_writes_hybrid(source, args.alpha, args.beta);
+                bfs->run_with_remote_writes_hybrid(source, args.alpha, args.beta);
                 break;
-            case BEAMER_HYBRID: bfs.run_beamer(source, args.alpha, args.beta); break;
+            case BEAMER_HYBRID: bfs->run_beamer(source, args.alpha, args.beta); break;
             }
-            const double time_ms = bfs.last_stats().ms;
+            const double time_ms = bfs->last_stats().ms;
             if (args.check_results) {
                 LOG("Checking results...\n");
-                if (bfs.check(source)) {
+                if (bfs->check(source)) {
                     LOG("PASS\n");
                 } else {
                     LOG("FAIL\n");
                     success = false;
                 }
             }
-            const long traversed = bfs.count_num_traversed_edges();
+            const long traversed = bfs->count_num_traversed_edges();
             edges_all += traversed;
             ms_all += time_ms;
             teps.push_back(traversed / (time_ms / 1000));
