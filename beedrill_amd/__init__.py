@@ -52,13 +52,16 @@ class RunStats(ctypes.Structure):
         return {f: getattr(self, f) for f, _ in self._fields_}
 
 
+TUNING_FIELDS = ("block", "grid_per_cu", "wave_threshold", "task_threshold", "task_edges",
+                 "bu_words", "bu_cap", "level_driver", "time_steps", "bu_head", "step_kernels",
+                 "bu_grid_per_cu")
+
+
 class Tuning(ctypes.Structure):
-    _fields_ = [(n, ctypes.c_int32) for n in (
-        "block", "grid_per_cu", "wave_threshold", "task_threshold", "task_edges",
-        "bu_words", "bu_cap", "level_driver", "time_steps", "reserved")]
+    _fields_ = [(n, ctypes.c_int32) for n in TUNING_FIELDS] + [("reserved", ctypes.c_int32 * 4)]
 
     def as_dict(self):
-        return {f: getattr(self, f) for f, _ in self._fields_ if f != "reserved"}
+        return {f: getattr(self, f) for f in TUNING_FIELDS}
 
 
 class BeedrillError(RuntimeError):

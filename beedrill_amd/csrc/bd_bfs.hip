@@ -57,6 +57,7 @@ constexpr int kMaxBlock = 512;
 struct step_args {
     const int64_t *off;
     const int32_t *cols;
+    const int32_t *head;   // bottom-up head index (neighbour of largest degree)
     uint64_t *vis;
     const uint64_t *front;
     uint64_t *next_bm;
@@ -81,7 +82,7 @@ struct step_args {
     int32_t close_only;    // only publish the state before this step
     int32_t force_dir;     // -1: ctl->dir; 0/1: run this direction (profiling)
     int32_t wave_t, task_t, task_edges;
-    int32_t bu_words, bu_cap;
+    int32_t bu_words, bu_cap, bu_head;
     int32_t policy, script_len, log_cap;
     int32_t mlp_h1, mlp_h2, has_mlp;
 };
@@ -142,32 +143,38 @@ __device__ __forceinline__ int seg_of(const int64_t *pref, int64_t i) {
     return s;
 }
 
-// Per-workgroup LDS state shared by both directions.  The network scratch of
-// the direction decision reuses the queue staging area.
+// Per-workgroup LDS state.  The network scratch of the direction choice
+// (used before the step body starts), the top-down staging and tile arrays
+// and the bottom-up found words and pending lists share one union; it is
+// sized by the workgroup so that 8 workgroups of 256 fit in a CU's LDS.
+template <int BLOCK>
 struct step_smem {
-    union {
-        int32_t q[kQCap];
-        mlp_scratch mlp;
-    };
-    int32_t qn;
-    int64_t qpos, qopos;
-    int32_t qfit;
-    int use_mlp;
-    float x[8];
-    unsigned long long red[4][kMaxBlock / 64];
     ctl_block st;  // the search state before this step
     long long s_new, s_new_deg;
     int s_dir;
-    // top-down tile
-    int64_t t_beg[kMaxBlock];
-    int64_t t_end[kMaxBlock];
-    int32_t t_src[kMaxBlock];
-    int32_t t_pref[kMaxBlock + 1];
-    int32_t t_med[kMaxBlock];
-    int32_t t_wsum[kMaxBlock / 64];
-    int32_t t_nmed;
-    // bottom-up: found bits of the chunk each wave owns
-    uint64_t b_found[kMaxBlock / 64][64];
+    int use_mlp;
+    float x[8];
+    unsigned long long red[4][BLOCK / 64];
+    int32_t qn;
+    int64_t qpos, qopos;
+    int32_t qfit;
+    union {
+        mlp_scratch mlp;
+        struct {  // top-down
+            int32_t q[kQCap];
+            int64_t t_beg[BLOCK];
+            int64_t t_end[BLOCK];
+            int32_t t_src[BLOCK];
+            int32_t t_pref[BLOCK + 1];
+            int32_t t_med[BLOCK];
+            int32_t t_wsum[BLOCK / 64];
+            int32_t t_nmed;
+        };
+        struct {  // bottom-up: found bits of each wave's chunk, head misses
+            uint64_t b_found[BLOCK / 64][64];
+            int32_t b_pend[BLOCK / 64][128];
+        };
+    };
 };
 
 // Reserve n slots of the output list for this workgroup's segment; returns
@@ -243,7 +250,8 @@ __device__ void emit_tasks(const step_args &a, bool huge, int32_t u, int64_t b, 
 // Emit vertex u of the next frontier: high-degree vertices become task
 // slices, the others are staged in LDS.  Every lane of the wave must call
 // this together (emit = false for idle lanes).
-__device__ __forceinline__ void emit_next(const step_args &a, step_smem &s, bool emit, int32_t u,
+template <int BLOCK>
+__device__ __forceinline__ void emit_next(const step_args &a, step_smem<BLOCK> &s, bool emit, int32_t u,
                                           int64_t b, int64_t e) {
     const bool huge = emit && (e - b) >= a.task_t;
     if (__ballot(huge)) emit_tasks(a, huge, u, b, e);
@@ -288,7 +296,7 @@ __device__ __forceinline__ void emit_next(const step_args &a, step_smem &s, bool
 // Write the staged queue entries to this workgroup's segment (whole
 // workgroup).
 template <int BLOCK>
-__device__ void flush_queue(const step_args &a, step_smem &s) {
+__device__ void flush_queue(const step_args &a, step_smem<BLOCK> &s) {
     __syncthreads();
     const int n = s.qn < kQCap ? s.qn : kQCap;
     if (n > 0) {
@@ -321,7 +329,8 @@ __device__ void flush_queue(const step_args &a, step_smem &s) {
 // with a device-scope load, which bypasses the CU's L1: a stale L1 copy would
 // send every later edge into a popular vertex to the atomic, and atomics on
 // one word serialize (~12 ns each).
-__device__ __forceinline__ void td_visit(const step_args &a, step_smem &s, bool valid, int32_t u,
+template <int BLOCK>
+__device__ __forceinline__ void td_visit(const step_args &a, step_smem<BLOCK> &s, bool valid, int32_t u,
                                          int32_t src, unsigned long long &cnt,
                                          unsigned long long &dsum, unsigned long long &dmax) {
     bool won = false;
@@ -352,7 +361,7 @@ __device__ __forceinline__ void td_visit(const step_args &a, step_smem &s, bool 
 }
 
 template <int BLOCK>
-__device__ void td_body(const step_args &a, step_smem &s, unsigned long long &cnt,
+__device__ void td_body(const step_args &a, step_smem<BLOCK> &s, unsigned long long &cnt,
                         unsigned long long &dsum, unsigned long long &dmax) {
     const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
     const int64_t nitems = s.st.qpref[kSeg + 1];
@@ -455,7 +464,7 @@ __device__ void td_body(const step_args &a, step_smem &s, unsigned long long &cn
 // vertices at a time: short lists are walked by their own lane in lockstep,
 // lists of degree >= wave_t by the whole wave.
 template <int BLOCK>
-__device__ void td_bitmap_body(const step_args &a, step_smem &s, unsigned long long &cnt,
+__device__ void td_bitmap_body(const step_args &a, step_smem<BLOCK> &s, unsigned long long &cnt,
                                unsigned long long &dsum, unsigned long long &dmax) {
     const int lane = lane_id(), wid = threadIdx.x >> 6;
     const int CW = a.bu_words;
@@ -559,76 +568,112 @@ __device__ __forceinline__ int32_t bu_scan(const step_args &a, int64_t *kp, int6
     return -1;
 }
 
+// Scan the lists of the wave's pending vertices (those whose head index is
+// not in the frontier): lane i takes pend[i], i < n, and tests its list in
+// CSR order; lists longer than bu_cap are finished by the whole wave.  Found
+// vertices get depth and parent and their bit in the wave's found words
+// (chunk word (v >> 6) - w0).
 template <int BLOCK>
-__device__ void bu_body(const step_args &a, step_smem &s, unsigned long long &cnt,
+__device__ void bu_pending(const step_args &a, step_smem<BLOCK> &s, const int32_t *pend, int n,
+                           int64_t w0, unsigned long long &cnt, unsigned long long &dsum,
+                           unsigned long long &dmax, unsigned long long &exam) {
+    const int lane = lane_id(), wid = threadIdx.x >> 6;
+    const bool valid = lane < n;
+    const int64_t v = valid ? pend[lane] : 0;
+    int32_t par = -1;
+    int64_t b = 0, e = 0, k = 0;
+    if (valid) {
+        b = a.off[v];
+        e = a.off[v + 1];
+        k = b;
+        par = bu_scan(a, &k, e, a.bu_cap, exam);
+    }
+    uint64_t lm = __ballot(valid && par < 0 && k < e);
+    while (lm) {
+        const int src_lane = __ffsll((unsigned long long)lm) - 1;
+        lm &= lm - 1;
+        const int64_t kb = __shfl(k, src_lane), ke = __shfl(e, src_lane);
+        int32_t hit = -1;
+        for (int64_t k0 = kb; k0 < ke && hit < 0; k0 += 64) {
+            const int64_t kk = k0 + lane;
+            const int32_t u = kk < ke ? a.cols[kk] : -1;
+            const bool h = u >= 0 && ((a.front[u >> 6] >> (u & 63)) & 1);
+            const uint64_t hm = __ballot(h);
+            const int64_t rest = ke - k0 < 64 ? ke - k0 : 64;
+            const int64_t seen = hm ? (int64_t)__ffsll((unsigned long long)hm) : rest;
+            if (lane == src_lane) exam += (unsigned long long)seen;
+            if (hm) hit = __shfl(u, __ffsll((unsigned long long)hm) - 1);
+        }
+        if (lane == src_lane) par = hit;
+    }
+    if (par >= 0) {
+        a.depth[v] = a.level + 1;
+        a.parent[v] = par;
+        const unsigned long long d = (unsigned long long)(e - b);
+        ++cnt;
+        dsum += d;
+        dmax = d > dmax ? d : dmax;
+        atomicOr((unsigned long long *)&s.b_found[wid][(v >> 6) - w0],
+                 (unsigned long long)(1ULL << (v & 63)));
+    }
+}
+
+// Bottom-up step.  Each wave owns chunks of bu_words visited words and ranks
+// their unvisited vertices with a wave prefix sum; per round of 64 vertices
+// each lane reads its vertex's head index (the neighbour most likely to be
+// in the frontier, read from a dense array in vertex order) and tests it.
+// Vertices whose head misses are queued in LDS and their lists scanned 64 at
+// a time, so the rare long scans do not hold up every round.  Found bits are
+// gathered in LDS and written by the word owner: no global atomics.
+template <int BLOCK>
+__device__ void bu_body(const step_args &a, step_smem<BLOCK> &s, unsigned long long &cnt,
                         unsigned long long &dsum, unsigned long long &dmax,
                         unsigned long long &exam) {
     const int lane = lane_id(), wid = threadIdx.x >> 6;
     const int CW = a.bu_words;
     const int64_t nchunks = (a.nwords + CW - 1) / CW;
-    for (int64_t c0 = (int64_t)blockIdx.x * (BLOCK / 64); c0 < nchunks;
-         c0 += (int64_t)gridDim.x * (BLOCK / 64)) {
-        const int64_t chunk = c0 + wid;
-        if (chunk < nchunks) {
-            const int64_t w0 = chunk * CW;
-            const int64_t wi = w0 + lane;
-            const bool own = lane < CW && wi < a.nwords;
-            const uint64_t vw = own ? a.vis[wi] : ~0ULL;
-            if (own) a.clr[wi] = 0;
-            const uint64_t unv = ~vw;
-            const int cntw = __popcll(unv);
-            int excl = (int)wave_incl_scan(cntw);
-            const int total = __shfl(excl, 63);
-            excl -= cntw;
-            s.b_found[wid][lane] = 0;
-            wave_lds_sync();
-            for (int r = 0; r < total; r += 64) {
-                const int g = r + lane;
-                const bool valid = g < total;
-                const int gg = valid ? g : total - 1;
-                // word j holding the gg-th unvisited vertex: largest j with excl_j <= gg
-                int j = 0;
+    const uint64_t below = (1ULL << lane) - 1;
+    int32_t *pend = s.b_pend[wid];
+    for (int64_t chunk = (int64_t)blockIdx.x * (BLOCK / 64) + wid; chunk < nchunks;
+         chunk += (int64_t)gridDim.x * (BLOCK / 64)) {
+        const int64_t w0 = chunk * CW;
+        const int64_t wi = w0 + lane;
+        const bool own = lane < CW && wi < a.nwords;
+        const uint64_t vw = own ? a.vis[wi] : ~0ULL;
+        if (own) a.clr[wi] = 0;
+        const uint64_t unv = ~vw;
+        const int cntw = __popcll(unv);
+        int excl = (int)wave_incl_scan(cntw);
+        const int total = __shfl(excl, 63);
+        excl -= cntw;
+        s.b_found[wid][lane] = 0;
+        int np = 0;  // pending vertices (wave-uniform)
+        wave_lds_sync();
+        for (int r = 0; r < total; r += 64) {
+            const int g = r + lane;
+            const bool valid = g < total;
+            const int gg = valid ? g : total - 1;
+            // word j holding the gg-th unvisited vertex: largest j with excl_j <= gg
+            int j = 0;
 #pragma unroll
-                for (int st = 32; st > 0; st >>= 1) {
-                    const int mid = j + st;
-                    const int em = __shfl(excl, mid < 64 ? mid : 63);
-                    if (mid < 64 && em <= gg) j = mid;
-                }
-                const uint64_t bits = (uint64_t)__shfl((long long)unv, j);
-                const int ej = __shfl(excl, j);
-                const int bpos = select_bit(bits, gg - ej);
-                const int64_t v = (w0 + j) * 64 + bpos;
-                int32_t par = -1;
-                int64_t b = 0, e = 0, k = 0;
-                if (valid) {
-                    b = a.off[v];
-                    e = a.off[v + 1];
-                    k = b;
-                    par = bu_scan(a, &k, e, a.bu_cap, exam);
-                }
-                // lists longer than bu_cap: the whole wave finishes them
-                uint64_t lm = __ballot(valid && par < 0 && k < e);
-                while (lm) {
-                    const int src_lane = __ffsll((unsigned long long)lm) - 1;
-                    lm &= lm - 1;
-                    const int64_t kb = __shfl(k, src_lane), ke = __shfl(e, src_lane);
-                    int32_t hit = -1;
-                    for (int64_t k0 = kb; k0 < ke && hit < 0; k0 += 64) {
-                        const int64_t kk = k0 + lane;
-                        const int32_t u = kk < ke ? a.cols[kk] : -1;
-                        const bool h = u >= 0 && ((a.front[u >> 6] >> (u & 63)) & 1);
-                        const uint64_t hm = __ballot(h);
-                        const int64_t rest = ke - k0 < 64 ? ke - k0 : 64;
-                        const int64_t seen = hm ? (int64_t)__ffsll((unsigned long long)hm) : rest;
-                        if (lane == src_lane) exam += (unsigned long long)seen;
-                        if (hm) hit = __shfl(u, __ffsll((unsigned long long)hm) - 1);
-                    }
-                    if (lane == src_lane) par = hit;
-                }
-                const bool found = par >= 0;
-                if (found) {
+            for (int st = 32; st > 0; st >>= 1) {
+                const int mid = j + st;
+                const int em = __shfl(excl, mid < 64 ? mid : 63);
+                if (mid < 64 && em <= gg) j = mid;
+            }
+            const uint64_t bits = (uint64_t)__shfl((long long)unv, j);
+            const int ej = __shfl(excl, j);
+            const int bpos = select_bit(bits, gg - ej);
+            const int64_t v = (w0 + j) * 64 + bpos;
+            bool miss = valid;
+            if (valid && a.bu_head) {
+                const int32_t h = a.head[v];
+                const int64_t b = a.off[v], e = a.off[v + 1];
+                ++exam;
+                if (h >= 0 && ((a.front[h >> 6] >> (h & 63)) & 1)) {
+                    miss = false;
                     a.depth[v] = a.level + 1;
-                    a.parent[v] = par;
+                    a.parent[v] = h;
                     const unsigned long long d = (unsigned long long)(e - b);
                     ++cnt;
                     dsum += d;
@@ -637,15 +682,31 @@ __device__ void bu_body(const step_args &a, step_smem &s, unsigned long long &cn
                              (unsigned long long)(1ULL << bpos));
                 }
             }
-            wave_lds_sync();
-            if (own) {
-                const uint64_t f = s.b_found[wid][lane];
-                a.next_bm[wi] = f;
-                if (f) a.vis[wi] = vw | f;
+            const uint64_t mm = __ballot(miss);
+            if (miss) pend[np + __popcll(mm & below)] = (int32_t)v;
+            np += __popcll(mm);
+            if (np >= 64) {
+                wave_lds_sync();
+                bu_pending(a, s, pend, 64, w0, cnt, dsum, dmax, exam);
+                const int rest = np - 64;
+                const int32_t t = lane < rest ? pend[64 + lane] : 0;
+                wave_lds_sync();
+                if (lane < rest) pend[lane] = t;
+                np = rest;
+                wave_lds_sync();
             }
         }
-        __syncthreads();
-        if (s.qn >= kQCap / 2) flush_queue<BLOCK>(a, s);
+        if (np > 0) {
+            wave_lds_sync();
+            bu_pending(a, s, pend, np, w0, cnt, dsum, dmax, exam);
+        }
+        wave_lds_sync();
+        if (own) {
+            const uint64_t f = s.b_found[wid][lane];
+            a.next_bm[wi] = f;
+            if (f) a.vis[wi] = vw | f;
+        }
+        wave_lds_sync();
     }
 }
 
@@ -681,7 +742,8 @@ __device__ void beamer_next(ctl_block *c, int32_t last_dir, int64_t nnew, int64_
 // Choose the direction of the step described by *c on one wave (no
 // workgroup barrier): lane 0 updates the state; the learned policy evaluates
 // its network on the wave.
-__device__ void choose_dir(const step_args &a, step_smem &s, ctl_block *c, int32_t last_dir,
+template <int BLOCK>
+__device__ void choose_dir(const step_args &a, step_smem<BLOCK> &s, ctl_block *c, int32_t last_dir,
                            int64_t nnew, int64_t nnew_deg) {
     if (lane_id() == 0) {
         s.use_mlp = 0;
@@ -720,42 +782,55 @@ __device__ void choose_dir(const step_args &a, step_smem &s, ctl_block *c, int32
     wave_lds_sync();
 }
 
-// Prefix sums of the segment lengths of a list (segment capacity seg_cap,
-// overflow capacity ovf_cap) into pref[0..kSeg+1].  One wave: lane s takes
-// segment s, lane 63 adds the overflow segment.
-__device__ void seg_prefix(unsigned long long (*len)[kLine], int64_t seg_cap, int64_t ovf_cap,
+// Prefix sums of the segment lengths of a list into pref[0..kSeg+1] on one
+// wave: lane s holds the length of segment s (capped at seg_cap), `ovf` the
+// overflow segment's (capped at ovf_cap; read by every lane).
+__device__ void seg_prefix(int64_t n, int64_t ovf, int64_t seg_cap, int64_t ovf_cap,
                            int64_t *pref) {
     const int l = lane_id();
-    int64_t n = (int64_t)len[l][0];
     n = n < seg_cap ? n : seg_cap;
-    int64_t no = (int64_t)len[kSeg][0];
-    no = no < ovf_cap ? no : ovf_cap;
+    ovf = ovf < ovf_cap ? ovf : ovf_cap;
     const int64_t incl = wave_incl_scan(n);
     pref[l + 1] = incl;
     if (l == 0) pref[0] = 0;
-    if (l == 63) pref[kSeg + 1] = incl + no;
+    if (l == 63) pref[kSeg + 1] = incl + ovf;
 }
+
+constexpr int kCtlWords = (int)(sizeof(ctl_block) / 8);
+static_assert(kCtlWords <= 3 * 64, "ctl_block is copied as three words per lane");
 
 // Rebuild, in s.st, the state before step L = a.level from the published
 // state before step L-1 and the counters of step L-1: close step L-1
 // (record, totals), describe frontier L (segment prefix sums) and choose
 // its direction.  Every workgroup of the step runs this on its first wave
 // and gets the same result, so no workgroup waits for another; `publish`
-// (workgroup 0) also writes the step record.
-__device__ void rebuild_state(const step_args &a, step_smem &s, bool publish) {
+// (workgroup 0) also writes the step record.  All global loads are issued
+// before any is used, so the rebuild costs one memory round trip.
+template <int BLOCK>
+__device__ void rebuild_state(const step_args &a, step_smem<BLOCK> &s, bool publish) {
     const int L = a.level;
     const int l = lane_id();
     seg_counters *g = a.seg;
+    const uint64_t *src = (const uint64_t *)&a.c[(L - 1) & 1];
+    const int pa = (L - 1) % 3, pq = L % 3;
+    uint64_t cw[3];
+#pragma unroll
+    for (int i = 0; i < 3; ++i) cw[i] = l + 64 * i < kCtlWords ? src[l + 64 * i] : 0;
+    unsigned long long ac[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) ac[k] = g->acc[pa][l][k];
+    const int64_t ql = (int64_t)g->qlen[pq][l][0], qo = (int64_t)g->qlen[pq][kSeg][0];
+    const int64_t tl = (int64_t)g->tlen[pq][l][0], to = (int64_t)g->tlen[pq][kSeg][0];
     {
-        const uint64_t *src = (const uint64_t *)&a.c[(L - 1) & 1];
         uint64_t *dst = (uint64_t *)&s.st;
-        for (int i = l; i < (int)(sizeof(ctl_block) / 8); i += 64) dst[i] = src[i];
+#pragma unroll
+        for (int i = 0; i < 3; ++i)
+            if (l + 64 * i < kCtlWords) dst[l + 64 * i] = cw[i];
     }
-    const int pa = (L - 1) % 3;
-    const unsigned long long nn = wave_sum(g->acc[pa][l][0]);
-    const unsigned long long nd = wave_sum(g->acc[pa][l][1]);
-    const unsigned long long ex = wave_sum(g->acc[pa][l][2]);
-    const unsigned long long mx = wave_max(g->acc[pa][l][3]);
+    const unsigned long long nn = wave_sum(ac[0]);
+    const unsigned long long nd = wave_sum(ac[1]);
+    const unsigned long long ex = wave_sum(ac[2]);
+    const unsigned long long mx = wave_max(ac[3]);
     wave_lds_sync();
     ctl_block *c = &s.st;
     if (l == 0) {
@@ -803,8 +878,8 @@ __device__ void rebuild_state(const step_args &a, step_smem &s, bool publish) {
         s.s_dir = dir;
     }
     wave_lds_sync();
-    seg_prefix(g->qlen[L % 3], a.qseg_cap, a.nv, c->qpref);
-    seg_prefix(g->tlen[L % 3], a.tseg_cap, a.task_cap, c->tpref);
+    seg_prefix(ql, qo, a.qseg_cap, a.nv, c->qpref);
+    seg_prefix(tl, to, a.tseg_cap, a.task_cap, c->tpref);
     wave_lds_sync();
     if (s.s_new > 0) choose_dir(a, s, c, s.s_dir, s.s_new, s.s_new_deg);
 }
@@ -813,7 +888,8 @@ __device__ void rebuild_state(const step_args &a, step_smem &s, bool publish) {
 // copies once the search is done, so later launches see it), and clear the
 // counters the next step fills: acc[(L+1)%3] and the segment lengths
 // [(L+2)%3], neither of which anyone reads or writes during step L.
-__device__ void publish_state(const step_args &a, step_smem &s) {
+template <int BLOCK>
+__device__ void publish_state(const step_args &a, step_smem<BLOCK> &s) {
     const int L = a.level;
     const int l = lane_id();
     seg_counters *g = a.seg;
@@ -833,7 +909,9 @@ __device__ void publish_state(const step_args &a, step_smem &s) {
     if (l == 0) {
         g->qlen[(L + 2) % 3][kSeg][0] = 0;
         g->tlen[(L + 2) % 3][kSeg][0] = 0;
-        if (done) g->t_end = wall_clock64();
+        // the first launch to publish "done" stamps the end (the step
+        // kernel of the other direction republishes the same state)
+        if (done && g->t_end == 0) g->t_end = wall_clock64();
     }
 }
 
@@ -842,7 +920,7 @@ __device__ void publish_state(const step_args &a, step_smem &s) {
 // Sum the block's counters and add them to its slot (spread over kSlots
 // cache lines so no line sees more than a few dozen atomics per step).
 template <int BLOCK>
-__device__ void block_flush_counters(const step_args &a, step_smem &s, unsigned long long cnt,
+__device__ void block_flush_counters(const step_args &a, step_smem<BLOCK> &s, unsigned long long cnt,
                                      unsigned long long dsum, unsigned long long exam,
                                      unsigned long long dmax) {
     const int lane = lane_id(), wid = threadIdx.x >> 6;
@@ -872,15 +950,21 @@ __device__ void block_flush_counters(const step_args &a, step_smem &s, unsigned 
     }
 }
 
+// Which directions a step kernel carries: one each (the host launches both
+// for a level, and the one that does not match the device's choice returns
+// after the state rebuild), or both in one kernel.
+constexpr int kStepTD = BD_DIR_TD, kStepBU = BD_DIR_BU, kStepBoth = 2;
+
 // One BFS level.  The first wave of every workgroup rebuilds the state
 // before the step (closing the previous step and choosing this step's
 // direction); then the workgroup runs its share of the step.  close_only
-// launches (one workgroup) only publish the state.
-template <int BLOCK>
+// launches (one workgroup) only publish the state.  The rebuild is
+// idempotent, so a level may be launched more than once.
+template <int BLOCK, int DIRS>
 __global__ __launch_bounds__(BLOCK) void k_step(step_args a) {
     const int L = a.level;
     if (L > 0 && a.c[(L - 1) & 1].done) return;
-    __shared__ step_smem s;
+    __shared__ step_smem<BLOCK> s;
     if (threadIdx.x < 64) {
         if (L == 0) {
             const uint64_t *src = (const uint64_t *)&a.c[0];
@@ -895,8 +979,9 @@ __global__ __launch_bounds__(BLOCK) void k_step(step_args a) {
     __syncthreads();
     if (s.st.done || a.close_only) return;
     const int dir = a.force_dir >= 0 ? a.force_dir : s.st.dir;
+    if (DIRS != kStepBoth && dir != DIRS) return;
     unsigned long long cnt = 0, dsum = 0, exam = 0, dmax = 0;
-    if (dir == BD_DIR_TD) {
+    if (DIRS != kStepBU && dir == BD_DIR_TD) {
         for (int64_t i = (int64_t)blockIdx.x * BLOCK + threadIdx.x; i < a.nwords;
              i += (int64_t)gridDim.x * BLOCK)
             a.clr[i] = 0;
@@ -904,10 +989,10 @@ __global__ __launch_bounds__(BLOCK) void k_step(step_args a) {
             td_bitmap_body<BLOCK>(a, s, cnt, dsum, dmax);
         else
             td_body<BLOCK>(a, s, cnt, dsum, dmax);
-    } else {
+        flush_queue<BLOCK>(a, s);
+    } else if (DIRS != kStepTD) {
         bu_body<BLOCK>(a, s, cnt, dsum, dmax, exam);
     }
-    flush_queue<BLOCK>(a, s);
     block_flush_counters<BLOCK>(a, s, cnt, dsum, exam, dmax);
 }
 
@@ -916,7 +1001,7 @@ __global__ __launch_bounds__(BLOCK) void k_step(step_args a) {
 // (hybrid_bfs.cc:157-163), and the first direction chosen.
 __global__ __launch_bounds__(64) void k_init(step_args a, int64_t source) {
     seg_counters *g = a.seg;
-    __shared__ step_smem s;
+    __shared__ step_smem<64> s;
     const int l = lane_id();
     for (int w = 0; w < 3; ++w) {
         for (int i = l; i <= kSeg; i += 64) {
@@ -1032,6 +1117,61 @@ __global__ void k_count_traversed(const int32_t *__restrict__ depth,
     if ((threadIdx.x & 63) == 0 && s) atomicAdd(&out[(blockIdx.x % kSlots) * kLine], s);
 }
 
+// Head index of the bottom-up step: head[v] = the neighbour of v of largest
+// degree, the first in CSR order among equals (degrees saturated at
+// 2^31-1), -1 when v has none.  A vertex's highest-degree neighbour is the
+// one most likely to be reached first, so it is the likeliest parent when v
+// is scanned bottom-up; a dense array of them is read in vertex order, where
+// the first entry of each adjacency list costs a cache line per vertex.
+__device__ __forceinline__ unsigned long long head_key(const int64_t *off, int32_t u,
+                                                       int64_t pos) {
+    int64_t d = off[u + 1] - off[u];
+    d = d < 0x7fffffff ? d : 0x7fffffff;
+    return ((unsigned long long)d << 32) | (0xffffffffULL - (unsigned long long)pos);
+}
+
+// One wave per 64 vertices: a lane does its own vertex when the list is
+// shorter than 64, longer lists are done by the whole wave.
+__global__ __launch_bounds__(256) void k_head_index(const int64_t *__restrict__ off,
+                                                    const int32_t *__restrict__ cols, int64_t nv,
+                                                    int32_t *__restrict__ head) {
+    const int lane = threadIdx.x & 63;
+    const int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const int64_t nwaves = ((int64_t)gridDim.x * blockDim.x) >> 6;
+    for (int64_t base = wave * 64; base < nv; base += nwaves * 64) {
+        const int64_t v = base + lane;
+        const bool valid = v < nv;
+        int64_t b = 0, e = 0;
+        if (valid) {
+            b = off[v];
+            e = off[v + 1];
+        }
+        const bool alone = valid && e - b < 64;
+        if (alone) {
+            unsigned long long best = 0;
+            for (int64_t k = b; k < e; ++k) {
+                const unsigned long long key = head_key(off, cols[k], k - b);
+                best = key > best ? key : best;
+            }
+            head[v] = best ? cols[b + (int64_t)(0xffffffffULL - (best & 0xffffffffULL))] : -1;
+        }
+        uint64_t lm = __ballot(valid && !alone);
+        while (lm) {
+            const int src = __ffsll((unsigned long long)lm) - 1;
+            lm &= lm - 1;
+            const int64_t kb = __shfl(b, src), ke = __shfl(e, src);
+            unsigned long long best = 0;
+            for (int64_t k = kb + lane; k < ke; k += 64) {
+                const unsigned long long key = head_key(off, cols[k], k - kb);
+                best = key > best ? key : best;
+            }
+            best = wave_max(best);
+            if (lane == src)
+                head[v] = cols[kb + (int64_t)(0xffffffffULL - (best & 0xffffffffULL))];
+        }
+    }
+}
+
 // Streams a buffer larger than the last-level cache so a timed step starts
 // from cold caches.
 __global__ void k_scrub(const uint4 *__restrict__ p, int64_t n, uint4 *__restrict__ sink) {
@@ -1051,6 +1191,7 @@ step_args make_args(bd_ctx *c, int32_t level, int64_t alpha, int64_t beta, int p
     step_args a;
     a.off = c->d_off;
     a.cols = c->d_cols;
+    a.head = c->d_head;
     a.vis = c->d_vis;
     a.front = c->d_bm[level % 3];
     a.next_bm = c->d_bm[(level + 1) % 3];
@@ -1083,6 +1224,7 @@ step_args make_args(bd_ctx *c, int32_t level, int64_t alpha, int64_t beta, int p
     a.task_edges = std::max(64, c->tune.task_edges);
     a.bu_words = c->tune.bu_words > 0 ? std::min(64, c->tune.bu_words) : 32;
     a.bu_cap = c->tune.bu_cap > 0 ? c->tune.bu_cap : 32;
+    a.bu_head = c->tune.bu_head == 2 ? 0 : 1;
     a.policy = policy;
     a.script_len = (int32_t)c->script_len;
     a.log_cap = (int32_t)c->log_cap;
@@ -1092,21 +1234,60 @@ step_args make_args(bd_ctx *c, int32_t level, int64_t alpha, int64_t beta, int p
     return a;
 }
 
-int launch_step(bd_ctx *c, const step_args &a, int64_t grid = 0) {
+int read_ctl(bd_ctx *c);
+
+template <int B>
+void launch_step_block(bd_ctx *c, const step_args &a, int dirs, int64_t grid) {
+    if (dirs == kStepTD)
+        hipLaunchKernelGGL((k_step<B, kStepTD>), dim3(grid), dim3(B), 0, c->stream, a);
+    else if (dirs == kStepBU)
+        hipLaunchKernelGGL((k_step<B, kStepBU>), dim3(grid), dim3(B), 0, c->stream, a);
+    else
+        hipLaunchKernelGGL((k_step<B, kStepBoth>), dim3(grid), dim3(B), 0, c->stream, a);
+}
+
+// Launch the step kernel carrying `dirs` (kStepTD, kStepBU or kStepBoth);
+// grid <= 0 takes the tuned workgroups per CU of that kernel.
+int launch_step(bd_ctx *c, const step_args &a, int dirs, int64_t grid = 0) {
     const int blk = norm_block(c->tune.block);
-    if (grid <= 0) grid = (int64_t)c->cu_count * std::max(1, c->tune.grid_per_cu);
+    if (grid <= 0) {
+        const int per_cu = dirs == kStepBU ? c->tune.bu_grid_per_cu : c->tune.grid_per_cu;
+        grid = (int64_t)c->cu_count * std::max(1, per_cu);
+    }
     switch (blk) {
     case 128:
-        hipLaunchKernelGGL(k_step<128>, dim3(grid), dim3(128), 0, c->stream, a);
+        launch_step_block<128>(c, a, dirs, grid);
         break;
     case 512:
-        hipLaunchKernelGGL(k_step<512>, dim3(grid), dim3(512), 0, c->stream, a);
+        launch_step_block<512>(c, a, dirs, grid);
         break;
     default:
-        hipLaunchKernelGGL(k_step<256>, dim3(grid), dim3(256), 0, c->stream, a);
+        launch_step_block<256>(c, a, dirs, grid);
     }
     BD_HIP(c, hipGetLastError());
     return BD_OK;
+}
+
+// All work of level L in device-driven order: the kernel of each direction
+// (the one the device did not choose returns after its state rebuild), or
+// the kernel carrying both.
+int launch_level(bd_ctx *c, const step_args &a) {
+    int rc;
+    if (c->tune.step_kernels == 2) return launch_step(c, a, kStepBoth);
+    if ((rc = launch_step(c, a, kStepBU))) return rc;
+    return launch_step(c, a, kStepTD);
+}
+
+// Host-driven levels: publish the state before level L with one workgroup
+// (level 0 was published by k_init) and read it back.
+int close_level(bd_ctx *c, const step_args &a) {
+    int rc;
+    if (a.level > 0) {
+        step_args ca = a;
+        ca.close_only = 1;
+        if ((rc = launch_step(c, ca, kStepBU, 1))) return rc;
+    }
+    return read_ctl(c);
 }
 
 int launch_init(bd_ctx *c, const step_args &a, int64_t source) {
@@ -1187,6 +1368,17 @@ int finish_search(bd_ctx *c, int64_t source, bd_run_stats *out, double event_ms)
 
 }  // namespace
 
+int build_head_index(bd_ctx *c) {
+    if (c->d_head || c->nv == 0) return BD_OK;
+    int rc = dev_alloc(c, (void **)&c->d_head, sizeof(int32_t) * c->nv, "head index");
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_head_index, dim3(grid_for(c, c->nv, 256)), dim3(256), 0, c->stream,
+                       c->d_off, c->d_cols, c->nv, c->d_head);
+    BD_HIP(c, hipGetLastError());
+    BD_HIP(c, hipStreamSynchronize(c->stream));
+    return BD_OK;
+}
+
 // A search is k_init, then one k_step launch per level.  Step L publishes
 // the state before it (closing step L-1), so the launch that finds the
 // frontier empty marks the search done and later launches return at once.
@@ -1205,12 +1397,14 @@ int run_search(bd_ctx *c, int64_t source, int64_t alpha, int64_t beta, int polic
         for (;; ++L) {
             if (L > max_levels) return set_err(c, BD_EHIP, "search did not terminate");
             const step_args a = make_args(c, L, alpha, beta, policy);
-            if (timed) BD_HIP(c, hipEventRecord(c->ev[2], c->stream));
-            if ((rc = launch_step(c, a))) return rc;
-            if (timed) BD_HIP(c, hipEventRecord(c->ev[3], c->stream));
-            if ((rc = read_ctl(c))) return rc;
+            if ((rc = close_level(c, a))) return rc;
             if (c->h_ctl[L & 1].done) break;
+            const int dirs = c->tune.step_kernels == 2 ? kStepBoth : c->h_ctl[L & 1].dir;
+            if (timed) BD_HIP(c, hipEventRecord(c->ev[2], c->stream));
+            if ((rc = launch_step(c, a, dirs))) return rc;
             if (timed) {
+                BD_HIP(c, hipEventRecord(c->ev[3], c->stream));
+                BD_HIP(c, hipEventSynchronize(c->ev[3]));
                 float ms = 0;
                 BD_HIP(c, hipEventElapsedTime(&ms, c->ev[2], c->ev[3]));
                 step_ms.push_back(ms);
@@ -1221,7 +1415,7 @@ int run_search(bd_ctx *c, int64_t source, int64_t alpha, int64_t beta, int polic
         for (;;) {
             for (int k = 0; k < chunk; ++k, ++L) {
                 const step_args a = make_args(c, L, alpha, beta, policy);
-                if ((rc = launch_step(c, a))) return rc;
+                if ((rc = launch_level(c, a))) return rc;
             }
             if ((rc = read_ctl(c))) return rc;
             if (final_ctl(c)) break;
@@ -1393,10 +1587,7 @@ extern "C" int bd_bfs_profile(bd_ctx *c, int64_t source, int64_t alpha, int64_t 
         if (L > c->nv + 2) return set_err(c, BD_EHIP, "search did not terminate");
         if (L > 0) {
             // one workgroup closes step L-1 and publishes the state before L
-            step_args a = make_args(c, L, alpha, beta, policy);
-            a.close_only = 1;
-            if ((rc = launch_step(c, a, 1))) return rc;
-            if ((rc = read_ctl(c))) return rc;
+            if ((rc = close_level(c, make_args(c, L, alpha, beta, policy)))) return rc;
             if (c->h_ctl[L & 1].done) break;
         }
         const int32_t chosen = c->h_ctl[L & 1].dir;
@@ -1418,7 +1609,8 @@ extern "C" int bd_bfs_profile(bd_ctx *c, int64_t source, int64_t alpha, int64_t 
             hipLaunchKernelGGL(k_scrub, dim3(c->cu_count * 4), dim3(256), 0, c->stream, scrub,
                                scrub_n, scrub + scrub_n);
             BD_HIP(c, hipEventRecord(c->ev[2], c->stream));
-            if ((rc = launch_step(c, a))) return rc;
+            if ((rc = launch_step(c, a, c->tune.step_kernels == 2 ? kStepBoth : a.force_dir)))
+                return rc;
             BD_HIP(c, hipEventRecord(c->ev[3], c->stream));
             BD_HIP(c, hipEventSynchronize(c->ev[3]));
             BD_HIP(c, hipEventElapsedTime(&ms[pass], c->ev[2], c->ev[3]));

@@ -63,7 +63,9 @@ void free_graph(bd_ctx *c) {
     dev_free(c->d_off);
     dev_free(c->d_cols);
     dev_free(c->d_dead);
+    dev_free(c->d_head);
     dev_free(c->d_el);
+    c->d_head = nullptr;
     c->d_off = nullptr;
     c->d_cols = nullptr;
     c->d_dead = nullptr;
@@ -107,7 +109,7 @@ int alloc_search(bd_ctx *c) {
         if ((rc = dev_alloc(c, (void **)&t, sizeof(td_task) * (kSeg * c->tseg_cap + c->task_cap),
                             "tasks")))
             return rc;
-    return BD_OK;
+    return build_head_index(c);
 }
 
 }  // namespace bd
@@ -140,6 +142,9 @@ extern "C" int bd_create(bd_ctx **out, int device) {
     c->tune.task_edges = 128;
     c->tune.level_driver = 1;
     c->tune.time_steps = 2;
+    c->tune.bu_head = 1;
+    c->tune.step_kernels = 1;
+    c->tune.bu_grid_per_cu = 8;
     hipError_t e = hipSetDevice(device);
     if (e == hipSuccess) {
         hipDeviceProp_t prop;
@@ -244,6 +249,9 @@ extern "C" int bd_set_tuning(bd_ctx *c, const bd_tuning *t) {
     if (t->task_edges > 0) n.task_edges = std::max(t->task_edges, 64);
     if (t->level_driver == 1 || t->level_driver == 2) n.level_driver = t->level_driver;
     if (t->time_steps == 1 || t->time_steps == 2) n.time_steps = t->time_steps;
+    if (t->bu_head == 1 || t->bu_head == 2) n.bu_head = t->bu_head;
+    if (t->step_kernels == 1 || t->step_kernels == 2) n.step_kernels = t->step_kernels;
+    if (t->bu_grid_per_cu > 0) n.bu_grid_per_cu = std::min(t->bu_grid_per_cu, 64);
     bool realloc = (n.task_edges != c->tune.task_edges || n.task_threshold != c->tune.task_threshold);
     c->tune = n;
     if (realloc && c->d_off) return alloc_search(c);

@@ -302,9 +302,11 @@ int build_csr_from_el(bd_ctx *c, int64_t nv, int64_t m) {
     dev_free(c->d_off);
     dev_free(c->d_cols);
     dev_free(c->d_dead);
+    dev_free(c->d_head);
     c->d_off = nullptr;
     c->d_cols = nullptr;
     c->d_dead = nullptr;
+    c->d_head = nullptr;
     if ((rc = dev_alloc(c, &keys.p, sizeof(uint32_t) * n2, "directed keys"))) return rc;
     if ((rc = dev_alloc(c, &vals.p, sizeof(uint32_t) * n2, "directed values"))) return rc;
     if ((rc = dev_alloc(c, &skeys.p, sizeof(uint32_t) * n2, "sorted keys"))) return rc;

@@ -107,6 +107,7 @@ struct bd_ctx {
     int64_t *d_off = nullptr;
     int32_t *d_cols = nullptr;
     uint64_t *d_dead = nullptr;   // 1 = vertex of degree 0, or padding bit
+    int32_t *d_head = nullptr;    // bottom-up head index (built with the graph)
     uint64_t *d_el = nullptr;     // el64 order, packed (src << 32) | dst
     int64_t el_m = 0;
 
@@ -160,8 +161,11 @@ int dev_alloc(bd_ctx *c, void **p, size_t bytes, const char *what);
 void dev_free(void *p);
 // Release all graph-dependent buffers.
 void free_graph(bd_ctx *c);
-// (Re)allocate search buffers for the resident graph.
+// (Re)allocate search buffers for the resident graph (and build its head
+// index if it has none).
 int alloc_search(bd_ctx *c);
+// Head index of the bottom-up step for the resident graph.
+int build_head_index(bd_ctx *c);
 // Build device CSR from a device edge list (packed src<<32|dst, el order).
 int build_csr_from_el(bd_ctx *c, int64_t nv, int64_t m);
 int64_t grid_for(const bd_ctx *c, int64_t work_items, int block);

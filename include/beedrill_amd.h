@@ -180,7 +180,9 @@ int bd_bfs_profile(bd_ctx *ctx, int64_t source, int64_t alpha, int64_t beta, int
 typedef struct {
     int32_t block;           /* threads per workgroup of the step kernel:
                                 128, 256 or 512                           */
-    int32_t grid_per_cu;     /* workgroups per CU (the grid is persistent) */
+    int32_t grid_per_cu;     /* workgroups per CU of the top-down (or
+                                two-direction) kernel; the grid is
+                                persistent                                */
     int32_t wave_threshold;  /* top-down: degree at which a vertex gets a
                                 whole wave                                */
     int32_t task_threshold;  /* degree at which a frontier vertex is split
@@ -195,7 +197,15 @@ typedef struct {
                                 2 = host reads the counters after each step */
     int32_t time_steps;      /* 1 = time each step's kernels with HIP events
                                 (implies the host-driven loop), 2 = off */
-    int32_t reserved;
+    int32_t bu_head;         /* bottom-up: 1 = test each vertex's head index
+                                (its highest-degree neighbour) before its
+                                list; 2 = scan lists only, in CSR order   */
+    int32_t step_kernels;    /* 1 = one step kernel per direction (both are
+                                launched per level; the one not chosen on
+                                the device returns at once); 2 = one kernel
+                                carrying both directions                  */
+    int32_t bu_grid_per_cu;  /* workgroups per CU of the bottom-up kernel  */
+    int32_t reserved[4];
 } bd_tuning;
 int bd_set_tuning(bd_ctx *ctx, const bd_tuning *t);
 int bd_get_tuning(bd_ctx *ctx, bd_tuning *t);

@@ -87,6 +87,8 @@ def lib():
         L.oracle_pick_roots.argtypes = [ctypes.c_int64, _i64p, ctypes.c_int64, _i64p]
         L.oracle_aggregate.argtypes = [_f64p, ctypes.c_int64, _f64p]
         L.oracle_set_mutation.argtypes = [ctypes.c_int]
+        L.oracle_head_index.argtypes = [ctypes.c_int64, _i64p, _i32p, _i32p]
+        L.oracle_set_bu_head.argtypes = [ctypes.c_void_p]
         L.oracle_mlp_nparams.argtypes = [ctypes.c_int, ctypes.c_int]
         L.oracle_mlp_nparams.restype = ctypes.c_int64
         L.oracle_mlp_forward.argtypes = [_f32p, ctypes.c_int, ctypes.c_int, _f32p, ctypes.c_int64,
@@ -187,14 +189,31 @@ def check_graph(nv, src, dst, off, adj) -> bool:
 
 # ----------------------------------------------------------------------- BFS
 
-def beamer(off, adj, m, source, alpha=15, beta=18, log_cap=1 << 16):
-    """Returns (parent[int64, reference encoding], depth[int32], levels[list])."""
+def head_index(off, adj):
+    """Bottom-up head index: per vertex the neighbour of largest degree (first
+    in CSR order among equals), -1 for degree 0."""
+    nv = len(off) - 1
+    head = np.empty(nv, np.int32)
+    lib().oracle_head_index(nv, off, np.ascontiguousarray(adj, np.int32), head)
+    return head
+
+
+def beamer(off, adj, m, source, alpha=15, beta=18, log_cap=1 << 16, head=None):
+    """Returns (parent[int64, reference encoding], depth[int32], levels[list]).
+    With `head` (head_index), bottom-up edges_examined counts the head-first
+    scan of the product's kernel."""
     nv = len(off) - 1
     parent = np.empty(nv, np.int64)
     depth = np.empty(nv, np.int32)
     recs = (LevelRec * log_cap)()
-    n = lib().oracle_beamer(nv, m, off, np.ascontiguousarray(adj, np.int32), source, alpha, beta,
+    L = lib()
+    h = None if head is None else np.ascontiguousarray(head, np.int32)
+    L.oracle_set_bu_head(None if h is None else h.ctypes.data_as(ctypes.c_void_p))
+    try:
+        n = L.oracle_beamer(nv, m, off, np.ascontiguousarray(adj, np.int32), source, alpha, beta,
                             parent, depth, ctypes.cast(recs, ctypes.c_void_p), log_cap)
+    finally:
+        L.oracle_set_bu_head(None)
     if n < 0:
         raise ValueError("bad beamer args")
     levels = [{f: getattr(recs[i], f) for f, _ in LevelRec._fields_} for i in range(min(n, log_cap))]

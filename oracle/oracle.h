@@ -25,6 +25,7 @@
  *                           above, planted-mutation tests.
  *   oracle_check         -> planted corrupt parents must FAIL.
  *   oracle_count_traversed, oracle_aggregate -> closed forms / numpy.
+ *   oracle_head_index    -> numpy lexsort over (vertex, -degree, position).
  *   oracle_vperm, oracle_shuffle_keys -> build-defined permutation (the
  *          reference's std::random_shuffle under _GLIBCXX_PARALLEL is
  *          thread-count dependent); bijection and multiset tests.
@@ -124,6 +125,19 @@ typedef struct {
 int64_t oracle_beamer(int64_t nv, int64_t m, const int64_t *off, const int32_t *adj,
                       int64_t source, int64_t alpha, int64_t beta, int64_t *parent,
                       int32_t *depth, oracle_level_t *log, int64_t log_cap);
+
+/* Bottom-up head index (no reference counterpart; a derived index of the
+ * product's bottom-up step): head[v] = the neighbour of v of largest degree,
+ * the first in CSR order among equals, degrees saturated at 2^31-1; -1 when
+ * v has no neighbours. */
+void oracle_head_index(int64_t nv, const int64_t *off, const int32_t *adj, int32_t *head);
+
+/* With head != NULL, oracle_beamer's bottom-up edges_examined follows the
+ * product's head-first scan: each unvisited vertex reads head[v] (one id),
+ * and only when head[v] is not yet visited scans its list in CSR order up to
+ * the first visited neighbour.  Depths, parents and every other counter are
+ * unchanged.  NULL restores the in-order definition. */
+void oracle_set_bu_head(const int32_t *head);
 
 /* run_with_migrating_threads (hybrid_bfs.cc:197-213): top-down only. */
 int64_t oracle_td_only(int64_t nv, int64_t m, const int64_t *off, const int32_t *adj,

@@ -25,6 +25,26 @@
 static int g_mutation = 0;
 void oracle_set_mutation(int id) { g_mutation = id; }
 
+static const int32_t *g_bu_head = NULL;
+void oracle_set_bu_head(const int32_t *head) { g_bu_head = head; }
+
+void oracle_head_index(int64_t nv, const int64_t *off, const int32_t *adj, int32_t *head) {
+    const int64_t cap = ((int64_t)1 << 31) - 1;
+    for (int64_t v = 0; v < nv; ++v) {
+        int64_t best = -1, best_deg = -1;
+        for (int64_t e = off[v]; e < off[v + 1]; ++e) {
+            const int64_t u = adj[e];
+            int64_t d = off[u + 1] - off[u];
+            if (d > cap) d = cap;
+            if (d > best_deg) {
+                best_deg = d;
+                best = u;
+            }
+        }
+        head[v] = (int32_t)best;
+    }
+}
+
 /* ------------------------------------------------------------ graph */
 
 int oracle_build_csr(int64_t nv, int64_t m, const int64_t *src, const int64_t *dst,
@@ -197,14 +217,20 @@ static int64_t find_parent_unrolled(const bfs_state *b, int64_t child, int64_t *
 /* hybrid_bfs.cc:111-143: pass 1 writes only new_parent_, pass 2 commits, so a
  * vertex found in this step is never seen as a parent in the same step.
  * e_bu counts in-order early-exit scans (the GPU kernel's order), which is
- * what the bandwidth model uses. */
+ * what the bandwidth model uses; with a head index set, the head is read
+ * first (oracle_set_bu_head). */
 static int64_t bottom_up_step(bfs_state *b) {
     int64_t awake = 0;
     int64_t dummy = 0;
     b->e_bu = 0;
     for (int64_t child = 0; child < b->nv; ++child) {
         if (b->parent[child] >= 0) continue;
-        for (int64_t e = b->off[child]; e < b->off[child + 1]; ++e) {
+        int head_hit = 0;
+        if (g_bu_head && g_bu_head[child] >= 0) { /* degree-0 vertices read nothing */
+            b->e_bu++;
+            head_hit = b->parent[g_bu_head[child]] >= 0;
+        }
+        for (int64_t e = b->off[child]; e < b->off[child + 1] && !head_hit; ++e) {
             b->e_bu++;
             if (b->parent[b->adj[e]] >= 0) break;
         }

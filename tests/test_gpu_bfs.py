@@ -32,7 +32,9 @@ def run_and_compare(ctx, orc, off, adj, m, root, policy="beamer", alpha=15, beta
                     levels=True):
     st = ctx.bfs(root, alpha, beta, policy)
     depth, parent = ctx.fetch()
-    ref_parent, ref_depth, ref_lv = orc.beamer(off, adj, m, root, alpha, beta)
+    # the bottom-up kernel reads each vertex's head index first unless told not to
+    head = None if ctx.get_tuning()["bu_head"] == 2 else orc.head_index(off, adj)
+    ref_parent, ref_depth, ref_lv = orc.beamer(off, adj, m, root, alpha, beta, head=head)
     np.testing.assert_array_equal(depth, ref_depth)
     assert orc.check(off, adj, root, parent)
     assert ((parent < 0) == (depth < 0)).all()
@@ -163,6 +165,16 @@ def test_s16_host_driven_levels_identical(ctx, orc, s16):
         ctx.set_tuning(block=128, wave_threshold=1000, task_threshold=64, task_edges=64,
                        bu_words=64, bu_cap=1, grid_per_cu=1)
         run_and_compare(ctx, orc, off, adj, m, root)
+        # lists only, in CSR order: edges_examined is the in-order early-exit count
+        ctx.set_tuning(bu_head=2, bu_words=32, bu_cap=32, block=256, grid_per_cu=8)
+        run_and_compare(ctx, orc, off, adj, m, root)
+        # one kernel carrying both directions; odd bottom-up grid
+        ctx.set_tuning(bu_head=1, step_kernels=2)
+        run_and_compare(ctx, orc, off, adj, m, root)
+        ctx.set_tuning(step_kernels=1, bu_grid_per_cu=3, bu_words=5)
+        run_and_compare(ctx, orc, off, adj, m, root)
+        ctx.set_tuning(level_driver=2)
+        run_and_compare(ctx, orc, off, adj, m, root)
     finally:
         ctx.set_tuning(**t0)
 
@@ -182,7 +194,7 @@ def test_profile_times_both_directions(ctx, orc, s16):
     ctx.profile(root)
     lv = ctx.levels()
     depth, parent = ctx.fetch()
-    _, ref_depth, ref_lv = orc.beamer(off, adj, m, root)
+    _, ref_depth, ref_lv = orc.beamer(off, adj, m, root, head=orc.head_index(off, adj))
     np.testing.assert_array_equal(depth, ref_depth)
     assert orc.check(off, adj, root, parent)
     assert_same_levels(lv, ref_lv)

@@ -232,3 +232,58 @@ def test_aggregate_matches_scipy(orc):
     assert g["hmean"] == pytest.approx(hmean(x), rel=1e-12)
     assert g["stddev"] == pytest.approx(np.std(x, ddof=1), rel=1e-12)
     assert g["mean"] == pytest.approx(x.mean()) and g["min"] == x.min() and g["max"] == x.max()
+
+
+# ------------------------------------------------------ bottom-up head index
+
+def test_head_index_matches_lexsort(orc):
+    """head[v] = highest-degree neighbour, first in CSR order among equals:
+    restated with numpy's lexsort over (vertex, -degree, position)."""
+    nv, s, d = orc.make_rmat(12, 16, seed=4)
+    s = np.concatenate([s, [7, 7]])      # a self-loop and a duplicate stay in the CSR
+    d = np.concatenate([d, [7, int(d[0])]])
+    off, adj = orc.build_csr(nv, s, d)
+    deg = np.diff(off)
+    head = orc.head_index(off, adj)
+    owner = np.repeat(np.arange(nv), deg)
+    order = np.lexsort((np.arange(len(adj)), -deg[adj], owner))
+    expect = np.full(nv, -1, np.int64)
+    has = deg > 0
+    expect[has] = adj[order][off[:-1][has]]
+    np.testing.assert_array_equal(head, expect)
+    # brute force on a few vertices
+    for v in np.flatnonzero(has)[:50]:
+        nb = adj[off[v]:off[v + 1]]
+        assert head[v] == nb[int(np.argmax(deg[nb]))]
+
+
+def test_head_first_scan_changes_only_edges_examined(orc):
+    nv, s, d = orc.make_rmat(14, 16, seed=2)
+    off, adj, m = csr_of(orc, nv, s, d)
+    head = orc.head_index(off, adj)
+    for root in orc.pick_roots(off, 3):
+        p0, d0, l0 = orc.beamer(off, adj, m, int(root))
+        p1, d1, l1 = orc.beamer(off, adj, m, int(root), head=head)
+        np.testing.assert_array_equal(d0, d1)
+        np.testing.assert_array_equal(p0, p1)
+        assert len(l0) == len(l1)
+        for a, b in zip(l0, l1):
+            for k in a:
+                if k != "edges_examined" or a["dir"] == TD:
+                    assert a[k] == b[k], k
+        # closed form of the head-first count for every bottom-up step:
+        # per scanned vertex 1 for the head, plus the in-order scan when the
+        # head was not yet visited
+        deg = np.diff(off)
+        for rec in l1:
+            if rec["dir"] != BU:
+                continue
+            L = rec["level"]
+            scanned = (deg > 0) & ((d1 > L) | (d1 < 0))
+            hv = scanned & (d1[np.maximum(head, 0)] <= L) & (d1[np.maximum(head, 0)] >= 0)
+            want = int(scanned.sum())
+            for v in np.flatnonzero(scanned & ~hv):
+                nb = adj[off[v]:off[v + 1]]
+                seen = (d1[nb] >= 0) & (d1[nb] <= L)
+                want += int(np.argmax(seen)) + 1 if seen.any() else len(nb)
+            assert rec["edges_examined"] == want
