@@ -54,11 +54,11 @@ class RunStats(ctypes.Structure):
 
 TUNING_FIELDS = ("block", "grid_per_cu", "wave_threshold", "task_threshold", "task_edges",
                  "bu_words", "bu_cap", "level_driver", "time_steps", "bu_head", "step_kernels",
-                 "bu_grid_per_cu")
+                 "bu_grid_per_cu", "learned_margin_milli")
 
 
 class Tuning(ctypes.Structure):
-    _fields_ = [(n, ctypes.c_int32) for n in TUNING_FIELDS] + [("reserved", ctypes.c_int32 * 4)]
+    _fields_ = [(n, ctypes.c_int32) for n in TUNING_FIELDS] + [("reserved", ctypes.c_int32 * 3)]
 
     def as_dict(self):
         return {f: getattr(self, f) for f in TUNING_FIELDS}
@@ -112,7 +112,7 @@ def lib():
         "bd_get_tuning": ([vp, ctypes.POINTER(Tuning)], ctypes.c_int),
         "bd_bfs_multi_root": ([vp, _i64p, ctypes.c_int, vp, ctypes.POINTER(ctypes.c_double),
                                ctypes.POINTER(i64)], ctypes.c_int),
-        "bd_bfs_multi_graph": ([vp, ctypes.c_int, _i64p, _i64p, _i32p, _i64p, vp,
+        "bd_bfs_multi_graph": ([vp, ctypes.c_int, _i64p, _i64p, vp,
                                 ctypes.POINTER(ctypes.c_double)], ctypes.c_int),
         "bd_mlp_set": ([vp, _f32p, ctypes.c_int, ctypes.c_int], ctypes.c_int),
         "bd_mlp_get": ([vp, vp, i64, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)],
@@ -130,6 +130,21 @@ def lib():
         fn.restype = res
     _lib = L
     return L
+
+
+def union_csr(graphs):
+    """Disjoint union of CSR graphs [(offsets, cols, m), ...]: graph g's
+    vertices are renumbered from vbase[g].  Returns (offsets, cols, m, vbase)."""
+    nvs = [len(o) - 1 for o, _, _ in graphs]
+    vbase = np.concatenate([[0], np.cumsum(nvs)]).astype(np.int64)
+    ebase = np.concatenate([[0], np.cumsum([len(c) for _, c, _ in graphs])]).astype(np.int64)
+    off = np.empty(vbase[-1] + 1, np.int64)
+    cols = np.empty(ebase[-1], np.int32)
+    for g, (o, c, _) in enumerate(graphs):
+        off[vbase[g]:vbase[g + 1]] = np.asarray(o[:-1], np.int64) + ebase[g]
+        cols[ebase[g]:ebase[g + 1]] = np.asarray(c, np.int64) + vbase[g]
+    off[-1] = ebase[-1]
+    return off, cols, int(sum(m for _, _, m in graphs)), vbase
 
 
 def exported_symbols():
@@ -315,18 +330,24 @@ class Context:
                                            ctypes.byref(ms), ctypes.byref(lv)))
         return dp, ms.value, lv.value
 
-    def multi_graph(self, vbase, off, cols, roots, want_depths=True):
+    def multi_graph(self, vbase, roots, want_depths=True):
+        """Search every graph of a disjoint union (the resident graph) from its
+        root at once; vbase[g] is graph g's first vertex, roots are local."""
         vbase = np.ascontiguousarray(vbase, np.int64)
-        off = np.ascontiguousarray(off, np.int64)
-        cols = np.ascontiguousarray(cols, np.int32)
         roots = np.ascontiguousarray(roots, np.int64)
-        ntot = int(vbase[-1])
-        dp = np.empty(ntot, np.int32) if want_depths else None
+        dp = np.empty(int(vbase[-1]), np.int32) if want_depths else None
         ms = ctypes.c_double(0)
-        self._ok(self._L.bd_bfs_multi_graph(self._h, len(roots), vbase, off, cols, roots,
+        self._ok(self._L.bd_bfs_multi_graph(self._h, len(roots), vbase, roots,
                                             dp.ctypes.data if dp is not None else None,
                                             ctypes.byref(ms)))
         return dp, ms.value
+
+    def upload_union(self, graphs):
+        """Upload the disjoint union of CSR graphs [(offsets, cols, m), ...];
+        returns vbase (ngraphs+1 entries)."""
+        off, cols, m, vbase = union_csr(graphs)
+        self.upload_csr(off, cols, m)
+        return vbase
 
     # ------------------------------------------------------------------- MLP
     def mlp_set(self, params, h1=32, h2=32):

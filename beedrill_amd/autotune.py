@@ -1,0 +1,120 @@
+"""Autotuner for the search kernels' launch shape and thresholds on MI355X.
+
+Coordinate descent over the bd_tuning fields: starting from the library's
+defaults, each parameter in turn is set to every value of its range with the
+others fixed, and the value with the best harmonic-mean GTEPS (Beamer,
+alpha 15, beta 18) is kept; the passes repeat until nothing improves.  The
+objective is measured on a graph and roots that the benchmark does not use
+(RMAT generator seed 1; roots 65-96 of the lcg(0) sequence), so the result is
+not fitted to the benchmark's own inputs.
+
+    python -m beedrill_amd.autotune --scale 24 --out beedrill_amd/models/autotune_rmat_s24.json
+
+Prints one line per trial and writes the best tuning and the trial log as
+JSON.  The library's compiled-in defaults are the values this found on
+MI355X.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+if os.path.dirname(HERE) not in sys.path:
+    sys.path.insert(0, os.path.dirname(HERE))
+
+import beedrill_amd as bd  # noqa: E402
+
+# Values tried per field (all within the ranges bd_set_tuning accepts).
+SPACE = {
+    "bu_grid_per_cu": [4, 6, 8, 12, 16],
+    "bu_words": [8, 16, 32, 48, 64],
+    "bu_cap": [8, 16, 32, 64, 128],
+    "grid_per_cu": [2, 4, 6, 8, 12, 16],
+    "wave_threshold": [8, 16, 32, 64, 128],
+    "task_threshold": [128, 256, 512, 1024, 4096],
+    "task_edges": [64, 128, 256, 512],
+    "block": [128, 256, 512],
+    "step_kernels": [1, 2],
+    "bu_head": [1, 2],
+}
+
+
+def hmean_gteps(ctx, roots, reps=1):
+    g = []
+    for _ in range(reps):
+        for r in roots:
+            st = ctx.bfs(int(r), 15, 18, "beamer")
+            g.append(st["traversed_edges"] / (st["ms"] * 1e-3) / 1e9)
+    g = np.asarray(g)
+    return float(len(g) / np.sum(1.0 / g))
+
+
+def tune(ctx, roots, passes=3, log=print):
+    best = {k: ctx.get_tuning()[k] for k in SPACE}
+    ctx.set_tuning(**best)
+    for r in roots[:4]:  # warm up
+        ctx.bfs(int(r))
+    best_val = hmean_gteps(ctx, roots)
+    trials = [{"tuning": dict(best), "gteps": best_val}]
+    log(json.dumps(trials[-1]))
+    for p in range(passes):
+        improved = False
+        for field, values in SPACE.items():
+            for v in values:
+                if v == best[field]:
+                    continue
+                cand = dict(best, **{field: v})
+                ctx.set_tuning(**cand)
+                val = hmean_gteps(ctx, roots)
+                trials.append({"tuning": cand, "gteps": val})
+                log(json.dumps({"pass": p, field: v, "gteps": round(val, 2)}))
+                if val > best_val * 1.005:  # ignore differences within noise
+                    best, best_val, improved = cand, val, True
+            ctx.set_tuning(**best)
+        if not improved:
+            break
+    # confirm: best and defaults measured again, twice over
+    ctx.set_tuning(**best)
+    confirm = hmean_gteps(ctx, roots, reps=2)
+    return best, confirm, trials
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--scale", type=int, default=24)
+    ap.add_argument("--edge-factor", type=int, default=16)
+    ap.add_argument("--seed", type=int, default=1)
+    ap.add_argument("--roots", type=int, default=32)
+    ap.add_argument("--passes", type=int, default=3)
+    ap.add_argument("--out", default=None)
+    args = ap.parse_args()
+    ctx = bd.Context(0)
+    t0 = time.time()
+    ctx.generate_rmat(args.scale, args.edge_factor, seed=args.seed)
+    # roots after the benchmark's 64
+    roots = ctx.pick_roots(64 + args.roots)[64:]
+    defaults = ctx.get_tuning()
+    ctx.set_tuning(**defaults)
+    for r in roots[:4]:
+        ctx.bfs(int(r))
+    base = hmean_gteps(ctx, roots, reps=2)
+    best, confirm, trials = tune(ctx, roots, args.passes)
+    out = {"graph": {"kind": "rmat", "scale": args.scale, "edge_factor": args.edge_factor,
+                     "seed": args.seed, "roots": "lcg(0) picks 65-%d" % (64 + args.roots)},
+           "defaults": {k: defaults[k] for k in SPACE}, "defaults_gteps": base,
+           "best": best, "best_gteps": confirm, "trials": len(trials),
+           "seconds": round(time.time() - t0, 1), "log": trials}
+    print(json.dumps({k: v for k, v in out.items() if k != "log"}))
+    if args.out:
+        with open(args.out, "w") as f:
+            json.dump(out, f, indent=1)
+
+
+if __name__ == "__main__":
+    main()

@@ -43,6 +43,7 @@
 #include <algorithm>
 #include <vector>
 
+#include "bd_device.h"
 #include "bd_internal.h"
 #include "bd_mlp.h"
 
@@ -85,52 +86,10 @@ struct step_args {
     int32_t bu_words, bu_cap, bu_head;
     int32_t policy, script_len, log_cap;
     int32_t mlp_h1, mlp_h2, has_mlp;
+    float mlp_margin;      // log2-cost margin for the network to override Beamer
 };
 
 // ----------------------------------------------------------------- helpers
-
-__device__ __forceinline__ unsigned long long wave_sum(unsigned long long x) {
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) x += (unsigned long long)__shfl_xor((long long)x, o);
-    return x;
-}
-
-__device__ __forceinline__ unsigned long long wave_max(unsigned long long x) {
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) {
-        const unsigned long long y = (unsigned long long)__shfl_xor((long long)x, o);
-        x = y > x ? y : x;
-    }
-    return x;
-}
-
-__device__ __forceinline__ long long wave_incl_scan(long long x) {
-    const int lane = threadIdx.x & 63;
-#pragma unroll
-    for (int o = 1; o < 64; o <<= 1) {
-        const long long y = __shfl_up(x, o);
-        if (lane >= o) x += y;
-    }
-    return x;
-}
-
-__device__ __forceinline__ int lane_id() { return threadIdx.x & 63; }
-
-// Position of the k-th set bit (k from 0) of x.
-__device__ __forceinline__ int select_bit(uint64_t x, int k) {
-    int pos = 0;
-#pragma unroll
-    for (int s = 32; s > 0; s >>= 1) {
-        const uint64_t low = x & ((1ULL << s) - 1);
-        const int c = __popcll(low);
-        if (k >= c) {
-            k -= c;
-            x >>= s;
-            pos += s;
-        }
-    }
-    return pos;
-}
 
 // Index of the segment holding item i: the largest s with pref[s] <= i.
 __device__ __forceinline__ int seg_of(const int64_t *pref, int64_t i) {
@@ -760,13 +719,13 @@ __device__ void choose_dir(const step_args &a, step_smem<BLOCK> &s, ctl_block *c
             break;
         }
         case BD_POLICY_LEARNED:
+            // Beamer's choice and state, which the network may override
+            beamer_next(c, last_dir, nnew, nnew_deg);
             if (a.has_mlp) {
                 s.use_mlp = 1;
                 mlp_features_dev(c->frontier, c->frontier_deg, c->max_frontier_deg, c->unvisited,
                                  c->unvisited_deg, c->level, c->nv, a.m, last_dir, s.x);
-                break;
             }
-            beamer_next(c, last_dir, nnew, nnew_deg);  // no network: Beamer
             break;
         default:
             beamer_next(c, last_dir, nnew, nnew_deg);
@@ -776,8 +735,27 @@ __device__ void choose_dir(const step_args &a, step_smem<BLOCK> &s, ctl_block *c
     if (s.use_mlp) {
         float y[2];
         mlp_forward_wave(a.mlp, a.mlp_h1, a.mlp_h2, s.x, y, &s.mlp);
-        // predicted log-costs; a NaN prediction falls back to top-down
-        if (lane_id() == 0) c->dir = (y[1] < y[0]) ? BD_DIR_BU : BD_DIR_TD;
+        // y = predicted log2 cost of (top-down, bottom-up).  The network
+        // overrides Beamer only when it predicts the other direction cheaper
+        // by more than the margin; a NaN prediction keeps Beamer's choice.
+        if (lane_id() == 0) {
+            const float m = a.mlp_margin;
+            int want = c->dir;
+            if (c->dir == BD_DIR_TD && y[1] + m < y[0]) want = BD_DIR_BU;
+            if (c->dir == BD_DIR_BU && y[0] + m < y[1]) want = BD_DIR_TD;
+            if (want != c->dir) {
+                // leave Beamer's state as if it had chosen `want`
+                if (want == BD_DIR_BU) {
+                    c->edges_to_check += c->scout;  // undo the top-down charge
+                    c->in_bu_loop = 1;
+                    c->old_awake = c->frontier;
+                } else {
+                    c->in_bu_loop = 0;
+                    c->edges_to_check -= c->scout;
+                }
+                c->dir = want;
+            }
+        }
     }
     wave_lds_sync();
 }
@@ -997,12 +975,18 @@ __global__ __launch_bounds__(BLOCK) void k_step(step_args a) {
 }
 
 // Start a search: clear() (hybrid_bfs.cc:433-443) has been done by k_clear;
-// here the counters are zeroed, the source is pushed and marked
-// (hybrid_bfs.cc:157-163), and the first direction chosen.
-__global__ __launch_bounds__(64) void k_init(step_args a, int64_t source) {
+// here the counters are zeroed, the sources are pushed and marked
+// (hybrid_bfs.cc:157-163), and the first direction chosen.  One source is
+// the reference's search; several (srcs[0..nsrc), one wave claiming them 64
+// at a time, duplicates once) start every search of a batch of disjoint
+// graphs at once.  Frontier 0 is written to segment 0 of list 0 and its
+// overflow segment.
+__global__ __launch_bounds__(64) void k_init(step_args a, int64_t source,
+                                             const int64_t *__restrict__ srcs, int64_t nsrc) {
     seg_counters *g = a.seg;
     __shared__ step_smem<64> s;
     const int l = lane_id();
+    const uint64_t below = (1ULL << l) - 1;
     for (int w = 0; w < 3; ++w) {
         for (int i = l; i <= kSeg; i += 64) {
             g->qlen[w][i][0] = 0;
@@ -1014,58 +998,92 @@ __global__ __launch_bounds__(64) void k_init(step_args a, int64_t source) {
     if (l == 0) {
         g->task_overflow = 0;
         g->t_end = 0;
-        const int64_t b = a.off[source], e = a.off[source + 1];
-        const int64_t dsrc = e - b;
         memset(&a.c[0], 0, 2 * sizeof(ctl_block));
+    }
+    int64_t nq = 0, nt = 0;  // wave-uniform list lengths
+    unsigned long long cnt = 0, dsum = 0, dmax = 0, lost = 0;
+    for (int64_t i0 = 0; i0 < nsrc; i0 += 64) {
+        const int64_t i = i0 + l;
+        const bool valid = i < nsrc;
+        const int64_t v = valid ? (srcs ? srcs[i] : source) : 0;
+        bool won = false;
+        int64_t b = 0, e = 0;
+        if (valid) {
+            // claims go through bm[0] (zeroed by k_clear): vis already holds
+            // the degree-0 vertices, which can still be sources
+            const uint64_t bit = 1ULL << (v & 63);
+            const unsigned long long old =
+                atomicOr((unsigned long long *)&a.next_bm[v >> 6], (unsigned long long)bit);
+            won = !(old & bit);
+            if (won) {
+                atomicOr((unsigned long long *)&a.vis[v >> 6], (unsigned long long)bit);
+                a.depth[v] = 0;
+                a.parent[v] = (int32_t)v;
+                b = a.off[v];
+                e = a.off[v + 1];
+                const unsigned long long d = (unsigned long long)(e - b);
+                ++cnt;
+                dsum += d;
+                dmax = d > dmax ? d : dmax;
+            }
+        }
+        const bool huge = won && (e - b) >= a.task_t;
+        const bool q = won && !huge;
+        const uint64_t qm = __ballot(q);
+        if (q) {
+            const int64_t p = nq + __popcll(qm & below);
+            a.qout[p < a.qseg_cap ? p : kSeg * a.qseg_cap + (p - a.qseg_cap)] = (int32_t)v;
+        }
+        nq += __popcll(qm);
+        const long long ntk = huge ? (e - b + a.task_edges - 1) / a.task_edges : 0;
+        const long long incl = wave_incl_scan(ntk);
+        for (long long t = 0; t < ntk; ++t) {
+            td_task tk;
+            tk.beg = b + t * a.task_edges;
+            tk.end = tk.beg + a.task_edges < e ? tk.beg + a.task_edges : e;
+            tk.src = (int32_t)v;
+            tk.pad = 0;
+            const int64_t p = nt + incl - ntk + t;
+            if (p < a.tseg_cap)
+                a.tout[p] = tk;
+            else if (p - a.tseg_cap < a.task_cap)
+                a.tout[kSeg * a.tseg_cap + (p - a.tseg_cap)] = tk;
+            else
+                ++lost;
+        }
+        nt += __shfl(incl, 63);
+    }
+    cnt = wave_sum(cnt);
+    dsum = wave_sum(dsum);
+    dmax = wave_max(dmax);
+    lost = wave_sum(lost);
+    if (l == 0) {
+        if (lost) g->task_overflow = lost;
         c->nv = a.nv;
         c->m2 = 2 * a.m;
         c->alpha = a.alpha;
         c->beta = a.beta;
         c->policy = a.policy;
         c->edges_to_check = 2 * a.m;
-        c->scout = dsrc;
+        c->scout = (int64_t)dsum;
         c->prev_dir = -1;
-        c->frontier = 1;
-        c->frontier_deg = dsrc;
-        c->max_frontier_deg = dsrc;
-        c->unvisited = a.nv - 1;
-        c->unvisited_deg = a.nnz - dsrc;
-        c->reached = 1;
-        c->reached_deg = dsrc;
-        a.vis[source >> 6] |= 1ULL << (source & 63);
-        a.next_bm[source >> 6] = 1ULL << (source & 63);  // bm[0]
-        a.depth[source] = 0;
-        a.parent[source] = (int32_t)source;
-        // frontier 0 lives in segment 0 of list 0
-        if (dsrc >= a.task_t) {
-            const int64_t nt = (dsrc + a.task_edges - 1) / a.task_edges;
-            for (int64_t t = 0; t < nt; ++t) {
-                td_task tk;
-                tk.beg = b + t * a.task_edges;
-                tk.end = tk.beg + a.task_edges < e ? tk.beg + a.task_edges : e;
-                tk.src = (int32_t)source;
-                tk.pad = 0;
-                if (t < a.tseg_cap)
-                    a.tout[t] = tk;
-                else
-                    a.tout[kSeg * a.tseg_cap + (t - a.tseg_cap)] = tk;
-            }
-            for (int k = 0; k < kSeg + 2; ++k) {
-                c->qpref[k] = 0;
-                c->tpref[k] = k == 0 ? 0 : (nt < a.tseg_cap ? nt : a.tseg_cap);
-            }
-            c->tpref[kSeg + 1] = nt;
-        } else {
-            a.qout[0] = (int32_t)source;
-            for (int k = 0; k < kSeg + 2; ++k) {
-                c->qpref[k] = k == 0 ? 0 : 1;
-                c->tpref[k] = 0;
-            }
+        c->frontier = (int64_t)cnt;
+        c->frontier_deg = (int64_t)dsum;
+        c->max_frontier_deg = (int64_t)dmax;
+        c->unvisited = a.nv - (int64_t)cnt;
+        c->unvisited_deg = a.nnz - (int64_t)dsum;
+        c->reached = (int64_t)cnt;
+        c->reached_deg = (int64_t)dsum;
+        const int64_t q0 = nq < a.qseg_cap ? nq : a.qseg_cap;
+        const int64_t t0 = nt < a.tseg_cap ? nt : a.tseg_cap;
+        for (int k = 0; k < kSeg + 2; ++k) {
+            c->qpref[k] = k == 0 ? 0 : (k == kSeg + 1 ? nq : q0);
+            c->tpref[k] = k == 0 ? 0 : (k == kSeg + 1 ? (nt < t0 + a.task_cap ? nt : t0 + a.task_cap) : t0);
         }
     }
     wave_lds_sync();
     // level 0: with no previous step Beamer applies only the outer test to
-    // scout = deg(source) (hybrid_bfs.cc:163-167)
+    // scout = deg(sources) (hybrid_bfs.cc:163-167)
     choose_dir(a, s, c, -1, 0, 0);
     if (l == 0) {
         __threadfence();
@@ -1231,6 +1249,7 @@ step_args make_args(bd_ctx *c, int32_t level, int64_t alpha, int64_t beta, int p
     a.mlp_h1 = c->mlp_h1;
     a.mlp_h2 = c->mlp_h2;
     a.has_mlp = c->has_mlp ? 1 : 0;
+    a.mlp_margin = (float)c->tune.learned_margin_milli * 1e-3f;
     return a;
 }
 
@@ -1285,13 +1304,14 @@ int close_level(bd_ctx *c, const step_args &a) {
     if (a.level > 0) {
         step_args ca = a;
         ca.close_only = 1;
-        if ((rc = launch_step(c, ca, kStepBU, 1))) return rc;
+        if ((rc = launch_step(c, ca, kStepBoth, 1))) return rc;
     }
     return read_ctl(c);
 }
 
-int launch_init(bd_ctx *c, const step_args &a, int64_t source) {
-    hipLaunchKernelGGL(k_init, dim3(1), dim3(64), 0, c->stream, a, source);
+int launch_init(bd_ctx *c, const step_args &a, int64_t source, const int64_t *srcs,
+                int64_t nsrc) {
+    hipLaunchKernelGGL(k_init, dim3(1), dim3(64), 0, c->stream, a, source, srcs, nsrc);
     BD_HIP(c, hipGetLastError());
     return BD_OK;
 }
@@ -1308,7 +1328,8 @@ int check_run_args(bd_ctx *c, int64_t source, int64_t alpha, int64_t beta, int p
     return BD_OK;
 }
 
-int start_search(bd_ctx *c, int64_t source, int64_t alpha, int64_t beta, int policy) {
+int start_search(bd_ctx *c, int64_t source, int64_t alpha, int64_t beta, int policy,
+                 const int64_t *srcs = nullptr, int64_t nsrc = 1) {
     const int64_t n = std::max(c->nv, c->nwords);
     hipLaunchKernelGGL(k_clear, dim3(grid_for(c, n, 256)), dim3(256), 0, c->stream, c->d_depth,
                        c->d_parent, c->nv, c->d_vis, c->d_dead, c->d_bm[0], c->d_bm[1],
@@ -1319,7 +1340,7 @@ int start_search(bd_ctx *c, int64_t source, int64_t alpha, int64_t beta, int pol
     a.next_bm = c->d_bm[0];
     a.qout = c->d_q[0];
     a.tout = c->d_tasks[0];
-    return launch_init(c, a, source);
+    return launch_init(c, a, source, srcs, nsrc);
 }
 
 // Copy both published states to the pinned mirror and wait.
@@ -1382,12 +1403,13 @@ int build_head_index(bd_ctx *c) {
 // A search is k_init, then one k_step launch per level.  Step L publishes
 // the state before it (closing step L-1), so the launch that finds the
 // frontier empty marks the search done and later launches return at once.
+// With srcs (device, nsrc entries) the search starts from all of them.
 int run_search(bd_ctx *c, int64_t source, int64_t alpha, int64_t beta, int policy,
-               bd_run_stats *out) {
+               bd_run_stats *out, const int64_t *srcs = nullptr, int64_t nsrc = 1) {
     int rc;
     if ((rc = check_run_args(c, source, alpha, beta, policy))) return rc;
     BD_HIP(c, hipSetDevice(c->device));
-    if ((rc = start_search(c, source, alpha, beta, policy))) return rc;
+    if ((rc = start_search(c, source, alpha, beta, policy, srcs, nsrc))) return rc;
     BD_HIP(c, hipEventRecord(c->ev[0], c->stream));
     const int64_t max_levels = c->nv + 2;
     int32_t L = 0;
@@ -1629,5 +1651,86 @@ extern "C" int bd_bfs_profile(bd_ctx *c, int64_t source, int64_t alpha, int64_t 
         c->last_log[i].ms_other = times[i].second;
     }
     if (out) out->ms = total;
+    return BD_OK;
+}
+
+// ------------------------------------------------------- many graphs at once
+
+namespace bd {
+namespace {
+
+// Flags an edge whose ends lie in different graphs of the batch (graph g
+// owns vertices [vbase[g], vbase[g+1])).
+__global__ void k_check_parts(const int64_t *__restrict__ off, const int32_t *__restrict__ cols,
+                              int64_t nv, const int64_t *__restrict__ vbase, int ng,
+                              int *__restrict__ bad) {
+    for (int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v < nv;
+         v += (int64_t)gridDim.x * blockDim.x) {
+        int lo = 0, hi = ng;  // largest g with vbase[g] <= v
+        while (hi - lo > 1) {
+            const int mid = (lo + hi) >> 1;
+            if (vbase[mid] <= v)
+                lo = mid;
+            else
+                hi = mid;
+        }
+        const int64_t gb = vbase[lo], ge = vbase[lo + 1];
+        for (int64_t k = off[v]; k < off[v + 1]; ++k) {
+            const int64_t u = cols[k];
+            if (u < gb || u >= ge) {
+                *bad = 1;
+                break;
+            }
+        }
+    }
+}
+
+}  // namespace
+}  // namespace bd
+
+// Many disjoint graphs (the resident graph is their union) searched from
+// their roots at once: one direction-optimizing search whose frontier 0
+// holds every root.  As no edge joins two graphs, each vertex's depth is its
+// distance from its own graph's root.  Beamer's rules (alpha 15, beta 18)
+// run on the union.
+extern "C" int bd_bfs_multi_graph(bd_ctx *c, int ngraphs, const int64_t *vbase,
+                                  const int64_t *roots, int32_t *depths, double *ms) {
+    if (!c || ngraphs < 1 || !vbase || !roots) return BD_EINVAL;
+    if (!c->d_off) return set_err(c, BD_ESTATE, "no graph");
+    if (vbase[0] != 0 || vbase[ngraphs] != c->nv)
+        return set_err(c, BD_EINVAL, "vbase must run from 0 to the number of vertices");
+    std::vector<int64_t> src(ngraphs);
+    for (int g = 0; g < ngraphs; ++g) {
+        if (vbase[g + 1] <= vbase[g]) return set_err(c, BD_EINVAL, "empty or unordered graph");
+        if (roots[g] < 0 || roots[g] >= vbase[g + 1] - vbase[g])
+            return set_err(c, BD_ERANGE, "root out of range of its graph");
+        src[g] = vbase[g] + roots[g];
+    }
+    BD_HIP(c, hipSetDevice(c->device));
+    int rc;
+    int64_t *d = nullptr;
+    if ((rc = dev_alloc(c, (void **)&d, sizeof(int64_t) * (2 * ngraphs + 1) + sizeof(int),
+                        "batch sources")))
+        return rc;
+    struct guard {
+        void *p;
+        ~guard() { dev_free(p); }
+    } gd{d};
+    int64_t *d_src = d, *d_vbase = d + ngraphs;
+    int *d_bad = (int *)(d + 2 * ngraphs + 1);
+    BD_HIP(c, hipMemcpy(d_src, src.data(), sizeof(int64_t) * ngraphs, hipMemcpyHostToDevice));
+    BD_HIP(c, hipMemcpy(d_vbase, vbase, sizeof(int64_t) * (ngraphs + 1), hipMemcpyHostToDevice));
+    BD_HIP(c, hipMemset(d_bad, 0, sizeof(int)));
+    hipLaunchKernelGGL(k_check_parts, dim3(grid_for(c, c->nv, 256)), dim3(256), 0, c->stream,
+                       c->d_off, c->d_cols, c->nv, d_vbase, ngraphs, d_bad);
+    BD_HIP(c, hipGetLastError());
+    int bad = 0;
+    BD_HIP(c, hipMemcpyAsync(&bad, d_bad, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    BD_HIP(c, hipStreamSynchronize(c->stream));
+    if (bad) return set_err(c, BD_EINVAL, "an edge joins two graphs of the batch");
+    bd_run_stats st;
+    if ((rc = run_search(c, src[0], 15, 18, BD_POLICY_BEAMER, &st, d_src, ngraphs))) return rc;
+    if (ms) *ms = st.ms;
+    if (depths) return bd_bfs_fetch(c, depths, nullptr);
     return BD_OK;
 }

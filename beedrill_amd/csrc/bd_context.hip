@@ -145,6 +145,7 @@ extern "C" int bd_create(bd_ctx **out, int device) {
     c->tune.bu_head = 1;
     c->tune.step_kernels = 1;
     c->tune.bu_grid_per_cu = 8;
+    c->tune.learned_margin_milli = 250;
     hipError_t e = hipSetDevice(device);
     if (e == hipSuccess) {
         hipDeviceProp_t prop;
@@ -252,6 +253,7 @@ extern "C" int bd_set_tuning(bd_ctx *c, const bd_tuning *t) {
     if (t->bu_head == 1 || t->bu_head == 2) n.bu_head = t->bu_head;
     if (t->step_kernels == 1 || t->step_kernels == 2) n.step_kernels = t->step_kernels;
     if (t->bu_grid_per_cu > 0) n.bu_grid_per_cu = std::min(t->bu_grid_per_cu, 64);
+    if (t->learned_margin_milli > 0) n.learned_margin_milli = t->learned_margin_milli;
     bool realloc = (n.task_edges != c->tune.task_edges || n.task_threshold != c->tune.task_threshold);
     c->tune = n;
     if (realloc && c->d_off) return alloc_search(c);

@@ -99,6 +99,45 @@ def log_steps(ctx, graph, roots_per_graph):
     return xs, ys, masks, meta
 
 
+MARGINS_MILLI = (1, 125, 250, 500, 1000, 2000)
+
+
+def choose_margin(ctx, params, roots_per_graph=8, log=print):
+    """Pick the override margin on held-out graphs (generator seed 303,
+    scales 20-22): for each margin, the harmonic-mean GTEPS of the learned
+    policy over Beamer's on the same roots, combined as a geometric mean over
+    the graphs; the best margin wins (Beamer itself if no margin gains)."""
+    ctx.mlp_set(params, 32, 32)
+    tune = ctx.get_tuning()
+    ratios = {m: [] for m in MARGINS_MILLI}
+    try:
+        for scale in (20, 21, 22):
+            for a in (0.50, 0.57):
+                bc = (0.95 - a) / 2
+                ctx.generate_rmat(scale, 16, a, bc, bc, 0.05, seed=303)
+                roots = ctx.pick_roots(roots_per_graph)
+
+                def hm(policy):
+                    g = []
+                    for r in roots:
+                        st = ctx.bfs(int(r), 15, 18, policy)
+                        g.append(st["traversed_edges"] / st["ms"])
+                    return len(g) / sum(1.0 / x for x in g)
+
+                hm("beamer")
+                base = hm("beamer")
+                for m in MARGINS_MILLI:
+                    ctx.set_tuning(learned_margin_milli=m)
+                    ratios[m].append(hm("learned") / base)
+                log(f"[margin] scale {scale} A {a:.2f}: " + " ".join(
+                    f"{m}:{ratios[m][-1]:.3f}" for m in MARGINS_MILLI))
+    finally:
+        ctx.set_tuning(learned_margin_milli=tune["learned_margin_milli"])
+    geo = {m: float(np.exp(np.mean(np.log(v)))) for m, v in ratios.items()}
+    best = max(geo, key=geo.get)
+    return best, geo
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--out", default=DEFAULT_OUT)
@@ -134,6 +173,7 @@ def main():
     chosen = np.where(pred[:, 1] < pred[:, 0], bu, td)
     beamer = np.array([m["td_us"] if m["dir"] == bd.DIR_TD else m["bu_us"] for m in META])
     best = np.minimum(td, bu)
+    margin, geo = choose_margin(ctx, params)
     os.makedirs(os.path.dirname(os.path.abspath(args.out)), exist_ok=True)
     params.astype(np.float32).tofile(args.out)
     summary = {
@@ -149,6 +189,8 @@ def main():
                      "min(level,64)/64", "log2(1+max deg F)/log2(1+V)",
                      "log2(1+2M/V)/8", "previous step bottom-up"],
         "targets": ["log2 us top-down step", "log2 us bottom-up step"],
+        "margin_milli": margin,
+        "heldout_learned_over_beamer": geo,
     }
     with open(os.path.splitext(args.out)[0] + ".json", "w") as f:
         json.dump(summary, f, indent=1)

@@ -27,6 +27,9 @@ METRIC = "harmonic-mean GTEPS over 64 roots, RMAT s24; learned vs Beamer policy"
 HBM_PEAK_GBS = 8000.0
 MODEL = os.path.join(ROOT, "beedrill_amd", "models", "direction_mlp_32x32.f32")
 TRAFFIC = os.path.join(ROOT, "profiles", "bottom_up_traffic_rmat_s24.json")
+# The bottom-up instance of the step kernel, as rocprofv3 names it.
+BU_KERNEL = "k_step<256, 1>"
+COPY_BYTES = 256 << 20  # calibration copy: 256 MiB read + 256 MiB written
 
 
 def log(msg):
@@ -68,6 +71,11 @@ def bu_bytes(rec, nv, n_isolated):
 
 
 def roofline(ctx, roots, policy, nv, n_isolated):
+    """Bottom-up kernel against the HBM roofline.  The searches run with the
+    host reading the state between levels, so every bottom-up step is one
+    launch of the bottom-up kernel alone, timed with HIP events on the
+    library's stream; achieved = algorithmic bytes / launch time, summed over
+    the bottom-up launches of these searches."""
     t = ctx.get_tuning()
     ctx.set_tuning(time_steps=1)
     tot_b, tot_ms, n = 0.0, 0.0, 0
@@ -86,11 +94,58 @@ def roofline(ctx, roots, policy, nv, n_isolated):
     if os.path.exists(TRAFFIC):
         with open(TRAFFIC) as f:
             traffic = json.load(f).get("bytes_per_launch")
-    return {"kernel": "k_bottom_up", "bound": "hbm", "achieved": round(achieved, 1),
+    return {"kernel": BU_KERNEL, "bound": "hbm", "achieved": round(achieved, 1),
             "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
             "traffic": traffic, "launches": n,
-            "algorithmic_bytes_per_launch": tot_b / max(n, 1),
-            "avg_launch_ms": tot_ms / max(n, 1)}
+            "algorithmic_bytes_per_launch": round(tot_b / max(n, 1)),
+            "avg_launch_ms": round(tot_ms / max(n, 1), 5)}
+
+
+def copy_calibration():
+    """A plain device copy of known size, for scaling the FETCH_SIZE and
+    WRITE_SIZE counters (run under rocprofv3 --pmc with the roofline leg)."""
+    import torch
+    src = torch.ones(COPY_BYTES // 4, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    for _ in range(3):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+
+
+def summarize_pmc(fetch_csv, write_csv, out_path):
+    """Traffic per bottom-up launch from two rocprofv3 --pmc runs of the
+    roofline leg (FETCH_SIZE, then WRITE_SIZE; KiB per dispatch), each scaled
+    by what the counter reads for the calibration copy of known size."""
+    import csv
+
+    def per_kernel(path, counter):
+        rows = {}
+        with open(path) as f:
+            for row in csv.DictReader(f):
+                if row.get("Counter_Name", counter) != counter:
+                    continue
+                name = row["Kernel_Name"]
+                rows.setdefault(name, []).append(float(row["Counter_Value"]))
+        return rows
+
+    out = {}
+    for counter, path, copy_bytes in (("FETCH_SIZE", fetch_csv, COPY_BYTES),
+                                      ("WRITE_SIZE", write_csv, COPY_BYTES)):
+        rows = per_kernel(path, counter)
+        bu = [v for k, vs in rows.items() if BU_KERNEL in k for v in vs]
+        cp = [v for k, vs in rows.items() if "copy" in k.lower() for v in vs]
+        if not bu or not cp:
+            raise SystemExit(f"{path}: no {BU_KERNEL} or copy dispatches for {counter}")
+        scale = copy_bytes / (float(np.median(cp)) * 1024.0)
+        out[counter] = {"kib_per_launch": float(np.mean(bu)), "launches": len(bu),
+                        "copy_kib": float(np.median(cp)), "scale": scale,
+                        "bytes_per_launch": float(np.mean(bu)) * 1024.0 * scale}
+    out["bytes_per_launch"] = round(out["FETCH_SIZE"]["bytes_per_launch"] +
+                                    out["WRITE_SIZE"]["bytes_per_launch"])
+    out["kernel"] = BU_KERNEL
+    with open(out_path, "w") as f:
+        json.dump(out, f, indent=1)
+    print(json.dumps(out))
 
 
 def cpu_baseline(ctx, roots, budget_s):
@@ -124,7 +179,16 @@ def main():
     ap.add_argument("--cpu-budget", type=float, default=20.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--model", default=MODEL, help="direction network parameters (float32)")
+    ap.add_argument("--leg", choices=("all", "roofline"), default="all",
+                    help="roofline: only the bottom-up roofline searches and the calibration "
+                         "copy (the command profiled with rocprofv3)")
+    ap.add_argument("--summarize-pmc", nargs=2, metavar=("FETCH_CSV", "WRITE_CSV"),
+                    help="write the bottom-up traffic file from two rocprofv3 --pmc runs of "
+                         "--leg roofline, and exit")
     args = ap.parse_args()
+    if args.summarize_pmc:
+        summarize_pmc(args.summarize_pmc[0], args.summarize_pmc[1], TRAFFIC)
+        return
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only; --gpus {args.gpus} is not supported "
               "(the search is not sharded across GPUs in this build)", file=sys.stderr)
@@ -142,16 +206,32 @@ def main():
     off_h = np.empty(nv + 1, np.int64)
     ctx._ok(ctx._L.bd_graph_download(ctx._h, off_h.ctypes.data, None))
     n_isolated = int(np.sum(np.diff(off_h) == 0))
+    have_model = os.path.exists(args.model)
+    margin = None
+    if have_model:
+        ctx.mlp_set(np.fromfile(args.model, np.float32), 32, 32)
+        meta = os.path.splitext(args.model)[0] + ".json"
+        if os.path.exists(meta):
+            with open(meta) as f:
+                margin = json.load(f).get("margin_milli")
+        if margin:
+            ctx.set_tuning(learned_margin_milli=int(margin))
+    policy = "learned" if have_model else "beamer"
+
+    if args.leg == "roofline":
+        for r in roots[:args.warmup]:
+            ctx.bfs(int(r), 15, 18, policy)
+        roof = roofline(ctx, roots[:args.roofline_roots], policy, nv, n_isolated)
+        copy_calibration()
+        print(json.dumps({"leg": "roofline", "roofline": roof}))
+        return
 
     results = {"beamer": run_policy(ctx, roots, "beamer", args.warmup)}
     log(f"beamer: {results['beamer']['hmean_gteps']:.2f} GTEPS (harmonic mean)")
-    have_model = os.path.exists(args.model)
     if have_model:
-        ctx.mlp_set(np.fromfile(args.model, np.float32), 32, 32)
         results["learned"] = run_policy(ctx, roots, "learned", args.warmup)
         log(f"learned: {results['learned']['hmean_gteps']:.2f} GTEPS (harmonic mean)")
     head = results["learned"] if have_model else results["beamer"]
-    policy = "learned" if have_model else "beamer"
     roof = roofline(ctx, roots[:args.roofline_roots], policy, nv, n_isolated)
     log(f"roofline: {roof['achieved']} GB/s over {roof['launches']} bottom-up launches")
     cpu = None if args.no_cpu else cpu_baseline(ctx, roots, args.cpu_budget)
@@ -173,6 +253,7 @@ def main():
         "config": {"workload": f"RMAT scale {args.scale}, edge factor {args.edge_factor}, "
                                f"A/B/C/D .57/.19/.19/.05, seed {args.seed}, {args.steps} lcg(0) "
                                f"roots, alpha 15 beta 18, policy {policy}",
+                   "learned_margin_milli": ctx.get_tuning()["learned_margin_milli"],
                    "num_vertices": nv, "num_edges": m, "nnz": nnz,
                    "graph_build_s": round(gen_s, 2)},
         "roofline": roof,

@@ -43,8 +43,10 @@ typedef enum {
 
 /* Direction policies.  BEAMER is run_beamer (hybrid_bfs.cc:152-191);
  * TD_ONLY is run_with_migrating_threads (hybrid_bfs.cc:197-213); LEARNED
- * replaces the two predicates at hybrid_bfs.cc:167 and :178-179 with the MLP
- * cost model and keeps Beamer as the fallback when no model is loaded;
+ * evaluates the MLP cost model beside the two predicates at
+ * hybrid_bfs.cc:167 and :178-179 and takes its direction when it predicts a
+ * gain larger than bd_tuning.learned_margin_milli; with no model loaded it
+ * is Beamer;
  * BU_ONLY and SCRIPT exist for testing and for timing both directions. */
 typedef enum {
     BD_POLICY_BEAMER = 0,
@@ -205,27 +207,40 @@ typedef struct {
                                 the device returns at once); 2 = one kernel
                                 carrying both directions                  */
     int32_t bu_grid_per_cu;  /* workgroups per CU of the bottom-up kernel  */
-    int32_t reserved[4];
+    int32_t learned_margin_milli; /* LEARNED: the network overrides Beamer's
+                                choice only when it predicts the other
+                                direction cheaper by more than this many
+                                thousandths of a log2 unit (1 = the network
+                                decides alone)                            */
+    int32_t reserved[3];
 } bd_tuning;
 int bd_set_tuning(bd_ctx *ctx, const bd_tuning *t);
 int bd_get_tuning(bd_ctx *ctx, bd_tuning *t);
 
 /* ------------------------------------------------------- batched searches */
 
-/* 64-root bit-parallel search (no reference counterpart; replaces 64 trips
- * round the trial loop hybrid_bfs_main.cc:252-302).  n <= 64 roots share one
- * launch sequence; depths[r*nv + v] (may be NULL) receives root r's depths. */
+/* Bit-parallel search from up to 64 roots at once on the resident graph (no
+ * reference counterpart; replaces up to 64 trips round the sequential trial
+ * loop, hybrid_bfs_main.cc:252-302).  Every vertex carries a 64-bit mask of
+ * the roots that reached it; each level pushes frontier masks to neighbours
+ * or pulls them from neighbours, chosen per level from the frontier's and
+ * the open vertices' degree sums.  depths (may be NULL) receives
+ * depths[r*nv + v] = depth of v from roots[r] (-1 unreached); *ms the time
+ * from the first kernel to the last; *levels the number of levels. */
 int bd_bfs_multi_root(bd_ctx *ctx, const int64_t *roots, int n, int32_t *depths,
                       double *ms, int64_t *levels);
 
-/* Many independent graphs, one root each, in one launch (no reference
- * counterpart).  Graphs are given as concatenated CSRs: goff[g] is the first
- * vertex of graph g in the concatenated offsets, gcol[g] its first column.
- * Offsets are local to each graph.  depths (may be NULL) is concatenated in
- * the same vertex order. */
-int bd_bfs_multi_graph(bd_ctx *ctx, int ngraphs, const int64_t *vbase, const int64_t *offsets,
-                       const int32_t *cols, const int64_t *roots, int32_t *depths,
-                       double *ms);
+/* Many disjoint graphs searched at once (no reference counterpart).  The
+ * resident graph is their union: graph g owns vertices [vbase[g],
+ * vbase[g+1]) (vbase has ngraphs+1 entries, vbase[0] = 0, vbase[ngraphs] =
+ * nv) and no edge joins two graphs (BD_EINVAL otherwise).  roots[g] is a
+ * vertex of graph g, numbered within it.  One direction-optimizing search
+ * (Beamer, alpha 15, beta 18) starts from every root; depths (may be NULL,
+ * nv entries) gets each vertex's depth from its own graph's root, *ms the
+ * device time of the search.  bd_bfs_fetch / bd_bfs_levels then describe
+ * this search. */
+int bd_bfs_multi_graph(bd_ctx *ctx, int ngraphs, const int64_t *vbase, const int64_t *roots,
+                       int32_t *depths, double *ms);
 
 /* --------------------------------------------------- learned direction MLP */
 

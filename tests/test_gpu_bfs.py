@@ -283,3 +283,92 @@ def test_s22_depth_bit_exact_and_parents_valid(ctx, orc):
         assert 1 + sum(r["next"] for r in lv) == st["reached"] == int((depth >= 0).sum())
         assert st["traversed_edges"] == int(deg[depth >= 0].sum()) // 2
         assert any(r["dir"] == bd.DIR_BU for r in lv)
+
+
+# ------------------------------------------------------------ batched searches
+
+def test_multi_root_matches_oracle(ctx, orc):
+    """64 roots in one bit-parallel search: every root's depth array is the
+    oracle's textbook BFS from that root."""
+    ctx.generate_rmat(12, 16, seed=5)
+    off, adj = ctx.csr()
+    roots = ctx.pick_roots(64)
+    dp, ms, levels = ctx.multi_root(roots)
+    assert dp.shape == (64, len(off) - 1) and ms > 0
+    for r, root in enumerate(roots):
+        np.testing.assert_array_equal(dp[r], orc.bfs_depth(off, adj, int(root)), err_msg=str(r))
+    assert levels == int(dp.max()) + 1
+    # fewer roots, a repeated root, an isolated root, then the 64 again
+    deg = np.diff(off)
+    iso = int(np.flatnonzero(deg == 0)[0])
+    few = np.array([roots[3], iso, roots[3], roots[7], 0], np.int64)
+    dp2, _, _ = ctx.multi_root(few)
+    for r, root in enumerate(few):
+        np.testing.assert_array_equal(dp2[r], orc.bfs_depth(off, adj, int(root)))
+    dp3, _, _ = ctx.multi_root(roots)
+    np.testing.assert_array_equal(dp3, dp)
+
+
+def test_multi_root_s16_and_lattice(ctx, orc):
+    ctx.generate_rmat(16, 16)
+    off, adj = ctx.csr()
+    roots = ctx.pick_roots(64)
+    dp, _, _ = ctx.multi_root(roots)
+    for r in (0, 17, 63):
+        np.testing.assert_array_equal(dp[r], orc.bfs_depth(off, adj, int(roots[r])))
+    _, ms, _ = ctx.multi_root(roots, want_depths=False)
+    assert ms > 0
+    ctx.generate_lattice(40, 50, seed=1)  # long, thin levels: every level pushes or pulls little
+    off, adj = ctx.csr()
+    roots = np.array([0, 1999, 1000, 77], np.int64)
+    dp, _, levels = ctx.multi_root(roots)
+    for r, root in enumerate(roots):
+        np.testing.assert_array_equal(dp[r], orc.bfs_depth(off, adj, int(root)))
+
+
+def test_multi_root_errors(ctx):
+    nv, s, d = path_graph(10)
+    ctx.from_edges(nv, s, d)
+    for roots, code in [(np.zeros(0, np.int64), -1), (np.zeros(65, np.int64), -1),
+                        (np.array([3, 10], np.int64), -6), (np.array([-1], np.int64), -6)]:
+        with pytest.raises(bd.BeedrillError) as e:
+            ctx.multi_root(roots)
+        assert e.value.code == code
+
+
+def test_multi_graph_matches_oracle(ctx, orc):
+    """Many disjoint graphs of uneven size searched from their roots at once:
+    each graph's depths are the oracle's BFS of that graph alone."""
+    rng = np.random.default_rng(3)
+    graphs, roots = [], []
+    for g in range(24):
+        scale, ef = int(rng.integers(4, 11)), int(rng.integers(8, 33))
+        nv, s, d = orc.make_rmat(scale, ef, seed=100 + g)
+        off, adj = orc.build_csr(nv, s, d)
+        graphs.append((off, adj, len(s)))
+        roots.append(int(orc.pick_roots(off, 1)[0]) if g != 5 else
+                     int(np.flatnonzero(np.diff(off) == 0)[0]) if (np.diff(off) == 0).any() else 0)
+    vbase = ctx.upload_union(graphs)
+    dp, ms = ctx.multi_graph(vbase, roots)
+    assert ms > 0
+    for g, (off, adj, _) in enumerate(graphs):
+        np.testing.assert_array_equal(dp[vbase[g]:vbase[g + 1]],
+                                      orc.bfs_depth(off, adj, roots[g]), err_msg=str(g))
+    parent = ctx.fetch()[1]
+    for g, (off, adj, _) in enumerate(graphs):
+        p = parent[vbase[g]:vbase[g + 1]]
+        assert orc.check(off, adj, roots[g], np.where(p >= 0, p - vbase[g], -1)), g
+    lv = ctx.levels()
+    assert lv[0]["frontier"] == len(set(int(vbase[g]) + r for g, r in enumerate(roots)))
+    # errors: an edge between graphs, bad boundaries, a root outside its graph
+    with pytest.raises(bd.BeedrillError) as e:
+        ctx.multi_graph(np.array([0, 3, vbase[-1]]), [0, 0])
+    assert e.value.code == -1
+    with pytest.raises(bd.BeedrillError) as e:
+        ctx.multi_graph(vbase[:-1], roots[:-1])
+    assert e.value.code == -1
+    bad = list(roots)
+    bad[2] = int(vbase[3] - vbase[2])
+    with pytest.raises(bd.BeedrillError) as e:
+        ctx.multi_graph(vbase, bad)
+    assert e.value.code == -6
