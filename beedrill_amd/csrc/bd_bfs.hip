@@ -128,6 +128,10 @@ struct step_smem {
             int32_t t_med[BLOCK];
             int32_t t_wsum[BLOCK / 64];
             int32_t t_nmed;
+            // bitmap frontier: a range's words and their rank prefix
+            uint64_t w_bits[BLOCK];
+            int32_t w_pref[BLOCK + 1];
+            int32_t w_wsum[BLOCK / 64];
         };
         struct {  // bottom-up: found bits of each wave's chunk, head misses
             uint64_t b_found[BLOCK / 64][64];
@@ -319,86 +323,87 @@ __device__ __forceinline__ void td_visit(const step_args &a, step_smem<BLOCK> &s
     emit_next(a, s, won, u, b, e);
 }
 
+// One tile of frontier vertices, one per thread (v < 0: none).  Vertices of
+// degree >= wave_t get a wave each; the edges of the others are concatenated
+// by a block-wide scan and every thread takes one.  Whole workgroup.
 template <int BLOCK>
-__device__ void td_body(const step_args &a, step_smem<BLOCK> &s, unsigned long long &cnt,
-                        unsigned long long &dsum, unsigned long long &dmax) {
+__device__ void td_tile(const step_args &a, step_smem<BLOCK> &s, int32_t v,
+                        unsigned long long &cnt, unsigned long long &dsum,
+                        unsigned long long &dmax) {
     const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
-    const int64_t nitems = s.st.qpref[kSeg + 1];
-    const int64_t ntasks = s.st.tpref[kSeg + 1];
-
-    for (int64_t tile = blockIdx.x; tile * BLOCK < nitems; tile += gridDim.x) {
-        if (tid == 0) s.t_nmed = 0;
-        __syncthreads();
-        const int64_t slot = tile * BLOCK + tid;
-        int32_t v = -1;
-        if (slot < nitems) {
-            const int sg = seg_of(s.st.qpref, slot);
-            v = a.qin[(int64_t)sg * a.qseg_cap + (slot - s.st.qpref[sg])];
-        }
-        int64_t b = 0, e = 0;
-        if (v >= 0) {
-            b = a.off[v];
-            e = a.off[v + 1];
-        }
-        const int64_t dg = e - b;
-        int32_t small = 0;
-        if (dg >= a.wave_t) {
-            const int p = atomicAdd(&s.t_nmed, 1);
-            s.t_med[p] = tid;
-        } else {
-            small = (int32_t)dg;
-        }
-        s.t_beg[tid] = b;
-        s.t_end[tid] = e;
-        s.t_src[tid] = v;
-        int32_t incl = small;
+    if (tid == 0) s.t_nmed = 0;
+    __syncthreads();
+    int64_t b = 0, e = 0;
+    if (v >= 0) {
+        b = a.off[v];
+        e = a.off[v + 1];
+    }
+    const int64_t dg = e - b;
+    int32_t small = 0;
+    if (dg >= a.wave_t) {
+        const int p = atomicAdd(&s.t_nmed, 1);
+        s.t_med[p] = tid;
+    } else {
+        small = (int32_t)dg;
+    }
+    s.t_beg[tid] = b;
+    s.t_end[tid] = e;
+    s.t_src[tid] = v;
+    int32_t incl = small;
 #pragma unroll
-        for (int o = 1; o < 64; o <<= 1) {
-            const int32_t y = __shfl_up(incl, o);
-            if (lane >= o) incl += y;
-        }
-        if (lane == 63) s.t_wsum[wid] = incl;
-        __syncthreads();
-        int32_t wbase = 0;
-        for (int w2 = 0; w2 < wid; ++w2) wbase += s.t_wsum[w2];
-        s.t_pref[tid] = wbase + incl - small;
-        if (tid == BLOCK - 1) s.t_pref[BLOCK] = wbase + incl;
-        __syncthreads();
-        const int32_t total = s.t_pref[BLOCK];
-        const int nmed = s.t_nmed;
-        for (int i = wid; i < nmed; i += BLOCK / 64) {
-            const int t = s.t_med[i];
-            const int64_t mb = s.t_beg[t], me = s.t_end[t];
-            const int32_t src = s.t_src[t];
-            for (int64_t k0 = mb; k0 < me; k0 += 64) {
-                const int64_t k = k0 + lane;
-                const bool valid = k < me;
-                const int32_t u = valid ? a.cols[k] : 0;
-                td_visit(a, s, valid, u, src, cnt, dsum, dmax);
-            }
-        }
-        for (int32_t i0 = 0; i0 < total; i0 += BLOCK) {
-            const int32_t i = i0 + tid;
-            const bool valid = i < total;
-            int32_t u = 0, src = 0;
-            if (valid) {
-                int lo = 0, hi = BLOCK;
-                while (hi - lo > 1) {
-                    const int mid = (lo + hi) >> 1;
-                    if (s.t_pref[mid] <= i)
-                        lo = mid;
-                    else
-                        hi = mid;
-                }
-                src = s.t_src[lo];
-                u = a.cols[s.t_beg[lo] + (i - s.t_pref[lo])];
-            }
+    for (int o = 1; o < 64; o <<= 1) {
+        const int32_t y = __shfl_up(incl, o);
+        if (lane >= o) incl += y;
+    }
+    if (lane == 63) s.t_wsum[wid] = incl;
+    __syncthreads();
+    int32_t wbase = 0;
+    for (int w2 = 0; w2 < wid; ++w2) wbase += s.t_wsum[w2];
+    s.t_pref[tid] = wbase + incl - small;
+    if (tid == BLOCK - 1) s.t_pref[BLOCK] = wbase + incl;
+    __syncthreads();
+    const int32_t total = s.t_pref[BLOCK];
+    const int nmed = s.t_nmed;
+    for (int i = wid; i < nmed; i += BLOCK / 64) {
+        const int t = s.t_med[i];
+        const int64_t mb = s.t_beg[t], me = s.t_end[t];
+        const int32_t src = s.t_src[t];
+        for (int64_t k0 = mb; k0 < me; k0 += 64) {
+            const int64_t k = k0 + lane;
+            const bool valid = k < me;
+            const int32_t u = valid ? a.cols[k] : 0;
             td_visit(a, s, valid, u, src, cnt, dsum, dmax);
         }
-        __syncthreads();
-        if (s.qn >= kQCap / 2) flush_queue<BLOCK>(a, s);
     }
-    // task slices, one wave each, in workgroup-synchronous rounds
+    for (int32_t i0 = 0; i0 < total; i0 += BLOCK) {
+        const int32_t i = i0 + tid;
+        const bool valid = i < total;
+        int32_t u = 0, src = 0;
+        if (valid) {
+            int lo = 0, hi = BLOCK;
+            while (hi - lo > 1) {
+                const int mid = (lo + hi) >> 1;
+                if (s.t_pref[mid] <= i)
+                    lo = mid;
+                else
+                    hi = mid;
+            }
+            src = s.t_src[lo];
+            u = a.cols[s.t_beg[lo] + (i - s.t_pref[lo])];
+        }
+        td_visit(a, s, valid, u, src, cnt, dsum, dmax);
+    }
+    __syncthreads();
+    if (s.qn >= kQCap / 2) flush_queue<BLOCK>(a, s);
+}
+
+// Task slices of the frontier's high-degree vertices, one wave each, in
+// workgroup-synchronous rounds.
+template <int BLOCK>
+__device__ void td_tasks(const step_args &a, step_smem<BLOCK> &s, unsigned long long &cnt,
+                         unsigned long long &dsum, unsigned long long &dmax) {
+    const int lane = lane_id(), wid = threadIdx.x >> 6;
+    const int64_t ntasks = s.st.tpref[kSeg + 1];
     for (int64_t t0 = (int64_t)blockIdx.x * (BLOCK / 64); t0 < ntasks;
          t0 += (int64_t)gridDim.x * (BLOCK / 64)) {
         const int64_t t = t0 + wid;
@@ -417,71 +422,69 @@ __device__ void td_body(const step_args &a, step_smem<BLOCK> &s, unsigned long l
     }
 }
 
+// Top-down step over the frontier lists (queue tiles, then task slices).
+template <int BLOCK>
+__device__ void td_body(const step_args &a, step_smem<BLOCK> &s, unsigned long long &cnt,
+                        unsigned long long &dsum, unsigned long long &dmax) {
+    const int64_t nitems = s.st.qpref[kSeg + 1];
+    for (int64_t tile = blockIdx.x; tile * BLOCK < nitems; tile += gridDim.x) {
+        const int64_t slot = tile * BLOCK + threadIdx.x;
+        int32_t v = -1;
+        if (slot < nitems) {
+            const int sg = seg_of(s.st.qpref, slot);
+            v = a.qin[(int64_t)sg * a.qseg_cap + (slot - s.st.qpref[sg])];
+        }
+        td_tile<BLOCK>(a, s, v, cnt, dsum, dmax);
+    }
+    td_tasks<BLOCK>(a, s, cnt, dsum, dmax);
+}
+
 // Top-down step whose frontier is only in the depth-L bitmap (the previous
-// step was bottom-up, which does not build lists).  Each wave takes chunks of
-// bu_words frontier words, ranks their set bits, and takes 64 frontier
-// vertices at a time: short lists are walked by their own lane in lockstep,
-// lists of degree >= wave_t by the whole wave.
+// step was bottom-up, which does not build lists).  Each workgroup takes
+// ranges of BLOCK frontier words, ranks their set bits with a block-wide
+// scan, and feeds them to td_tile BLOCK vertices at a time.
 template <int BLOCK>
 __device__ void td_bitmap_body(const step_args &a, step_smem<BLOCK> &s, unsigned long long &cnt,
                                unsigned long long &dsum, unsigned long long &dmax) {
-    const int lane = lane_id(), wid = threadIdx.x >> 6;
-    const int CW = a.bu_words;
-    const int64_t nchunks = (a.nwords + CW - 1) / CW;
-    for (int64_t c0 = (int64_t)blockIdx.x * (BLOCK / 64); c0 < nchunks;
-         c0 += (int64_t)gridDim.x * (BLOCK / 64)) {
-        const int64_t chunk = c0 + wid;
-        if (chunk < nchunks) {
-            const int64_t w0 = chunk * CW;
-            const int64_t wi = w0 + lane;
-            const uint64_t fw = (lane < CW && wi < a.nwords) ? a.front[wi] : 0ULL;
-            const int cntw = __popcll(fw);
-            int excl = (int)wave_incl_scan(cntw);
-            const int total = __shfl(excl, 63);
-            excl -= cntw;
-            for (int r = 0; r < total; r += 64) {
-                const int g = r + lane;
-                const bool valid = g < total;
-                const int gg = valid ? g : total - 1;
-                int j = 0;
+    const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+    const int64_t nranges = (a.nwords + BLOCK - 1) / BLOCK;
+    for (int64_t r = blockIdx.x; r < nranges; r += gridDim.x) {
+        const int64_t wi = r * BLOCK + tid;
+        const uint64_t w = wi < a.nwords ? a.front[wi] : 0ULL;
+        const int32_t c = __popcll(w);
+        s.w_bits[tid] = w;
+        int32_t incl = c;
 #pragma unroll
-                for (int st = 32; st > 0; st >>= 1) {
-                    const int mid = j + st;
-                    const int em = __shfl(excl, mid < 64 ? mid : 63);
-                    if (mid < 64 && em <= gg) j = mid;
+        for (int o = 1; o < 64; o <<= 1) {
+            const int32_t y = __shfl_up(incl, o);
+            if (lane >= o) incl += y;
+        }
+        if (lane == 63) s.w_wsum[wid] = incl;
+        __syncthreads();
+        int32_t wbase = 0;
+        for (int w2 = 0; w2 < wid; ++w2) wbase += s.w_wsum[w2];
+        s.w_pref[tid] = wbase + incl - c;
+        if (tid == BLOCK - 1) s.w_pref[BLOCK] = wbase + incl;
+        __syncthreads();
+        const int32_t total = s.w_pref[BLOCK];
+        for (int32_t t0 = 0; t0 < total; t0 += BLOCK) {
+            const int32_t i = t0 + tid;
+            int32_t v = -1;
+            if (i < total) {
+                int lo = 0, hi = BLOCK;  // largest j with w_pref[j] <= i
+                while (hi - lo > 1) {
+                    const int mid = (lo + hi) >> 1;
+                    if (s.w_pref[mid] <= i)
+                        lo = mid;
+                    else
+                        hi = mid;
                 }
-                const uint64_t bits = (uint64_t)__shfl((long long)fw, j);
-                const int bpos = select_bit(bits, gg - __shfl(excl, j));
-                const int32_t v = (int32_t)((w0 + j) * 64 + bpos);
-                int64_t b = 0, e = 0;
-                if (valid) {
-                    b = a.off[v];
-                    e = a.off[v + 1];
-                }
-                const bool small = valid && (e - b) < a.wave_t;
-                const int64_t maxd = (int64_t)wave_max(small ? (unsigned long long)(e - b) : 0ULL);
-                for (int64_t k = 0; k < maxd; ++k) {
-                    const bool vv = small && k < e - b;
-                    const int32_t u = vv ? a.cols[b + k] : 0;
-                    td_visit(a, s, vv, u, v, cnt, dsum, dmax);
-                }
-                uint64_t big = __ballot(valid && !small);
-                while (big) {
-                    const int o = __ffsll((unsigned long long)big) - 1;
-                    big &= big - 1;
-                    const int64_t bo = __shfl(b, o), eo = __shfl(e, o);
-                    const int32_t vo = __shfl(v, o);
-                    for (int64_t k0 = bo; k0 < eo; k0 += 64) {
-                        const int64_t k = k0 + lane;
-                        const bool vv = k < eo;
-                        const int32_t u = vv ? a.cols[k] : 0;
-                        td_visit(a, s, vv, u, vo, cnt, dsum, dmax);
-                    }
-                }
+                v = (int32_t)((r * BLOCK + lo) * 64 +
+                              select_bit(s.w_bits[lo], i - s.w_pref[lo]));
             }
+            td_tile<BLOCK>(a, s, v, cnt, dsum, dmax);
         }
         __syncthreads();
-        if (s.qn >= kQCap / 2) flush_queue<BLOCK>(a, s);
     }
 }
 
@@ -1270,7 +1273,9 @@ void launch_step_block(bd_ctx *c, const step_args &a, int dirs, int64_t grid) {
 int launch_step(bd_ctx *c, const step_args &a, int dirs, int64_t grid = 0) {
     const int blk = norm_block(c->tune.block);
     if (grid <= 0) {
-        const int per_cu = dirs == kStepBU ? c->tune.bu_grid_per_cu : c->tune.grid_per_cu;
+        // the bottom-up grid is its own only when it has its own kernel
+        const int per_cu = (dirs == kStepBU && c->tune.step_kernels == 1) ? c->tune.bu_grid_per_cu
+                                                                         : c->tune.grid_per_cu;
         grid = (int64_t)c->cu_count * std::max(1, per_cu);
     }
     switch (blk) {
@@ -1421,7 +1426,8 @@ int run_search(bd_ctx *c, int64_t source, int64_t alpha, int64_t beta, int polic
             const step_args a = make_args(c, L, alpha, beta, policy);
             if ((rc = close_level(c, a))) return rc;
             if (c->h_ctl[L & 1].done) break;
-            const int dirs = c->tune.step_kernels == 2 ? kStepBoth : c->h_ctl[L & 1].dir;
+            // the host knows the direction: launch that direction's kernel
+            const int dirs = c->h_ctl[L & 1].dir;
             if (timed) BD_HIP(c, hipEventRecord(c->ev[2], c->stream));
             if ((rc = launch_step(c, a, dirs))) return rc;
             if (timed) {
@@ -1631,8 +1637,7 @@ extern "C" int bd_bfs_profile(bd_ctx *c, int64_t source, int64_t alpha, int64_t 
             hipLaunchKernelGGL(k_scrub, dim3(c->cu_count * 4), dim3(256), 0, c->stream, scrub,
                                scrub_n, scrub + scrub_n);
             BD_HIP(c, hipEventRecord(c->ev[2], c->stream));
-            if ((rc = launch_step(c, a, c->tune.step_kernels == 2 ? kStepBoth : a.force_dir)))
-                return rc;
+            if ((rc = launch_step(c, a, a.force_dir))) return rc;
             BD_HIP(c, hipEventRecord(c->ev[3], c->stream));
             BD_HIP(c, hipEventSynchronize(c->ev[3]));
             BD_HIP(c, hipEventElapsedTime(&ms[pass], c->ev[2], c->ev[3]));

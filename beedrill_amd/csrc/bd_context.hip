@@ -143,7 +143,7 @@ extern "C" int bd_create(bd_ctx **out, int device) {
     c->tune.level_driver = 1;
     c->tune.time_steps = 2;
     c->tune.bu_head = 1;
-    c->tune.step_kernels = 1;
+    c->tune.step_kernels = 2;
     c->tune.bu_grid_per_cu = 8;
     c->tune.learned_margin_milli = 250;
     hipError_t e = hipSetDevice(device);

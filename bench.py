@@ -148,6 +148,17 @@ def summarize_pmc(fetch_csv, write_csv, out_path):
     print(json.dumps(out))
 
 
+def batched_roots(ctx, roots, sequential_ms):
+    """The same 64 roots in one bit-parallel search (depths of every root
+    written), against the sum of the 64 sequential searches' times."""
+    ctx.multi_root(roots[:8], want_depths=False)  # warm up
+    dp, ms, levels = ctx.multi_root(roots, want_depths=True)
+    del dp
+    return {"roots": len(roots), "ms": round(ms, 4), "levels": int(levels),
+            "sequential_ms": round(sequential_ms, 4),
+            "speedup_vs_sequential": round(sequential_ms / ms, 3) if ms > 0 else None}
+
+
 def cpu_baseline(ctx, roots, budget_s):
     import oracle
     off, adj = ctx.csr()
@@ -232,6 +243,8 @@ def main():
         results["learned"] = run_policy(ctx, roots, "learned", args.warmup)
         log(f"learned: {results['learned']['hmean_gteps']:.2f} GTEPS (harmonic mean)")
     head = results["learned"] if have_model else results["beamer"]
+    batch = batched_roots(ctx, roots, results["beamer"]["mean_ms"] * len(roots))
+    log(f"64 roots batched: {batch['ms']:.3f} ms vs {batch['sequential_ms']:.3f} ms sequential")
     roof = roofline(ctx, roots[:args.roofline_roots], policy, nv, n_isolated)
     log(f"roofline: {roof['achieved']} GB/s over {roof['launches']} bottom-up launches")
     cpu = None if args.no_cpu else cpu_baseline(ctx, roots, args.cpu_budget)
@@ -258,6 +271,7 @@ def main():
                    "graph_build_s": round(gen_s, 2)},
         "roofline": roof,
         "cpu_baseline": cpu,
+        "batched_64_roots": batch,
         "policies": {k: {kk: vv for kk, vv in v.items() if kk != "per_root_gteps"}
                      for k, v in results.items()},
         "learned_vs_beamer": (round(results["learned"]["hmean_gteps"] /

@@ -202,11 +202,14 @@ typedef struct {
     int32_t bu_head;         /* bottom-up: 1 = test each vertex's head index
                                 (its highest-degree neighbour) before its
                                 list; 2 = scan lists only, in CSR order   */
-    int32_t step_kernels;    /* 1 = one step kernel per direction (both are
-                                launched per level; the one not chosen on
-                                the device returns at once); 2 = one kernel
-                                carrying both directions                  */
-    int32_t bu_grid_per_cu;  /* workgroups per CU of the bottom-up kernel  */
+    int32_t step_kernels;    /* device-driven levels: 1 = one step kernel per
+                                direction (both are launched per level; the
+                                one not chosen on the device returns after
+                                its state rebuild); 2 = one kernel carrying
+                                both directions.  Host-driven levels always
+                                launch the chosen direction's kernel      */
+    int32_t bu_grid_per_cu;  /* workgroups per CU of the bottom-up kernel
+                                (step_kernels 1; otherwise grid_per_cu)   */
     int32_t learned_margin_milli; /* LEARNED: the network overrides Beamer's
                                 choice only when it predicts the other
                                 direction cheaper by more than this many
