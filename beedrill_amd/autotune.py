@@ -33,12 +33,12 @@ import beedrill_amd as bd  # noqa: E402
 # Values tried per field (all within the ranges bd_set_tuning accepts).
 SPACE = {
     "bu_grid_per_cu": [4, 6, 8, 12, 16],
-    "bu_words": [8, 16, 32, 48, 64],
+    "bu_words": [16, 32, 48, 64],
     "bu_cap": [8, 16, 32, 64, 128],
-    "grid_per_cu": [2, 4, 6, 8, 12, 16],
-    "wave_threshold": [8, 16, 32, 64, 128],
-    "task_threshold": [128, 256, 512, 1024, 4096],
-    "task_edges": [64, 128, 256, 512],
+    "grid_per_cu": [2, 3, 4, 5, 6, 8, 12],
+    "wave_threshold": [16, 32, 64, 128, 256],
+    "task_threshold": [16, 32, 48, 64, 96, 128, 256],
+    "task_edges": [64, 128, 192, 256, 384, 512],
     "block": [128, 256, 512],
     "step_kernels": [1, 2],
     "bu_head": [1, 2],

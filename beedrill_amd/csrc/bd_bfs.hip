@@ -53,12 +53,11 @@ namespace {
 
 constexpr int kMaxLevelsPerSync = 64;
 constexpr int kQCap = 2048;  // LDS-staged queue entries per workgroup
-constexpr int kMaxBlock = 512;
 
 struct step_args {
     const int64_t *off;
     const int32_t *cols;
-    const int32_t *head;   // bottom-up head index (neighbour of largest degree)
+    const uint64_t *head;  // bottom-up head index: (degree << 32) | neighbour of largest degree
     uint64_t *vis;
     const uint64_t *front;
     uint64_t *next_bm;
@@ -74,6 +73,7 @@ struct step_args {
     bd_level_rec *log;
     const int32_t *script;
     const float *mlp;
+    const uint4 *mlp_ops;  // the network's operands in MFMA lane order
     int64_t nwords;
     int64_t task_cap;
     int64_t qseg_cap, tseg_cap;
@@ -102,10 +102,9 @@ __device__ __forceinline__ int seg_of(const int64_t *pref, int64_t i) {
     return s;
 }
 
-// Per-workgroup LDS state.  The network scratch of the direction choice
-// (used before the step body starts), the top-down staging and tile arrays
-// and the bottom-up found words and pending lists share one union; it is
-// sized by the workgroup so that 8 workgroups of 256 fit in a CU's LDS.
+// Per-workgroup LDS state.  The top-down staging and tile arrays and the
+// bottom-up found words, pending lists and first hits share one union, sized
+// by the workgroup.
 template <int BLOCK>
 struct step_smem {
     ctl_block st;  // the search state before this step
@@ -118,7 +117,6 @@ struct step_smem {
     int64_t qpos, qopos;
     int32_t qfit;
     union {
-        mlp_scratch mlp;
         struct {  // top-down
             int32_t q[kQCap];
             int64_t t_beg[BLOCK];
@@ -133,9 +131,11 @@ struct step_smem {
             int32_t w_pref[BLOCK + 1];
             int32_t w_wsum[BLOCK / 64];
         };
-        struct {  // bottom-up: found bits of each wave's chunk, head misses
+        struct {  // bottom-up: found bits of each wave's chunk, head misses,
+                  // first hit per pending list
             uint64_t b_found[BLOCK / 64][64];
             int32_t b_pend[BLOCK / 64][128];
+            unsigned long long b_hit[BLOCK / 64][64];
         };
     };
 };
@@ -490,51 +490,14 @@ __device__ void td_bitmap_body(const step_args &a, step_smem<BLOCK> &s, unsigned
 
 // --------------------------------------------------------------- bottom-up
 
-// Scan up to `cap` neighbours from position *kp in CSR order; returns the
-// first one in the frontier or -1, and the position reached in *kp.  The
-// first neighbour is tested alone (most vertices stop there), then four at a
-// time.
-__device__ __forceinline__ int32_t bu_scan(const step_args &a, int64_t *kp, int64_t e,
-                                           int64_t cap, unsigned long long &exam) {
-    int64_t k = *kp;
-    const int64_t stop = (k + cap < e) ? k + cap : e;
-    if (k < stop) {
-        const int32_t u = a.cols[k];
-        ++exam;
-        ++k;
-        if ((a.front[u >> 6] >> (u & 63)) & 1) {
-            *kp = k;
-            return u;
-        }
-    }
-    while (k < stop) {
-        int32_t u[4];
-        uint64_t w[4];
-#pragma unroll
-        for (int j = 0; j < 4; ++j) u[j] = (k + j < stop) ? a.cols[k + j] : -1;
-#pragma unroll
-        for (int j = 0; j < 4; ++j) w[j] = (u[j] >= 0) ? a.front[u[j] >> 6] : 0;
-#pragma unroll
-        for (int j = 0; j < 4; ++j) {
-            if (u[j] >= 0) {
-                ++exam;
-                if ((w[j] >> (u[j] & 63)) & 1) {
-                    *kp = k + j + 1;
-                    return u[j];
-                }
-            }
-        }
-        k += 4;
-    }
-    *kp = stop;
-    return -1;
-}
-
 // Scan the lists of the wave's pending vertices (those whose head index is
-// not in the frontier): lane i takes pend[i], i < n, and tests its list in
-// CSR order; lists longer than bu_cap are finished by the whole wave.  Found
-// vertices get depth and parent and their bit in the wave's found words
-// (chunk word (v >> 6) - w0).
+// not in the frontier), pend[i] for i < n.  The first bu_cap neighbours of
+// all n lists are laid end to end and taken 64 at a time, one per lane;
+// a hit records (position << 32 | neighbour) with an LDS atomic minimum, so
+// each vertex keeps its first hit in CSR order, as a sequential scan would.
+// Lists that hold no hit in their first bu_cap neighbours are finished by
+// the whole wave.  Found vertices get depth and parent and their bit in the
+// wave's found words (chunk word (v >> 6) - w0).
 template <int BLOCK>
 __device__ void bu_pending(const step_args &a, step_smem<BLOCK> &s, const int32_t *pend, int n,
                            int64_t w0, unsigned long long &cnt, unsigned long long &dsum,
@@ -542,21 +505,55 @@ __device__ void bu_pending(const step_args &a, step_smem<BLOCK> &s, const int32_
     const int lane = lane_id(), wid = threadIdx.x >> 6;
     const bool valid = lane < n;
     const int64_t v = valid ? pend[lane] : 0;
-    int32_t par = -1;
-    int64_t b = 0, e = 0, k = 0;
+    int64_t b = 0, e = 0;
     if (valid) {
         b = a.off[v];
         e = a.off[v + 1];
-        k = b;
-        par = bu_scan(a, &k, e, a.bu_cap, exam);
     }
-    uint64_t lm = __ballot(valid && par < 0 && k < e);
+    const int32_t len = (int32_t)(e - b < a.bu_cap ? e - b : a.bu_cap);
+    int32_t excl = (int32_t)wave_incl_scan(len);
+    const int32_t total = __shfl(excl, 63);
+    excl -= len;
+    unsigned long long *hit = s.b_hit[wid];
+    hit[lane] = ~0ULL;
+    wave_lds_sync();
+    for (int32_t k0 = 0; k0 < total; k0 += 64) {
+        const int32_t idx = k0 + lane;
+        // owner: the largest lane i with excl_i <= idx
+        int i = 0;
+#pragma unroll
+        for (int st = 32; st > 0; st >>= 1) {
+            const int mid = i + st;
+            const int32_t em = __shfl(excl, mid < 64 ? mid : 63);
+            if (mid < 64 && em <= idx) i = mid;
+        }
+        const int32_t pos = idx - __shfl(excl, i);
+        const int64_t bi = __shfl(b, i);
+        if (idx < total) {
+            const int32_t u = a.cols[bi + pos];
+            if ((a.front[u >> 6] >> (u & 63)) & 1)
+                atomicMin(&hit[i], ((unsigned long long)(uint32_t)pos << 32) | (uint32_t)u);
+        }
+    }
+    wave_lds_sync();
+    const unsigned long long hk = hit[lane];
+    int32_t par = -1;
+    if (valid) {
+        if (hk != ~0ULL) {
+            par = (int32_t)(uint32_t)hk;
+            exam += (hk >> 32) + 1;
+        } else {
+            exam += (unsigned long long)len;
+        }
+    }
+    // lists with no hit in their first bu_cap neighbours: the whole wave
+    uint64_t lm = __ballot(valid && par < 0 && e - b > len);
     while (lm) {
         const int src_lane = __ffsll((unsigned long long)lm) - 1;
         lm &= lm - 1;
-        const int64_t kb = __shfl(k, src_lane), ke = __shfl(e, src_lane);
-        int32_t hit = -1;
-        for (int64_t k0 = kb; k0 < ke && hit < 0; k0 += 64) {
+        const int64_t kb = __shfl(b, src_lane) + __shfl(len, src_lane), ke = __shfl(e, src_lane);
+        int32_t found = -1;
+        for (int64_t k0 = kb; k0 < ke && found < 0; k0 += 64) {
             const int64_t kk = k0 + lane;
             const int32_t u = kk < ke ? a.cols[kk] : -1;
             const bool h = u >= 0 && ((a.front[u >> 6] >> (u & 63)) & 1);
@@ -564,9 +561,9 @@ __device__ void bu_pending(const step_args &a, step_smem<BLOCK> &s, const int32_
             const int64_t rest = ke - k0 < 64 ? ke - k0 : 64;
             const int64_t seen = hm ? (int64_t)__ffsll((unsigned long long)hm) : rest;
             if (lane == src_lane) exam += (unsigned long long)seen;
-            if (hm) hit = __shfl(u, __ffsll((unsigned long long)hm) - 1);
+            if (hm) found = __shfl(u, __ffsll((unsigned long long)hm) - 1);
         }
-        if (lane == src_lane) par = hit;
+        if (lane == src_lane) par = found;
     }
     if (par >= 0) {
         a.depth[v] = a.level + 1;
@@ -578,6 +575,7 @@ __device__ void bu_pending(const step_args &a, step_smem<BLOCK> &s, const int32_
         atomicOr((unsigned long long *)&s.b_found[wid][(v >> 6) - w0],
                  (unsigned long long)(1ULL << (v & 63)));
     }
+    wave_lds_sync();  // hit[] is rewritten by the next batch
 }
 
 // Bottom-up step.  Each wave owns chunks of bu_words visited words and ranks
@@ -611,12 +609,10 @@ __device__ void bu_body(const step_args &a, step_smem<BLOCK> &s, unsigned long l
         s.b_found[wid][lane] = 0;
         int np = 0;  // pending vertices (wave-uniform)
         wave_lds_sync();
-        for (int r = 0; r < total; r += 64) {
-            const int g = r + lane;
-            const bool valid = g < total;
-            const int gg = valid ? g : total - 1;
-            // word j holding the gg-th unvisited vertex: largest j with excl_j <= gg
-            int j = 0;
+        // rank g of the chunk's unvisited vertices -> word j, bit bpos
+        auto locate = [&](int g, int &j, int &bpos) {
+            const int gg = g < total ? g : total - 1;
+            j = 0;
 #pragma unroll
             for (int st = 32; st > 0; st >>= 1) {
                 const int mid = j + st;
@@ -624,19 +620,37 @@ __device__ void bu_body(const step_args &a, step_smem<BLOCK> &s, unsigned long l
                 if (mid < 64 && em <= gg) j = mid;
             }
             const uint64_t bits = (uint64_t)__shfl((long long)unv, j);
-            const int ej = __shfl(excl, j);
-            const int bpos = select_bit(bits, gg - ej);
+            bpos = select_bit(bits, gg - __shfl(excl, j));
+        };
+        // Rounds of 64 vertices, software-pipelined: the next round's head
+        // entry is loaded while this round's frontier bit is in flight.
+        int j, bpos;
+        locate(lane, j, bpos);
+        bool valid = lane < total;
+        uint64_t hd = (valid && a.bu_head) ? a.head[(w0 + j) * 64 + bpos] : 0ULL;
+        for (int r = 0; r < total; r += 64) {
             const int64_t v = (w0 + j) * 64 + bpos;
+            const int32_t h = (int32_t)(uint32_t)hd;
+            const bool probe = valid && a.bu_head && h >= 0;
+            const uint64_t fw = probe ? a.front[h >> 6] : 0ULL;
+            int jn, bposn;
+            locate(r + 64 + lane, jn, bposn);
+            const bool validn = r + 64 + lane < total;
+            const uint64_t hdn =
+                (validn && a.bu_head) ? a.head[(w0 + jn) * 64 + bposn] : 0ULL;
             bool miss = valid;
-            if (valid && a.bu_head) {
-                const int32_t h = a.head[v];
-                const int64_t b = a.off[v], e = a.off[v + 1];
+            if (probe) {
                 ++exam;
-                if (h >= 0 && ((a.front[h >> 6] >> (h & 63)) & 1)) {
+                if ((hd >> 32) == 1 && !((fw >> (h & 63)) & 1)) {
+                    // the only neighbour is the head, which missed: the list
+                    // scan would read it again and find nothing
+                    miss = false;
+                    ++exam;
+                } else if ((fw >> (h & 63)) & 1) {
                     miss = false;
                     a.depth[v] = a.level + 1;
                     a.parent[v] = h;
-                    const unsigned long long d = (unsigned long long)(e - b);
+                    const unsigned long long d = hd >> 32;
                     ++cnt;
                     dsum += d;
                     dmax = d > dmax ? d : dmax;
@@ -644,6 +658,10 @@ __device__ void bu_body(const step_args &a, step_smem<BLOCK> &s, unsigned long l
                              (unsigned long long)(1ULL << bpos));
                 }
             }
+            j = jn;
+            bpos = bposn;
+            valid = validn;
+            hd = hdn;
             const uint64_t mm = __ballot(miss);
             if (miss) pend[np + __popcll(mm & below)] = (int32_t)v;
             np += __popcll(mm);
@@ -706,7 +724,7 @@ __device__ void beamer_next(ctl_block *c, int32_t last_dir, int64_t nnew, int64_
 // its network on the wave.
 template <int BLOCK>
 __device__ void choose_dir(const step_args &a, step_smem<BLOCK> &s, ctl_block *c, int32_t last_dir,
-                           int64_t nnew, int64_t nnew_deg) {
+                           int64_t nnew, int64_t nnew_deg, const mlp_wave_regs &w) {
     if (lane_id() == 0) {
         s.use_mlp = 0;
         switch (a.policy) {
@@ -737,7 +755,7 @@ __device__ void choose_dir(const step_args &a, step_smem<BLOCK> &s, ctl_block *c
     wave_lds_sync();
     if (s.use_mlp) {
         float y[2];
-        mlp_forward_wave(a.mlp, a.mlp_h1, a.mlp_h2, s.x, y, &s.mlp);
+        mlp_eval(w, s.x, y);
         // y = predicted log2 cost of (top-down, bottom-up).  The network
         // overrides Beamer only when it predicts the other direction cheaper
         // by more than the margin; a NaN prediction keeps Beamer's choice.
@@ -797,6 +815,8 @@ __device__ void rebuild_state(const step_args &a, step_smem<BLOCK> &s, bool publ
     uint64_t cw[3];
 #pragma unroll
     for (int i = 0; i < 3; ++i) cw[i] = l + 64 * i < kCtlWords ? src[l + 64 * i] : 0;
+    mlp_wave_regs w;
+    if (a.policy == BD_POLICY_LEARNED && a.has_mlp) mlp_load(a.mlp_ops, w);
     unsigned long long ac[4];
 #pragma unroll
     for (int k = 0; k < 4; ++k) ac[k] = g->acc[pa][l][k];
@@ -862,7 +882,7 @@ __device__ void rebuild_state(const step_args &a, step_smem<BLOCK> &s, bool publ
     seg_prefix(ql, qo, a.qseg_cap, a.nv, c->qpref);
     seg_prefix(tl, to, a.tseg_cap, a.task_cap, c->tpref);
     wave_lds_sync();
-    if (s.s_new > 0) choose_dir(a, s, c, s.s_dir, s.s_new, s.s_new_deg);
+    if (s.s_new > 0) choose_dir(a, s, c, s.s_dir, s.s_new, s.s_new_deg, w);
 }
 
 // Workgroup 0, first wave: publish the state before step L as copy L&1 (both
@@ -1087,7 +1107,9 @@ __global__ __launch_bounds__(64) void k_init(step_args a, int64_t source,
     wave_lds_sync();
     // level 0: with no previous step Beamer applies only the outer test to
     // scout = deg(sources) (hybrid_bfs.cc:163-167)
-    choose_dir(a, s, c, -1, 0, 0);
+    mlp_wave_regs w;
+    if (a.policy == BD_POLICY_LEARNED && a.has_mlp) mlp_load(a.mlp_ops, w);
+    choose_dir(a, s, c, -1, 0, 0, w);
     if (l == 0) {
         __threadfence();
         g->t_start = wall_clock64();
@@ -1138,9 +1160,10 @@ __global__ void k_count_traversed(const int32_t *__restrict__ depth,
     if ((threadIdx.x & 63) == 0 && s) atomicAdd(&out[(blockIdx.x % kSlots) * kLine], s);
 }
 
-// Head index of the bottom-up step: head[v] = the neighbour of v of largest
-// degree, the first in CSR order among equals (degrees saturated at
-// 2^31-1), -1 when v has none.  A vertex's highest-degree neighbour is the
+// Head index of the bottom-up step: head[v] = (deg(v) << 32) | the
+// neighbour of v of largest degree, the first in CSR order among equals
+// (degrees saturated at 2^31-1), or 0xffffffff when v has none.  One 8-byte
+// load gives the bottom-up step both the likeliest parent and the degree.  A vertex's highest-degree neighbour is the
 // one most likely to be reached first, so it is the likeliest parent when v
 // is scanned bottom-up; a dense array of them is read in vertex order, where
 // the first entry of each adjacency list costs a cache line per vertex.
@@ -1155,7 +1178,7 @@ __device__ __forceinline__ unsigned long long head_key(const int64_t *off, int32
 // shorter than 64, longer lists are done by the whole wave.
 __global__ __launch_bounds__(256) void k_head_index(const int64_t *__restrict__ off,
                                                     const int32_t *__restrict__ cols, int64_t nv,
-                                                    int32_t *__restrict__ head) {
+                                                    uint64_t *__restrict__ head) {
     const int lane = threadIdx.x & 63;
     const int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
     const int64_t nwaves = ((int64_t)gridDim.x * blockDim.x) >> 6;
@@ -1174,7 +1197,10 @@ __global__ __launch_bounds__(256) void k_head_index(const int64_t *__restrict__ 
                 const unsigned long long key = head_key(off, cols[k], k - b);
                 best = key > best ? key : best;
             }
-            head[v] = best ? cols[b + (int64_t)(0xffffffffULL - (best & 0xffffffffULL))] : -1;
+            const uint32_t id =
+                best ? (uint32_t)cols[b + (int64_t)(0xffffffffULL - (best & 0xffffffffULL))]
+                     : 0xffffffffu;
+            head[v] = ((uint64_t)(e - b) << 32) | id;
         }
         uint64_t lm = __ballot(valid && !alone);
         while (lm) {
@@ -1188,7 +1214,8 @@ __global__ __launch_bounds__(256) void k_head_index(const int64_t *__restrict__ 
             }
             best = wave_max(best);
             if (lane == src)
-                head[v] = cols[kb + (int64_t)(0xffffffffULL - (best & 0xffffffffULL))];
+                head[v] = ((uint64_t)(ke - kb) << 32) |
+                          (uint32_t)cols[kb + (int64_t)(0xffffffffULL - (best & 0xffffffffULL))];
         }
     }
 }
@@ -1228,6 +1255,7 @@ step_args make_args(bd_ctx *c, int32_t level, int64_t alpha, int64_t beta, int p
     a.log = c->d_log;
     a.script = c->d_script;
     a.mlp = c->d_mlp;
+    a.mlp_ops = c->d_mlp_ops;
     a.nwords = c->nwords;
     a.task_cap = c->task_cap;
     a.qseg_cap = c->qseg_cap;
@@ -1241,10 +1269,10 @@ step_args make_args(bd_ctx *c, int32_t level, int64_t alpha, int64_t beta, int p
     a.close_only = 0;
     a.force_dir = -1;
     a.wave_t = std::max(1, c->tune.wave_threshold);
-    a.task_t = std::max(64, c->tune.task_threshold);
+    a.task_t = std::max(16, c->tune.task_threshold);
     a.task_edges = std::max(64, c->tune.task_edges);
     a.bu_words = c->tune.bu_words > 0 ? std::min(64, c->tune.bu_words) : 32;
-    a.bu_cap = c->tune.bu_cap > 0 ? c->tune.bu_cap : 32;
+    a.bu_cap = c->tune.bu_cap > 0 ? std::min(c->tune.bu_cap, 1 << 16) : 32;
     a.bu_head = c->tune.bu_head == 2 ? 0 : 1;
     a.policy = policy;
     a.script_len = (int32_t)c->script_len;
@@ -1396,7 +1424,7 @@ int finish_search(bd_ctx *c, int64_t source, bd_run_stats *out, double event_ms)
 
 int build_head_index(bd_ctx *c) {
     if (c->d_head || c->nv == 0) return BD_OK;
-    int rc = dev_alloc(c, (void **)&c->d_head, sizeof(int32_t) * c->nv, "head index");
+    int rc = dev_alloc(c, (void **)&c->d_head, sizeof(uint64_t) * c->nv, "head index");
     if (rc) return rc;
     hipLaunchKernelGGL(k_head_index, dim3(grid_for(c, c->nv, 256)), dim3(256), 0, c->stream,
                        c->d_off, c->d_cols, c->nv, c->d_head);

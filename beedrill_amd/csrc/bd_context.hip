@@ -102,7 +102,7 @@ int alloc_search(bd_ctx *c) {
     // Each vertex of degree >= task_threshold yields <= deg/task_edges + 1
     // tasks and appears in at most one frontier, so this bound is exact.
     const int64_t te = std::max(64, c->tune.task_edges);
-    const int64_t tt = std::max(64, c->tune.task_threshold);
+    const int64_t tt = std::max(16, c->tune.task_threshold);
     c->task_cap = c->nnz / te + c->nnz / tt + 16;
     c->tseg_cap = 2 * ((c->task_cap + kSeg - 1) / kSeg) + 1024;
     for (auto &t : c->d_tasks)
@@ -133,13 +133,15 @@ extern "C" int bd_create(bd_ctx **out, int device) {
     if (device < 0 || device >= n) return BD_EINVAL;
     bd_ctx *c = new bd_ctx();
     c->device = device;
-    c->tune.block = 256;
+    // launch shape and thresholds: beedrill_amd/autotune.py's best on MI355X
+    // (RMAT scale 24, generator seed 1, models/autotune_rmat_s24.json)
+    c->tune.block = 128;
     c->tune.grid_per_cu = 8;
-    c->tune.bu_words = 32;
+    c->tune.bu_words = 64;
     c->tune.bu_cap = 32;
-    c->tune.wave_threshold = 32;
-    c->tune.task_threshold = 256;
-    c->tune.task_edges = 128;
+    c->tune.wave_threshold = 256;
+    c->tune.task_threshold = 16;
+    c->tune.task_edges = 384;
     c->tune.level_driver = 1;
     c->tune.time_steps = 2;
     c->tune.bu_head = 1;
@@ -183,6 +185,7 @@ extern "C" int bd_destroy(bd_ctx *c) {
     dev_free(c->d_log);
     dev_free(c->d_script);
     dev_free(c->d_mlp);
+    dev_free(c->d_mlp_ops);
     if (c->h_ctl) (void)hipHostFree(c->h_ctl);
     for (auto &e : c->ev)
         if (e) (void)hipEventDestroy(e);
@@ -246,7 +249,7 @@ extern "C" int bd_set_tuning(bd_ctx *c, const bd_tuning *t) {
     if (t->bu_cap > 0) n.bu_cap = t->bu_cap;
     if (t->grid_per_cu > 0) n.grid_per_cu = std::min(t->grid_per_cu, 64);
     if (t->wave_threshold > 0) n.wave_threshold = t->wave_threshold;
-    if (t->task_threshold > 0) n.task_threshold = std::max(t->task_threshold, 64);
+    if (t->task_threshold > 0) n.task_threshold = std::max(t->task_threshold, 16);
     if (t->task_edges > 0) n.task_edges = std::max(t->task_edges, 64);
     if (t->level_driver == 1 || t->level_driver == 2) n.level_driver = t->level_driver;
     if (t->time_steps == 1 || t->time_steps == 2) n.time_steps = t->time_steps;

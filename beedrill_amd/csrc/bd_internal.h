@@ -107,7 +107,7 @@ struct bd_ctx {
     int64_t *d_off = nullptr;
     int32_t *d_cols = nullptr;
     uint64_t *d_dead = nullptr;   // 1 = vertex of degree 0, or padding bit
-    int32_t *d_head = nullptr;    // bottom-up head index (built with the graph)
+    uint64_t *d_head = nullptr;   // bottom-up head index (built with the graph)
     uint64_t *d_el = nullptr;     // el64 order, packed (src << 32) | dst
     int64_t el_m = 0;
 
@@ -139,6 +139,7 @@ struct bd_ctx {
 
     // learned policy
     float *d_mlp = nullptr;
+    uint4 *d_mlp_ops = nullptr;  // the step kernels' per-lane operand table
     int mlp_h1 = 0, mlp_h2 = 0;
     bool has_mlp = false;
     std::vector<float> h_mlp;
@@ -166,6 +167,8 @@ void free_graph(bd_ctx *c);
 int alloc_search(bd_ctx *c);
 // Head index of the bottom-up step for the resident graph.
 int build_head_index(bd_ctx *c);
+// Rebuild the step kernels' operand table of the direction network.
+int refresh_mlp_ops(bd_ctx *c);
 // Build device CSR from a device edge list (packed src<<32|dst, el order).
 int build_csr_from_el(bd_ctx *c, int64_t nv, int64_t m);
 int64_t grid_for(const bd_ctx *c, int64_t work_items, int block);

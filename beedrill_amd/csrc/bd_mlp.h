@@ -1,5 +1,5 @@
-// bd_mlp.h -- direction-cost network shared by the decide kernel and the
-// training/inference kernels.  8 inputs -> h1 -> h2 -> 2 outputs (predicted
+// bd_mlp.h -- direction-cost network shared by the step kernels (one
+// sample per step, on one wave) and the training/inference kernels.  8 inputs -> h1 -> h2 -> 2 outputs (predicted
 // log2 microseconds of a top-down and of a bottom-up step), ReLU, h1,h2 <= 32.
 // Parameter layout (row-major, out x in):
 //   W1[h1*8] b1[h1] W2[h2*h1] b2[h2] W3[2*h2] b3[2]
@@ -110,26 +110,120 @@ __device__ __forceinline__ void mlp_forward_rows_mfma(const float *p, int h1, in
     wave_lds_sync();
 }
 
-// LDS scratch of one single-wave forward pass.
-struct mlp_scratch {
-    float X[16][32];
-    float H[16][32];
-    float Y[16][32];
+// Weights of a single-sample forward pass held in registers as bf16, in the
+// operand layout of v_mfma_f32_16x16x16bf16_1k: lane l holds, for output
+// tile nt and input tile kt, B[n = 16 nt + l%16][k = 16 kt + 4 (l/16) + j],
+// j = 0..3, and the biases of its output columns.  Loading them is
+// independent of the step's counters, so a step issues these loads with
+// its other state loads and the forward pass itself touches no memory.
+struct mlp_wave_regs {
+    bf16x4 w1[2];     // layer 1 (k < 8: one input tile), output tiles 0, 1
+    bf16x4 w2[2][2];  // layer 2 [nt][kt]
+    bf16x4 w3[2];     // layer 3 (outputs 0, 1), [kt]
+    float b1[2], b2[2], b3;
 };
 
-// Single-sample forward on one wave (used when a step picks its direction).
-// x is in LDS; every lane returns y.
-__device__ inline void mlp_forward_wave(const float *p, int h1, int h2, const float *x,
-                                        float *y, mlp_scratch *s) {
-    const int l = threadIdx.x & 63;
-    for (int i = l; i < 16 * 32; i += 64) {
-        const int r = i / 32, cidx = i % 32;
-        s->X[r][cidx] = (r == 0 && cidx < kMlpIn) ? x[cidx] : 0.f;
+__device__ inline void mlp_prefetch(const float *p, int h1, int h2, mlp_wave_regs &w) {
+    const int l = threadIdx.x & 63, r = l & 15, q = l >> 4;
+    const float *w1 = p, *b1 = w1 + h1 * kMlpIn, *w2 = b1 + h1, *b2 = w2 + h2 * h1;
+    const float *w3 = b2 + h2, *b3 = w3 + kMlpOut * h2;
+#pragma unroll
+    for (int nt = 0; nt < 2; ++nt) {
+        const int n = 16 * nt + r;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const int kk = 4 * q + j;
+            w.w1[nt][j] = f32_to_bf16((n < h1 && kk < kMlpIn) ? w1[n * kMlpIn + kk] : 0.f);
+        }
+        w.b1[nt] = n < h1 ? b1[n] : 0.f;
+#pragma unroll
+        for (int kt = 0; kt < 2; ++kt)
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const int kk = 16 * kt + 4 * q + j;
+                w.w2[nt][kt][j] = f32_to_bf16((n < h2 && kk < h1) ? w2[n * h1 + kk] : 0.f);
+            }
+        w.b2[nt] = n < h2 ? b2[n] : 0.f;
     }
-    wave_lds_sync();
-    mlp_forward_rows_mfma(p, h1, h2, s->X, s->H, s->Y);
-    y[0] = s->Y[0][0];
-    y[1] = s->Y[0][1];
+#pragma unroll
+    for (int kt = 0; kt < 2; ++kt)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const int kk = 16 * kt + 4 * q + j;
+            w.w3[kt][j] = f32_to_bf16((r < kMlpOut && kk < h2) ? w3[r * h2 + kk] : 0.f);
+        }
+    w.b3 = r < kMlpOut ? b3[r] : 0.f;
+}
+
+// Forward pass of one sample x (kMlpIn floats that every lane can read) on
+// one wave, from prefetched weights.  Row 0 of each 16-row tile carries the
+// sample.  The result tile holds row 0 in lanes 0..15 (column = lane), so a
+// layer's activations reach the next layer's A operand (row 0, k = 4 (l/16)
+// + j of tile kt) by a shuffle from lane 4 (l/16) + j.  Same rounding as
+// mlp_forward_rows_mfma: bf16 operands, fp32 sums, bias and ReLU in fp32.
+// Every lane returns y.
+__device__ inline void mlp_eval(const mlp_wave_regs &w, const float *x, float *y) {
+    const int l = threadIdx.x & 63, r = l & 15, q = l >> 4;
+    bf16x4 a;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        const int kk = 4 * q + j;
+        a[j] = f32_to_bf16((r == 0 && kk < kMlpIn) ? x[kk] : 0.f);
+    }
+    const f32x4 z = {0.f, 0.f, 0.f, 0.f};
+    float h[2];
+#pragma unroll
+    for (int nt = 0; nt < 2; ++nt) {
+        const f32x4 d = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, w.w1[nt], z, 0, 0, 0);
+        const float v = d[0] + w.b1[nt];
+        h[nt] = v > 0.f ? v : 0.f;
+    }
+    bf16x4 a2[2];
+#pragma unroll
+    for (int kt = 0; kt < 2; ++kt)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const float v = __shfl(h[kt], 4 * q + j);
+            a2[kt][j] = f32_to_bf16(r == 0 ? v : 0.f);
+        }
+#pragma unroll
+    for (int nt = 0; nt < 2; ++nt) {
+        f32x4 d = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a2[0], w.w2[nt][0], z, 0, 0, 0);
+        d = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a2[1], w.w2[nt][1], d, 0, 0, 0);
+        const float v = d[0] + w.b2[nt];
+        h[nt] = v > 0.f ? v : 0.f;
+    }
+#pragma unroll
+    for (int kt = 0; kt < 2; ++kt)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const float v = __shfl(h[kt], 4 * q + j);
+            a2[kt][j] = f32_to_bf16(r == 0 ? v : 0.f);
+        }
+    f32x4 d = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a2[0], w.w3[0], z, 0, 0, 0);
+    d = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a2[1], w.w3[1], d, 0, 0, 0);
+    const float out = d[0] + w.b3;
+    y[0] = __shfl(out, 0);
+    y[1] = __shfl(out, 1);
+}
+
+// The prefetched weights as a table in device memory, 96 bytes per lane,
+// built once per network (k_mlp_ops): a step loads its lane's record with
+// six 16-byte loads issued beside its state loads.
+constexpr int kMlpOpsQuads = 6;
+union mlp_ops_rec {
+    mlp_wave_regs r;
+    uint4 q[kMlpOpsQuads];
+};
+static_assert(sizeof(mlp_wave_regs) <= sizeof(uint4) * kMlpOpsQuads, "operand record");
+
+__device__ inline void mlp_load(const uint4 *ops, mlp_wave_regs &w) {
+    const int l = threadIdx.x & 63;
+    mlp_ops_rec u;
+#pragma unroll
+    for (int i = 0; i < kMlpOpsQuads; ++i) u.q[i] = ops[l * kMlpOpsQuads + i];
+    w = u.r;
 }
 
 }  // namespace bd
+

@@ -198,6 +198,30 @@ double mask_count(const float *mask, int64_t n) {
 
 using namespace bd;
 
+// The step kernels' operand table of the current network (one wave).
+__global__ __launch_bounds__(64) void k_mlp_ops(const float *p, int h1, int h2, uint4 *ops) {
+    mlp_ops_rec u;
+    mlp_prefetch(p, h1, h2, u.r);
+#pragma unroll
+    for (int i = 0; i < kMlpOpsQuads; ++i) ops[threadIdx.x * kMlpOpsQuads + i] = u.q[i];
+}
+
+namespace bd {
+// Rebuild the operand table after the parameters change.
+int refresh_mlp_ops(bd_ctx *c) {
+    if (!c->d_mlp_ops) {
+        int rc = dev_alloc(c, (void **)&c->d_mlp_ops, sizeof(uint4) * 64 * kMlpOpsQuads,
+                           "mlp operands");
+        if (rc) return rc;
+    }
+    hipLaunchKernelGGL(k_mlp_ops, dim3(1), dim3(64), 0, c->stream, c->d_mlp, c->mlp_h1, c->mlp_h2,
+                       c->d_mlp_ops);
+    BD_HIP(c, hipGetLastError());
+    BD_HIP(c, hipStreamSynchronize(c->stream));
+    return BD_OK;
+}
+}  // namespace bd
+
 extern "C" int bd_mlp_set(bd_ctx *c, const float *params, int h1, int h2) {
     if (!c || !params) return BD_EINVAL;
     if (h1 < 1 || h2 < 1 || h1 > kMlpMaxHidden || h2 > kMlpMaxHidden)
@@ -212,8 +236,9 @@ extern "C" int bd_mlp_set(bd_ctx *c, const float *params, int h1, int h2) {
     BD_HIP(c, hipMemcpy(c->d_mlp, params, sizeof(float) * np, hipMemcpyHostToDevice));
     c->mlp_h1 = h1;
     c->mlp_h2 = h2;
-    c->has_mlp = true;
     c->h_mlp.assign(params, params + np);
+    if ((rc = refresh_mlp_ops(c))) return rc;
+    c->has_mlp = true;
     return BD_OK;
 }
 
@@ -349,5 +374,5 @@ extern "C" int bd_mlp_train(bd_ctx *c, const float *x, const float *t, const flo
     BD_HIP(c, hipStreamSynchronize(c->stream));
     if (loss) BD_HIP(c, hipMemcpy(loss, dl.p, sizeof(float), hipMemcpyDeviceToHost));
     BD_HIP(c, hipMemcpy(c->h_mlp.data(), c->d_mlp, sizeof(float) * np, hipMemcpyDeviceToHost));
-    return BD_OK;
+    return refresh_mlp_ops(c);
 }

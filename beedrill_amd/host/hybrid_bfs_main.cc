@@ -260,6 +260,25 @@ int main(int argc, char **argv) {
                 exit(1);
             }
             bd_check(g->context(), bd_mlp_set(g->context(), p.data(), 32, 32), "mlp_set");
+            // the override margin chosen at training time, from the JSON
+            // summary written next to the weights ("margin_milli": N)
+            std::string meta(args.mlp_weights);
+            const size_t dot = meta.rfind('.');
+            meta = (dot == std::string::npos ? meta : meta.substr(0, dot)) + ".json";
+            if (FILE *mf = fopen(meta.c_str(), "rb")) {
+                std::string text;
+                char buf[4096];
+                size_t k;
+                while ((k = fread(buf, 1, sizeof(buf), mf)) > 0) text.append(buf, k);
+                fclose(mf);
+                std::smatch mm;
+                if (std::regex_search(text, mm, std::regex("\"margin_milli\"\\s*:\\s*([0-9]+)"))) {
+                    bd_tuning t;
+                    memset(&t, 0, sizeof(t));
+                    t.learned_margin_milli = std::stoi(mm[1].str());
+                    bd_check(g->context(), bd_set_tuning(g->context(), &t), "set_tuning");
+                }
+            }
         }
 
         long edges_all = 0;

@@ -146,10 +146,28 @@ def main():
     ap.add_argument("--lr", type=float, default=3e-3)
     ap.add_argument("--scales", default="18,19,20,21,22")
     ap.add_argument("--seed", type=int, default=7)
+    ap.add_argument("--margin-only", metavar="WEIGHTS",
+                    help="keep these trained weights and only choose the override margin again "
+                         "(writes --out and its .json)")
     args = ap.parse_args()
     scales = tuple(int(s) for s in args.scales.split(","))
 
     ctx = bd.Context(0)
+    if args.margin_only:
+        params = np.fromfile(args.margin_only, np.float32)
+        margin, geo = choose_margin(ctx, params)
+        meta_in = os.path.splitext(args.margin_only)[0] + ".json"
+        summary = {}
+        if os.path.exists(meta_in):
+            with open(meta_in) as f:
+                summary = json.load(f)
+        summary["margin_milli"] = margin
+        summary["heldout_learned_over_beamer"] = geo
+        params.astype(np.float32).tofile(args.out)
+        with open(os.path.splitext(args.out)[0] + ".json", "w") as f:
+            json.dump(summary, f, indent=1)
+        print(json.dumps({"margin_milli": margin, "heldout_learned_over_beamer": geo}))
+        return
     t0 = time.time()
     X, Y, MASK, META = [], [], [], []
     for g in training_graphs(scales=scales):

@@ -15,6 +15,7 @@ Prints one JSON line.
 import argparse
 import json
 import os
+import re
 import sys
 import time
 
@@ -27,8 +28,9 @@ METRIC = "harmonic-mean GTEPS over 64 roots, RMAT s24; learned vs Beamer policy"
 HBM_PEAK_GBS = 8000.0
 MODEL = os.path.join(ROOT, "beedrill_amd", "models", "direction_mlp_32x32.f32")
 TRAFFIC = os.path.join(ROOT, "profiles", "bottom_up_traffic_rmat_s24.json")
-# The bottom-up instance of the step kernel, as rocprofv3 names it.
-BU_KERNEL = "k_step<256, 1>"
+# The bottom-up instance of the step kernel, as rocprofv3 names it
+# (k_step<workgroup size, 1>).
+BU_KERNEL_RE = r"k_step<\d+, 1>"
 COPY_BYTES = 256 << 20  # calibration copy: 256 MiB read + 256 MiB written
 
 
@@ -94,7 +96,7 @@ def roofline(ctx, roots, policy, nv, n_isolated):
     if os.path.exists(TRAFFIC):
         with open(TRAFFIC) as f:
             traffic = json.load(f).get("bytes_per_launch")
-    return {"kernel": BU_KERNEL, "bound": "hbm", "achieved": round(achieved, 1),
+    return {"kernel": f"k_step<{t['block']}, 1>", "bound": "hbm", "achieved": round(achieved, 1),
             "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
             "traffic": traffic, "launches": n,
             "algorithmic_bytes_per_launch": round(tot_b / max(n, 1)),
@@ -132,17 +134,17 @@ def summarize_pmc(fetch_csv, write_csv, out_path):
     for counter, path, copy_bytes in (("FETCH_SIZE", fetch_csv, COPY_BYTES),
                                       ("WRITE_SIZE", write_csv, COPY_BYTES)):
         rows = per_kernel(path, counter)
-        bu = [v for k, vs in rows.items() if BU_KERNEL in k for v in vs]
+        bu = [v for k, vs in rows.items() if re.search(BU_KERNEL_RE, k) for v in vs]
         cp = [v for k, vs in rows.items() if "copy" in k.lower() for v in vs]
         if not bu or not cp:
-            raise SystemExit(f"{path}: no {BU_KERNEL} or copy dispatches for {counter}")
+            raise SystemExit(f"{path}: no bottom-up or copy dispatches for {counter}")
         scale = copy_bytes / (float(np.median(cp)) * 1024.0)
         out[counter] = {"kib_per_launch": float(np.mean(bu)), "launches": len(bu),
                         "copy_kib": float(np.median(cp)), "scale": scale,
                         "bytes_per_launch": float(np.mean(bu)) * 1024.0 * scale}
     out["bytes_per_launch"] = round(out["FETCH_SIZE"]["bytes_per_launch"] +
                                     out["WRITE_SIZE"]["bytes_per_launch"])
-    out["kernel"] = BU_KERNEL
+    out["kernel"] = sorted(k for k in rows if re.search(BU_KERNEL_RE, k))
     with open(out_path, "w") as f:
         json.dump(out, f, indent=1)
     print(json.dumps(out))

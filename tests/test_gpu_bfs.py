@@ -372,3 +372,60 @@ def test_multi_graph_matches_oracle(ctx, orc):
     with pytest.raises(bd.BeedrillError) as e:
         ctx.multi_graph(vbase, bad)
     assert e.value.code == -6
+
+
+def test_learned_policy_any_network_same_depth(ctx, orc, s16):
+    """Whatever the network predicts, the learned policy only changes
+    directions: depths stay bit-exact and parents valid.  Random networks at
+    several override margins, and the trained network."""
+    ctx.generate_rmat(16, 16)
+    off, adj, m = s16
+    roots = [int(r) for r in ctx.pick_roots(3)]
+    rng = np.random.default_rng(11)
+    t0 = ctx.get_tuning()
+    dirs_seen = set()
+    try:
+        for trial in range(3):
+            ctx.mlp_set(rng.normal(0, 1.0, 32 * 8 + 32 + 32 * 32 + 32 + 2 * 32 + 2), 32, 32)
+            for margin in (1, 500, 4000):
+                ctx.set_tuning(learned_margin_milli=margin)
+                for root in roots:
+                    run_and_compare(ctx, orc, off, adj, m, root, policy="learned", levels=False)
+                    dirs_seen.add(tuple(r["dir"] for r in ctx.levels()))
+    finally:
+        ctx.set_tuning(**t0)
+    assert len(dirs_seen) > 1  # the networks did change directions
+
+
+def test_learned_decisions_follow_the_network(ctx, orc, s16):
+    """With the smallest override margin the step kernel's direction is the
+    network's preference, evaluated in the kernel (bf16 MFMA) from features it
+    computes itself.  The same network in fp32 on the host, from the step
+    records' features, must prefer the same direction wherever its two
+    predictions differ by more than bf16 rounding could move them."""
+    import os
+    ctx.generate_rmat(16, 16)
+    off, adj, m = s16
+    nv = len(off) - 1
+    path = os.path.join(os.path.dirname(os.path.dirname(__file__)), "beedrill_amd", "models",
+                        "direction_mlp_32x32.f32")
+    params = np.fromfile(path, np.float32)
+    ctx.mlp_set(params, 32, 32)
+    t0 = ctx.get_tuning()
+    checked = 0
+    try:
+        ctx.set_tuning(learned_margin_milli=1)
+        for root in ctx.pick_roots(6):
+            ctx.bfs(int(root), 15, 18, "learned")
+            lv = ctx.levels()
+            for i, rec in enumerate(lv):
+                prev = lv[i - 1]["dir"] if i > 0 else -1
+                x = bd.features(rec, nv, m, prev)
+                y = orc.mlp_forward(params, np.asarray([x], np.float32))[0]
+                if abs(y[1] - y[0]) < 0.1:
+                    continue
+                assert rec["dir"] == (bd.DIR_BU if y[1] < y[0] else bd.DIR_TD), (root, i, y)
+                checked += 1
+    finally:
+        ctx.set_tuning(**t0)
+    assert checked >= 10
