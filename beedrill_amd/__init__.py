@@ -54,11 +54,11 @@ class RunStats(ctypes.Structure):
 
 TUNING_FIELDS = ("block", "grid_per_cu", "wave_threshold", "task_threshold", "task_edges",
                  "bu_words", "bu_cap", "level_driver", "time_steps", "bu_head", "step_kernels",
-                 "bu_grid_per_cu", "learned_margin_milli")
+                 "bu_grid_per_cu", "learned_margin_milli", "ms_alpha")
 
 
 class Tuning(ctypes.Structure):
-    _fields_ = [(n, ctypes.c_int32) for n in TUNING_FIELDS] + [("reserved", ctypes.c_int32 * 3)]
+    _fields_ = [(n, ctypes.c_int32) for n in TUNING_FIELDS] + [("reserved", ctypes.c_int32 * 2)]
 
     def as_dict(self):
         return {f: getattr(self, f) for f in TUNING_FIELDS}
@@ -112,6 +112,8 @@ def lib():
         "bd_get_tuning": ([vp, ctypes.POINTER(Tuning)], ctypes.c_int),
         "bd_bfs_multi_root": ([vp, _i64p, ctypes.c_int, vp, ctypes.POINTER(ctypes.c_double),
                                ctypes.POINTER(i64)], ctypes.c_int),
+        "bd_copy_bandwidth": ([vp, i64, ctypes.c_int, ctypes.POINTER(ctypes.c_double)],
+                              ctypes.c_int),
         "bd_bfs_multi_graph": ([vp, ctypes.c_int, _i64p, _i64p, vp,
                                 ctypes.POINTER(ctypes.c_double)], ctypes.c_int),
         "bd_mlp_set": ([vp, _f32p, ctypes.c_int, ctypes.c_int], ctypes.c_int),
@@ -317,6 +319,12 @@ class Context:
         for k, v in kw.items():
             setattr(t, k, int(v))
         self._ok(self._L.bd_set_tuning(self._h, ctypes.byref(t)))
+
+    def copy_bandwidth(self, nbytes, reps=3):
+        """Best rate of a plain device copy of nbytes (read + write GB/s)."""
+        g = ctypes.c_double(0)
+        self._ok(self._L.bd_copy_bandwidth(self._h, int(nbytes), int(reps), ctypes.byref(g)))
+        return g.value
 
     # --------------------------------------------------------------- batches
     def multi_root(self, roots, want_depths=True):

@@ -9,12 +9,16 @@
 //   seen[v]   roots that have reached v
 //   F[v]      roots whose depth-L frontier holds v
 //   N[v]      roots whose depth-(L+1) frontier holds v (filled by step L)
+//   Fb, Nb    bitmaps of the vertices with F[v] != 0, N[v] != 0 (V/8 bytes,
+//             so they stay in L2; a wave writes its 64 vertices' word with a
+//             ballot)
 // A level is either
 //   push: every v with F[v] != 0 ORs F[v] & ~seen[u] into N[u] of each
 //         neighbour u (device atomics), then a commit pass keeps the bits
 //         not yet seen and clears F for reuse; or
-//   pull: every v not yet seen by all roots ORs F[u] over its neighbours,
-//         stopping as soon as every root it still waits for is covered.
+//   pull: every v not yet seen by all roots ORs F[u] over its neighbours
+//         that are in Fb, stopping as soon as every root it still waits for
+//         is covered.
 // The host picks the mode per level from the previous level's counters (the
 // frontier's degree sum against the degree sum of vertices still open), as
 // Beamer's rule picks a direction.  Depths are written root-major
@@ -41,6 +45,8 @@ struct ms_args {
     uint64_t *F;
     uint64_t *N;
     uint64_t *seen;
+    uint64_t *Fb;    // bit v: F[v] != 0 (frontier membership, V/64 words)
+    uint64_t *Nb;    // bit v: N[v] != 0, written by the commit or pull pass
     int32_t *depth;  // n rows of nv, or null
     unsigned long long *cnt;  // [kSlots][kLine]: frontier size, degree sum, open degree sum
     int64_t nv;
@@ -53,9 +59,30 @@ struct ms_args {
 // masked store per root, so each store covers the wave's 64 vertices.
 __device__ __forceinline__ void ms_record(const ms_args &a, int64_t v, bool valid, uint64_t nf,
                                           int32_t d) {
-    if (!a.depth || wave_or(valid ? nf : 0ULL) == 0) return;
-    for (int r = 0; r < a.nroots; ++r)
+    if (!a.depth) return;
+    // only the roots that some lane of the wave gained
+    for (uint64_t rem = wave_or(valid ? nf : 0ULL); rem; rem &= rem - 1) {
+        const int r = __ffsll((unsigned long long)rem) - 1;
         if (valid && ((nf >> r) & 1)) a.depth[(int64_t)r * a.nv + v] = d;
+    }
+}
+
+// Neighbour ids are read once per level: non-temporal loads, so the stream
+// does not push the bitmaps and masks out of L2.
+__device__ __forceinline__ int32_t ms_col(const ms_args &a, int64_t k) {
+    return __builtin_nontemporal_load(a.cols + k);
+}
+
+// Roots whose frontier holds u: the membership bitmap (small enough to stay
+// in L2) first, the 8-byte mask only for members.
+__device__ __forceinline__ uint64_t ms_frontier(const ms_args &a, int32_t u) {
+    return ((a.Fb[u >> 6] >> (u & 63)) & 1) ? a.F[u] : 0ULL;
+}
+
+// OR the bits of frontier vertex mask f that u has not seen into N[u].
+__device__ __forceinline__ void ms_push_to(const ms_args &a, int32_t u, uint64_t f) {
+    const uint64_t m = f & ~a.seen[u];
+    if (m) atomicOr((unsigned long long *)&a.N[u], (unsigned long long)m);
 }
 
 // Per-block counters added to slot blockIdx % kSlots.
@@ -94,6 +121,7 @@ __global__ void k_ms_init(ms_args a, const int64_t *__restrict__ roots, int n) {
         const uint64_t bit = 1ULL << i;
         const unsigned long long old = atomicOr((unsigned long long *)&a.F[v], bit);
         atomicOr((unsigned long long *)&a.seen[v], bit);
+        atomicOr((unsigned long long *)&a.Fb[v >> 6], 1ULL << (v & 63));
         if (a.depth) a.depth[(int64_t)i * a.nv + v] = 0;
         if (old == 0) {  // first root on this vertex
             c0 = 1;
@@ -118,32 +146,25 @@ __global__ __launch_bounds__(kMsBlock) void k_ms_push(ms_args a) {
     const int64_t wave = ((int64_t)blockIdx.x * kMsBlock + threadIdx.x) >> 6;
     const int64_t nwaves = ((int64_t)gridDim.x * kMsBlock) >> 6;
     for (int64_t base = wave * 64; base < a.nv; base += nwaves * 64) {
+        const uint64_t fw = a.Fb[base >> 6];
+        if (fw == 0) continue;  // no frontier vertex among these 64
         const int64_t v = base + lane;
-        const uint64_t f = v < a.nv ? a.F[v] : 0ULL;
+        const uint64_t f = ((fw >> lane) & 1) ? a.F[v] : 0ULL;
         int64_t b = 0, e = 0;
         if (f) {
             b = a.off[v];
             e = a.off[v + 1];
         }
         const bool alone = f && e - b < kMsWaveDeg;
-        if (alone) {
-            for (int64_t k = b; k < e; ++k) {
-                const int32_t u = a.cols[k];
-                const uint64_t m = f & ~a.seen[u];
-                if (m) atomicOr((unsigned long long *)&a.N[u], (unsigned long long)m);
-            }
-        }
+        if (alone)
+            for (int64_t k = b; k < e; ++k) ms_push_to(a, ms_col(a, k), f);
         uint64_t lm = __ballot(f && !alone);
         while (lm) {
             const int src = __ffsll((unsigned long long)lm) - 1;
             lm &= lm - 1;
             const int64_t kb = __shfl(b, src), ke = __shfl(e, src);
             const uint64_t fs = (uint64_t)__shfl((long long)f, src);
-            for (int64_t k = kb + lane; k < ke; k += 64) {
-                const int32_t u = a.cols[k];
-                const uint64_t m = fs & ~a.seen[u];
-                if (m) atomicOr((unsigned long long *)&a.N[u], (unsigned long long)m);
-            }
+            for (int64_t k = kb + lane; k < ke; k += 64) ms_push_to(a, ms_col(a, k), fs);
         }
     }
 }
@@ -174,6 +195,8 @@ __global__ __launch_bounds__(kMsBlock) void k_ms_commit(ms_args a) {
                 if (open) c2 += d;
             }
         }
+        const uint64_t word = __ballot(nf != 0);
+        if (lane_id() == 0 && v < a.nv) a.Nb[v >> 6] = word;
         ms_record(a, v, valid, nf, a.level + 1);
     }
     ms_count(a, c0, c1, c2);
@@ -204,11 +227,12 @@ __global__ __launch_bounds__(kMsBlock) void k_ms_pull(ms_args a) {
         if (alone) {
             int64_t k = b;
             for (; k + 4 <= e && (acc & want) != want; k += 4) {
-                const int32_t u0 = a.cols[k], u1 = a.cols[k + 1];
-                const int32_t u2 = a.cols[k + 2], u3 = a.cols[k + 3];
-                acc |= a.F[u0] | a.F[u1] | a.F[u2] | a.F[u3];
+                const int32_t u0 = ms_col(a, k), u1 = ms_col(a, k + 1);
+                const int32_t u2 = ms_col(a, k + 2), u3 = ms_col(a, k + 3);
+                acc |= ms_frontier(a, u0) | ms_frontier(a, u1) | ms_frontier(a, u2) |
+                       ms_frontier(a, u3);
             }
-            for (; k < e && (acc & want) != want; ++k) acc |= a.F[a.cols[k]];
+            for (; k < e && (acc & want) != want; ++k) acc |= ms_frontier(a, ms_col(a, k));
         }
         uint64_t lm = __ballot(want && !alone);
         while (lm) {
@@ -219,11 +243,13 @@ __global__ __launch_bounds__(kMsBlock) void k_ms_pull(ms_args a) {
             uint64_t got = 0;
             for (int64_t k0 = kb; k0 < ke && (got & ws) != ws; k0 += 64) {
                 const int64_t k = k0 + lane;
-                got |= wave_or(k < ke ? a.F[a.cols[k]] : 0ULL);
+                got |= wave_or(k < ke ? ms_frontier(a, ms_col(a, k)) : 0ULL);
             }
             if (lane == src) acc = got;
         }
         const uint64_t nf = acc & want;
+        const uint64_t word = __ballot(nf != 0);
+        if (lane == 0 && base < a.nv) a.Nb[base >> 6] = word;
         if (valid) {
             a.N[v] = nf;
             if (nf) {
@@ -254,6 +280,10 @@ int ms_buffers(bd_ctx *c, int nrows) {
         (rc = dev_alloc(c, (void **)&c->d_ms[3], sizeof(unsigned long long) * kSlots * kLine,
                         "mask counters")))
         return rc;
+    for (int i = 0; i < 2; ++i)
+        if (!c->d_ms_bits[i] && (rc = dev_alloc(c, (void **)&c->d_ms_bits[i],
+                                                sizeof(uint64_t) * c->nwords, "frontier bits")))
+            return rc;
     if (nrows > c->ms_depth_rows) {
         dev_free(c->d_ms_depth);
         c->d_ms_depth = nullptr;
@@ -303,6 +333,8 @@ int run_multi_root(bd_ctx *c, const int64_t *roots, int n, int32_t *depths, doub
     a.F = c->d_ms[0];
     a.N = c->d_ms[1];
     a.seen = c->d_ms[2];
+    a.Fb = c->d_ms_bits[0];
+    a.Nb = c->d_ms_bits[1];
     a.depth = depths ? c->d_ms_depth : nullptr;
     a.cnt = (unsigned long long *)c->d_ms[3];
     a.nv = nv;
@@ -315,6 +347,8 @@ int run_multi_root(bd_ctx *c, const int64_t *roots, int n, int32_t *depths, doub
     BD_HIP(c, hipMemcpyAsync(d_roots, roots, sizeof(int64_t) * n, hipMemcpyHostToDevice,
                              c->stream));
     for (int i = 0; i < 3; ++i) BD_HIP(c, hipMemsetAsync(c->d_ms[i], 0, mbytes, c->stream));
+    for (int i = 0; i < 2; ++i)
+        BD_HIP(c, hipMemsetAsync(c->d_ms_bits[i], 0, sizeof(uint64_t) * c->nwords, c->stream));
     if (a.depth) {
         const int64_t cells = (int64_t)n * nv;
         hipLaunchKernelGGL(k_fill_i32, dim3(grid_for(c, cells, 256)), dim3(256), 0, c->stream,
@@ -332,11 +366,22 @@ int run_multi_root(bd_ctx *c, const int64_t *roots, int n, int32_t *depths, doub
     if ((rc = ms_read_counts(c, cnt))) return rc;
     bool n_clean = true;  // N holds zeros (ready for a push)
     int64_t L = 0;
-    const int64_t alpha = 4;  // push while the frontier's edges are < open edges / alpha
+    // push while the frontier's edges * alpha are below the open edges
+    const unsigned long long alpha = (unsigned long long)std::max(1, c->tune.ms_alpha);
+    std::vector<bd_level_rec> log;
     while (cnt[0] > 0) {
         if (L > nv + 1) return set_err(c, BD_EHIP, "multi-root search did not terminate");
         a.level = (int32_t)L;
-        const bool push = cnt[1] * (unsigned long long)alpha < cnt[2];
+        const bool push = cnt[1] * alpha < cnt[2];
+        bd_level_rec r;
+        memset(&r, 0, sizeof(r));
+        r.level = L;
+        r.dir = push ? 0 : 1;
+        r.frontier = (int64_t)cnt[0];
+        r.frontier_deg = (int64_t)cnt[1];
+        r.unvisited_deg = (int64_t)cnt[2];
+        r.ms_other = -1.0;
+        BD_HIP(c, hipEventRecord(c->ev[2], c->stream));
         BD_HIP(c, hipMemsetAsync(a.cnt, 0, cbytes, c->stream));
         if (push) {
             if (!n_clean) BD_HIP(c, hipMemsetAsync(a.N, 0, mbytes, c->stream));
@@ -348,8 +393,16 @@ int run_multi_root(bd_ctx *c, const int64_t *roots, int n, int32_t *depths, doub
             n_clean = false;  // F still holds depth L
         }
         BD_HIP(c, hipGetLastError());
+        BD_HIP(c, hipEventRecord(c->ev[3], c->stream));
         if ((rc = ms_read_counts(c, cnt))) return rc;
+        float lms = 0;
+        BD_HIP(c, hipEventElapsedTime(&lms, c->ev[2], c->ev[3]));
+        r.ms = lms;
+        r.next = (int64_t)cnt[0];
+        r.next_deg = (int64_t)cnt[1];
+        log.push_back(r);
         std::swap(a.F, a.N);
+        std::swap(a.Fb, a.Nb);
         ++L;
     }
     BD_HIP(c, hipEventRecord(c->ev[1], c->stream));
@@ -358,6 +411,8 @@ int run_multi_root(bd_ctx *c, const int64_t *roots, int n, int32_t *depths, doub
     BD_HIP(c, hipEventElapsedTime(&t, c->ev[0], c->ev[1]));
     if (ms) *ms = t;
     if (levels) *levels = L;
+    c->last_log = log;
+    c->have_result = false;  // bd_bfs_levels now describes this search, not the last single one
     if (depths)
         BD_HIP(c, hipMemcpy(depths, a.depth, sizeof(int32_t) * n * nv, hipMemcpyDeviceToHost));
     return BD_OK;

@@ -36,6 +36,7 @@
 // direction is
 // picked on the device, so a search is a stream of launches with no host
 // round trip.
+#include <hip/hip_ext.h>
 #include <math.h>
 #include <stdio.h>
 #include <string.h>
@@ -530,7 +531,7 @@ __device__ void bu_pending(const step_args &a, step_smem<BLOCK> &s, const int32_
         const int32_t pos = idx - __shfl(excl, i);
         const int64_t bi = __shfl(b, i);
         if (idx < total) {
-            const int32_t u = a.cols[bi + pos];
+            const int32_t u = __builtin_nontemporal_load(a.cols + bi + pos);
             if ((a.front[u >> 6] >> (u & 63)) & 1)
                 atomicMin(&hit[i], ((unsigned long long)(uint32_t)pos << 32) | (uint32_t)u);
         }
@@ -627,7 +628,8 @@ __device__ void bu_body(const step_args &a, step_smem<BLOCK> &s, unsigned long l
         int j, bpos;
         locate(lane, j, bpos);
         bool valid = lane < total;
-        uint64_t hd = (valid && a.bu_head) ? a.head[(w0 + j) * 64 + bpos] : 0ULL;
+        uint64_t hd =
+            (valid && a.bu_head) ? __builtin_nontemporal_load(a.head + (w0 + j) * 64 + bpos) : 0ULL;
         for (int r = 0; r < total; r += 64) {
             const int64_t v = (w0 + j) * 64 + bpos;
             const int32_t h = (int32_t)(uint32_t)hd;
@@ -636,8 +638,9 @@ __device__ void bu_body(const step_args &a, step_smem<BLOCK> &s, unsigned long l
             int jn, bposn;
             locate(r + 64 + lane, jn, bposn);
             const bool validn = r + 64 + lane < total;
-            const uint64_t hdn =
-                (validn && a.bu_head) ? a.head[(w0 + jn) * 64 + bposn] : 0ULL;
+            const uint64_t hdn = (validn && a.bu_head)
+                                     ? __builtin_nontemporal_load(a.head + (w0 + jn) * 64 + bposn)
+                                     : 0ULL;
             bool miss = valid;
             if (probe) {
                 ++exam;
@@ -1286,19 +1289,29 @@ step_args make_args(bd_ctx *c, int32_t level, int64_t alpha, int64_t beta, int p
 
 int read_ctl(bd_ctx *c);
 
-template <int B>
-void launch_step_block(bd_ctx *c, const step_args &a, int dirs, int64_t grid) {
-    if (dirs == kStepTD)
-        hipLaunchKernelGGL((k_step<B, kStepTD>), dim3(grid), dim3(B), 0, c->stream, a);
-    else if (dirs == kStepBU)
-        hipLaunchKernelGGL((k_step<B, kStepBU>), dim3(grid), dim3(B), 0, c->stream, a);
+template <int B, int D>
+void launch_one(bd_ctx *c, const step_args &a, int64_t grid, bool timed) {
+    if (timed)  // the events are stamped by the dispatch itself: kernel time only
+        hipExtLaunchKernelGGL((k_step<B, D>), dim3(grid), dim3(B), 0, c->stream, c->ev[2],
+                              c->ev[3], 0, a);
     else
-        hipLaunchKernelGGL((k_step<B, kStepBoth>), dim3(grid), dim3(B), 0, c->stream, a);
+        hipLaunchKernelGGL((k_step<B, D>), dim3(grid), dim3(B), 0, c->stream, a);
+}
+
+template <int B>
+void launch_step_block(bd_ctx *c, const step_args &a, int dirs, int64_t grid, bool timed) {
+    if (dirs == kStepTD)
+        launch_one<B, kStepTD>(c, a, grid, timed);
+    else if (dirs == kStepBU)
+        launch_one<B, kStepBU>(c, a, grid, timed);
+    else
+        launch_one<B, kStepBoth>(c, a, grid, timed);
 }
 
 // Launch the step kernel carrying `dirs` (kStepTD, kStepBU or kStepBoth);
-// grid <= 0 takes the tuned workgroups per CU of that kernel.
-int launch_step(bd_ctx *c, const step_args &a, int dirs, int64_t grid = 0) {
+// grid <= 0 takes the tuned workgroups per CU of that kernel.  timed: the
+// launch stamps events ev[2] and ev[3] at the kernel's start and end.
+int launch_step(bd_ctx *c, const step_args &a, int dirs, int64_t grid = 0, bool timed = false) {
     const int blk = norm_block(c->tune.block);
     if (grid <= 0) {
         // the bottom-up grid is its own only when it has its own kernel
@@ -1308,13 +1321,13 @@ int launch_step(bd_ctx *c, const step_args &a, int dirs, int64_t grid = 0) {
     }
     switch (blk) {
     case 128:
-        launch_step_block<128>(c, a, dirs, grid);
+        launch_step_block<128>(c, a, dirs, grid, timed);
         break;
     case 512:
-        launch_step_block<512>(c, a, dirs, grid);
+        launch_step_block<512>(c, a, dirs, grid, timed);
         break;
     default:
-        launch_step_block<256>(c, a, dirs, grid);
+        launch_step_block<256>(c, a, dirs, grid, timed);
     }
     BD_HIP(c, hipGetLastError());
     return BD_OK;
@@ -1456,10 +1469,8 @@ int run_search(bd_ctx *c, int64_t source, int64_t alpha, int64_t beta, int polic
             if (c->h_ctl[L & 1].done) break;
             // the host knows the direction: launch that direction's kernel
             const int dirs = c->h_ctl[L & 1].dir;
-            if (timed) BD_HIP(c, hipEventRecord(c->ev[2], c->stream));
-            if ((rc = launch_step(c, a, dirs))) return rc;
+            if ((rc = launch_step(c, a, dirs, 0, timed))) return rc;
             if (timed) {
-                BD_HIP(c, hipEventRecord(c->ev[3], c->stream));
                 BD_HIP(c, hipEventSynchronize(c->ev[3]));
                 float ms = 0;
                 BD_HIP(c, hipEventElapsedTime(&ms, c->ev[2], c->ev[3]));
@@ -1664,9 +1675,7 @@ extern "C" int bd_bfs_profile(bd_ctx *c, int64_t source, int64_t alpha, int64_t 
             }
             hipLaunchKernelGGL(k_scrub, dim3(c->cu_count * 4), dim3(256), 0, c->stream, scrub,
                                scrub_n, scrub + scrub_n);
-            BD_HIP(c, hipEventRecord(c->ev[2], c->stream));
-            if ((rc = launch_step(c, a, a.force_dir))) return rc;
-            BD_HIP(c, hipEventRecord(c->ev[3], c->stream));
+            if ((rc = launch_step(c, a, a.force_dir, 0, true))) return rc;
             BD_HIP(c, hipEventSynchronize(c->ev[3]));
             BD_HIP(c, hipEventElapsedTime(&ms[pass], c->ev[2], c->ev[3]));
             BD_HIP(c, hipMemcpy(acc.data(), c->d_seg->acc[L % 3],

@@ -1,4 +1,5 @@
 // bd_context.hip -- context lifetime, errors, buffers and small queries.
+#include <hip/hip_ext.h>
 #include <stdio.h>
 #include <string.h>
 
@@ -45,6 +46,7 @@ static void free_search(bd_ctx *c) {
     for (auto &q : c->d_q) dev_free(q);
     for (auto &t : c->d_tasks) dev_free(t);
     for (auto &b : c->d_ms) dev_free(b);
+    for (auto &b : c->d_ms_bits) dev_free(b);
     dev_free(c->d_ms_depth);
     c->d_depth = c->d_parent = nullptr;
     c->d_vis = c->d_vis_save = nullptr;
@@ -52,6 +54,7 @@ static void free_search(bd_ctx *c) {
     for (auto &q : c->d_q) q = nullptr;
     for (auto &t : c->d_tasks) t = nullptr;
     for (auto &b : c->d_ms) b = nullptr;
+    for (auto &b : c->d_ms_bits) b = nullptr;
     c->d_ms_depth = nullptr;
     c->ms_depth_rows = 0;
     c->task_cap = 0;
@@ -148,6 +151,7 @@ extern "C" int bd_create(bd_ctx **out, int device) {
     c->tune.step_kernels = 2;
     c->tune.bu_grid_per_cu = 8;
     c->tune.learned_margin_milli = 250;
+    c->tune.ms_alpha = 4;
     hipError_t e = hipSetDevice(device);
     if (e == hipSuccess) {
         hipDeviceProp_t prop;
@@ -257,6 +261,7 @@ extern "C" int bd_set_tuning(bd_ctx *c, const bd_tuning *t) {
     if (t->step_kernels == 1 || t->step_kernels == 2) n.step_kernels = t->step_kernels;
     if (t->bu_grid_per_cu > 0) n.bu_grid_per_cu = std::min(t->bu_grid_per_cu, 64);
     if (t->learned_margin_milli > 0) n.learned_margin_milli = t->learned_margin_milli;
+    if (t->ms_alpha > 0) n.ms_alpha = t->ms_alpha;
     bool realloc = (n.task_edges != c->tune.task_edges || n.task_threshold != c->tune.task_threshold);
     c->tune = n;
     if (realloc && c->d_off) return alloc_search(c);
@@ -268,3 +273,60 @@ extern "C" int bd_get_tuning(bd_ctx *c, bd_tuning *t) {
     *t = c->tune;
     return BD_OK;
 }
+
+// ------------------------------------------------------------ measurement
+
+namespace {
+
+// A plain streaming copy (16-byte loads and stores, grid-stride): the
+// reference point for the hardware traffic counters and for achievable HBM
+// bandwidth.
+__global__ __launch_bounds__(256) void k_copy(const uint4 *__restrict__ src,
+                                              uint4 *__restrict__ dst, int64_t n) {
+    // four independent 16-byte loads in flight per thread
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    for (; i + 3 * stride < n; i += 4 * stride) {
+        const uint4 a = src[i], b = src[i + stride], c = src[i + 2 * stride],
+                    d = src[i + 3 * stride];
+        dst[i] = a;
+        dst[i + stride] = b;
+        dst[i + 2 * stride] = c;
+        dst[i + 3 * stride] = d;
+    }
+    for (; i < n; i += stride) dst[i] = src[i];
+}
+
+}  // namespace
+
+extern "C" int bd_copy_bandwidth(bd_ctx *c, int64_t bytes, int reps, double *gbps) {
+    if (!c || bytes < 16 || reps < 1) return BD_EINVAL;
+    BD_HIP(c, hipSetDevice(c->device));
+    const int64_t n = bytes / 16;
+    void *src = nullptr, *dst = nullptr;
+    int rc = dev_alloc(c, &src, sizeof(uint4) * n, "copy source");
+    if (!rc) rc = dev_alloc(c, &dst, sizeof(uint4) * n, "copy target");
+    struct guard {
+        void *a, *b;
+        ~guard() {
+            dev_free(a);
+            dev_free(b);
+        }
+    } g{src, dst};
+    if (rc) return rc;
+    BD_HIP(c, hipMemsetAsync(src, 1, sizeof(uint4) * n, c->stream));
+    const int64_t grid = (int64_t)c->cu_count * 4;
+    float best = 0;
+    for (int r = 0; r < reps; ++r) {
+        hipExtLaunchKernelGGL(k_copy, dim3(grid), dim3(256), 0, c->stream, c->ev[2], c->ev[3], 0,
+                              (const uint4 *)src, (uint4 *)dst, n);
+        BD_HIP(c, hipGetLastError());
+        BD_HIP(c, hipEventSynchronize(c->ev[3]));
+        float ms = 0;
+        BD_HIP(c, hipEventElapsedTime(&ms, c->ev[2], c->ev[3]));
+        if (r == 0 || ms < best) best = ms;
+    }
+    if (gbps) *gbps = best > 0 ? 2.0 * 16.0 * (double)n / (best * 1e-3) / 1e9 : 0.0;
+    return BD_OK;
+}
+

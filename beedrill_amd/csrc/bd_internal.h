@@ -145,7 +145,8 @@ struct bd_ctx {
     std::vector<float> h_mlp;
 
     // multi-root state
-    uint64_t *d_ms[4] = {nullptr, nullptr, nullptr, nullptr};
+    uint64_t *d_ms[4] = {nullptr, nullptr, nullptr, nullptr};  // F, N, seen, counters
+    uint64_t *d_ms_bits[2] = {nullptr, nullptr};              // frontier bitmaps
     int32_t *d_ms_depth = nullptr;
     int64_t ms_depth_rows = 0;
 

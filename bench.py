@@ -31,7 +31,7 @@ TRAFFIC = os.path.join(ROOT, "profiles", "bottom_up_traffic_rmat_s24.json")
 # The bottom-up instance of the step kernel, as rocprofv3 names it
 # (k_step<workgroup size, 1>).
 BU_KERNEL_RE = r"k_step<\d+, 1>"
-COPY_BYTES = 256 << 20  # calibration copy: 256 MiB read + 256 MiB written
+COPY_BYTES = 1 << 30  # calibration copy: 1 GiB read + 1 GiB written
 
 
 def log(msg):
@@ -103,15 +103,11 @@ def roofline(ctx, roots, policy, nv, n_isolated):
             "avg_launch_ms": round(tot_ms / max(n, 1), 5)}
 
 
-def copy_calibration():
-    """A plain device copy of known size, for scaling the FETCH_SIZE and
-    WRITE_SIZE counters (run under rocprofv3 --pmc with the roofline leg)."""
-    import torch
-    src = torch.ones(COPY_BYTES // 4, dtype=torch.float32, device="cuda")
-    dst = torch.empty_like(src)
-    for _ in range(3):
-        dst.copy_(src)
-    torch.cuda.synchronize()
+def copy_calibration(ctx):
+    """A plain device copy of known size (the library's k_copy), for scaling
+    the FETCH_SIZE and WRITE_SIZE counters when this leg runs under
+    rocprofv3 --pmc; returns its best rate (read + written GB/s)."""
+    return ctx.copy_bandwidth(COPY_BYTES, reps=3)
 
 
 def summarize_pmc(fetch_csv, write_csv, out_path):
@@ -135,7 +131,7 @@ def summarize_pmc(fetch_csv, write_csv, out_path):
                                       ("WRITE_SIZE", write_csv, COPY_BYTES)):
         rows = per_kernel(path, counter)
         bu = [v for k, vs in rows.items() if re.search(BU_KERNEL_RE, k) for v in vs]
-        cp = [v for k, vs in rows.items() if "copy" in k.lower() for v in vs]
+        cp = [v for k, vs in rows.items() if "k_copy" in k for v in vs]
         if not bu or not cp:
             raise SystemExit(f"{path}: no bottom-up or copy dispatches for {counter}")
         scale = copy_bytes / (float(np.median(cp)) * 1024.0)
@@ -235,8 +231,8 @@ def main():
         for r in roots[:args.warmup]:
             ctx.bfs(int(r), 15, 18, policy)
         roof = roofline(ctx, roots[:args.roofline_roots], policy, nv, n_isolated)
-        copy_calibration()
-        print(json.dumps({"leg": "roofline", "roofline": roof}))
+        copy_gbs = copy_calibration(ctx)
+        print(json.dumps({"leg": "roofline", "roofline": roof, "copy_gbs": round(copy_gbs, 1)}))
         return
 
     results = {"beamer": run_policy(ctx, roots, "beamer", args.warmup)}

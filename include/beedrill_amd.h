@@ -215,10 +215,20 @@ typedef struct {
                                 direction cheaper by more than this many
                                 thousandths of a log2 unit (1 = the network
                                 decides alone)                            */
-    int32_t reserved[3];
+    int32_t ms_alpha;        /* bit-parallel search: a level pushes while the
+                                frontier's degree sum * ms_alpha is below
+                                the degree sum of the vertices still open,
+                                else it pulls                             */
+    int32_t reserved[2];
 } bd_tuning;
 int bd_set_tuning(bd_ctx *ctx, const bd_tuning *t);
 int bd_get_tuning(bd_ctx *ctx, bd_tuning *t);
+
+/* A plain device copy of `bytes` bytes (read + write), `reps` times (no
+ * reference counterpart: the measurement reference for the bottom-up
+ * kernel's hardware traffic counters and achievable HBM bandwidth).  *gbps
+ * receives the best rate, read plus written bytes per second / 1e9. */
+int bd_copy_bandwidth(bd_ctx *ctx, int64_t bytes, int reps, double *gbps);
 
 /* ------------------------------------------------------- batched searches */
 
@@ -229,7 +239,10 @@ int bd_get_tuning(bd_ctx *ctx, bd_tuning *t);
  * or pulls them from neighbours, chosen per level from the frontier's and
  * the open vertices' degree sums.  depths (may be NULL) receives
  * depths[r*nv + v] = depth of v from roots[r] (-1 unreached); *ms the time
- * from the first kernel to the last; *levels the number of levels. */
+ * from the first kernel to the last; *levels the number of levels.
+ * bd_bfs_levels then returns one record per level: dir 0 = push, 1 = pull;
+ * frontier / frontier_deg = vertices holding any root's frontier and their
+ * degree sum; next = vertices gaining a root; ms = the level's time. */
 int bd_bfs_multi_root(bd_ctx *ctx, const int64_t *roots, int n, int32_t *depths,
                       double *ms, int64_t *levels);
 
