@@ -112,6 +112,7 @@ def lib():
         "bd_get_tuning": ([vp, ctypes.POINTER(Tuning)], ctypes.c_int),
         "bd_bfs_multi_root": ([vp, _i64p, ctypes.c_int, vp, ctypes.POINTER(ctypes.c_double),
                                ctypes.POINTER(i64)], ctypes.c_int),
+        "bd_bfs_launch_times": ([vp, vp, i64, ctypes.POINTER(i64)], ctypes.c_int),
         "bd_copy_bandwidth": ([vp, i64, ctypes.c_int, ctypes.POINTER(ctypes.c_double)],
                               ctypes.c_int),
         "bd_bfs_multi_graph": ([vp, ctypes.c_int, _i64p, _i64p, vp,
@@ -319,6 +320,15 @@ class Context:
         for k, v in kw.items():
             setattr(t, k, int(v))
         self._ok(self._L.bd_set_tuning(self._h, ctypes.byref(t)))
+
+    def launch_times(self):
+        """Kernel time (ms) of every step launch of the last search run with
+        time_steps=3."""
+        n = ctypes.c_int64(0)
+        self._ok(self._L.bd_bfs_launch_times(self._h, None, 0, ctypes.byref(n)))
+        out = np.empty(n.value, np.float64)
+        self._ok(self._L.bd_bfs_launch_times(self._h, out.ctypes.data, n.value, ctypes.byref(n)))
+        return out
 
     def copy_bandwidth(self, nbytes, reps=3):
         """Best rate of a plain device copy of nbytes (read + write GB/s)."""

@@ -1290,28 +1290,29 @@ step_args make_args(bd_ctx *c, int32_t level, int64_t alpha, int64_t beta, int p
 int read_ctl(bd_ctx *c);
 
 template <int B, int D>
-void launch_one(bd_ctx *c, const step_args &a, int64_t grid, bool timed) {
-    if (timed)  // the events are stamped by the dispatch itself: kernel time only
-        hipExtLaunchKernelGGL((k_step<B, D>), dim3(grid), dim3(B), 0, c->stream, c->ev[2],
-                              c->ev[3], 0, a);
+void launch_one(bd_ctx *c, const step_args &a, int64_t grid, hipEvent_t e0, hipEvent_t e1) {
+    if (e0 || e1)  // the events are stamped by the dispatch itself: kernel time only
+        hipExtLaunchKernelGGL((k_step<B, D>), dim3(grid), dim3(B), 0, c->stream, e0, e1, 0, a);
     else
         hipLaunchKernelGGL((k_step<B, D>), dim3(grid), dim3(B), 0, c->stream, a);
 }
 
 template <int B>
-void launch_step_block(bd_ctx *c, const step_args &a, int dirs, int64_t grid, bool timed) {
+void launch_step_block(bd_ctx *c, const step_args &a, int dirs, int64_t grid, hipEvent_t e0,
+                       hipEvent_t e1) {
     if (dirs == kStepTD)
-        launch_one<B, kStepTD>(c, a, grid, timed);
+        launch_one<B, kStepTD>(c, a, grid, e0, e1);
     else if (dirs == kStepBU)
-        launch_one<B, kStepBU>(c, a, grid, timed);
+        launch_one<B, kStepBU>(c, a, grid, e0, e1);
     else
-        launch_one<B, kStepBoth>(c, a, grid, timed);
+        launch_one<B, kStepBoth>(c, a, grid, e0, e1);
 }
 
 // Launch the step kernel carrying `dirs` (kStepTD, kStepBU or kStepBoth);
-// grid <= 0 takes the tuned workgroups per CU of that kernel.  timed: the
-// launch stamps events ev[2] and ev[3] at the kernel's start and end.
-int launch_step(bd_ctx *c, const step_args &a, int dirs, int64_t grid = 0, bool timed = false) {
+// grid <= 0 takes the tuned workgroups per CU of that kernel.  With events,
+// the launch stamps e0 at the kernel's start and e1 at its end.
+int launch_step(bd_ctx *c, const step_args &a, int dirs, int64_t grid = 0,
+                hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr) {
     const int blk = norm_block(c->tune.block);
     if (grid <= 0) {
         // the bottom-up grid is its own only when it has its own kernel
@@ -1321,13 +1322,13 @@ int launch_step(bd_ctx *c, const step_args &a, int dirs, int64_t grid = 0, bool 
     }
     switch (blk) {
     case 128:
-        launch_step_block<128>(c, a, dirs, grid, timed);
+        launch_step_block<128>(c, a, dirs, grid, e0, e1);
         break;
     case 512:
-        launch_step_block<512>(c, a, dirs, grid, timed);
+        launch_step_block<512>(c, a, dirs, grid, e0, e1);
         break;
     default:
-        launch_step_block<256>(c, a, dirs, grid, timed);
+        launch_step_block<256>(c, a, dirs, grid, e0, e1);
     }
     BD_HIP(c, hipGetLastError());
     return BD_OK;
@@ -1336,11 +1337,22 @@ int launch_step(bd_ctx *c, const step_args &a, int dirs, int64_t grid = 0, bool 
 // All work of level L in device-driven order: the kernel of each direction
 // (the one the device did not choose returns after its state rebuild), or
 // the kernel carrying both.
-int launch_level(bd_ctx *c, const step_args &a) {
+int launch_level(bd_ctx *c, const step_args &a, hipEvent_t e0 = nullptr,
+                 hipEvent_t e1 = nullptr) {
     int rc;
-    if (c->tune.step_kernels == 2) return launch_step(c, a, kStepBoth);
-    if ((rc = launch_step(c, a, kStepBU))) return rc;
-    return launch_step(c, a, kStepTD);
+    if (c->tune.step_kernels == 2) return launch_step(c, a, kStepBoth, 0, e0, e1);
+    if ((rc = launch_step(c, a, kStepBU, 0, e0, nullptr))) return rc;
+    return launch_step(c, a, kStepTD, 0, nullptr, e1);
+}
+
+// Timing events for the launches of one chunk of device-driven levels.
+int ensure_event_pool(bd_ctx *c, size_t n) {
+    while (c->ev_pool.size() < n) {
+        hipEvent_t e = nullptr;
+        BD_HIP(c, hipEventCreateWithFlags(&e, hipEventReleaseToDevice));
+        c->ev_pool.push_back(e);
+    }
+    return BD_OK;
 }
 
 // Host-driven levels: publish the state before level L with one workgroup
@@ -1356,8 +1368,12 @@ int close_level(bd_ctx *c, const step_args &a) {
 }
 
 int launch_init(bd_ctx *c, const step_args &a, int64_t source, const int64_t *srcs,
-                int64_t nsrc) {
-    hipLaunchKernelGGL(k_init, dim3(1), dim3(64), 0, c->stream, a, source, srcs, nsrc);
+                int64_t nsrc, hipEvent_t stop = nullptr) {
+    if (stop)  // stamped when k_init ends, i.e. when the first step starts
+        hipExtLaunchKernelGGL(k_init, dim3(1), dim3(64), 0, c->stream, nullptr, stop, 0, a, source,
+                              srcs, nsrc);
+    else
+        hipLaunchKernelGGL(k_init, dim3(1), dim3(64), 0, c->stream, a, source, srcs, nsrc);
     BD_HIP(c, hipGetLastError());
     return BD_OK;
 }
@@ -1375,7 +1391,7 @@ int check_run_args(bd_ctx *c, int64_t source, int64_t alpha, int64_t beta, int p
 }
 
 int start_search(bd_ctx *c, int64_t source, int64_t alpha, int64_t beta, int policy,
-                 const int64_t *srcs = nullptr, int64_t nsrc = 1) {
+                 const int64_t *srcs = nullptr, int64_t nsrc = 1, hipEvent_t init_end = nullptr) {
     const int64_t n = std::max(c->nv, c->nwords);
     hipLaunchKernelGGL(k_clear, dim3(grid_for(c, n, 256)), dim3(256), 0, c->stream, c->d_depth,
                        c->d_parent, c->nv, c->d_vis, c->d_dead, c->d_bm[0], c->d_bm[1],
@@ -1386,7 +1402,7 @@ int start_search(bd_ctx *c, int64_t source, int64_t alpha, int64_t beta, int pol
     a.next_bm = c->d_bm[0];
     a.qout = c->d_q[0];
     a.tout = c->d_tasks[0];
-    return launch_init(c, a, source, srcs, nsrc);
+    return launch_init(c, a, source, srcs, nsrc, init_end);
 }
 
 // Copy both published states to the pinned mirror and wait.
@@ -1455,7 +1471,12 @@ int run_search(bd_ctx *c, int64_t source, int64_t alpha, int64_t beta, int polic
     int rc;
     if ((rc = check_run_args(c, source, alpha, beta, policy))) return rc;
     BD_HIP(c, hipSetDevice(c->device));
-    if ((rc = start_search(c, source, alpha, beta, policy, srcs, nsrc))) return rc;
+    const bool per_chunk = c->tune.time_steps == 4;
+    if (per_chunk && (rc = ensure_event_pool(c, 2 * kMaxLevelsPerSync))) return rc;
+    // time_steps 4: the first chunk's span starts where k_init ends
+    if ((rc = start_search(c, source, alpha, beta, policy, srcs, nsrc,
+                           per_chunk ? c->ev_pool[0] : nullptr)))
+        return rc;
     BD_HIP(c, hipEventRecord(c->ev[0], c->stream));
     const int64_t max_levels = c->nv + 2;
     int32_t L = 0;
@@ -1469,7 +1490,9 @@ int run_search(bd_ctx *c, int64_t source, int64_t alpha, int64_t beta, int polic
             if (c->h_ctl[L & 1].done) break;
             // the host knows the direction: launch that direction's kernel
             const int dirs = c->h_ctl[L & 1].dir;
-            if ((rc = launch_step(c, a, dirs, 0, timed))) return rc;
+            if ((rc = launch_step(c, a, dirs, 0, timed ? c->ev[2] : nullptr,
+                                  timed ? c->ev[3] : nullptr)))
+                return rc;
             if (timed) {
                 BD_HIP(c, hipEventSynchronize(c->ev[3]));
                 float ms = 0;
@@ -1478,13 +1501,39 @@ int run_search(bd_ctx *c, int64_t source, int64_t alpha, int64_t beta, int polic
             }
         }
     } else {
-        int chunk = 8;
+        // time_steps 3: every launch carries kernel-stamped events;
+        // time_steps 4: one event pair spans each chunk of launches, and each
+        // launch of the chunk is given the chunk's mean
+        const bool per_launch = c->tune.time_steps == 3;
+        if (per_launch && (rc = ensure_event_pool(c, 2 * kMaxLevelsPerSync))) return rc;
+        c->last_launch_ms.clear();
+        int chunk = 16;  // most searches end within the first chunk
         for (;;) {
             for (int k = 0; k < chunk; ++k, ++L) {
                 const step_args a = make_args(c, L, alpha, beta, policy);
-                if ((rc = launch_level(c, a))) return rc;
+                hipEvent_t e0 = nullptr, e1 = nullptr;
+                if (per_launch) {
+                    e0 = c->ev_pool[2 * k];
+                    e1 = c->ev_pool[2 * k + 1];
+                } else if (per_chunk) {
+                    // later chunks start after a host wait: their first
+                    // launch stamps the start
+                    e0 = (k == 0 && L > 0) ? c->ev_pool[0] : nullptr;
+                    e1 = k == chunk - 1 ? c->ev_pool[1] : nullptr;
+                }
+                if ((rc = launch_level(c, a, e0, e1))) return rc;
             }
             if ((rc = read_ctl(c))) return rc;
+            for (int k = 0; per_launch && k < chunk; ++k) {
+                float ms = 0;
+                BD_HIP(c, hipEventElapsedTime(&ms, c->ev_pool[2 * k], c->ev_pool[2 * k + 1]));
+                c->last_launch_ms.push_back(ms);
+            }
+            if (per_chunk) {
+                float ms = 0;
+                BD_HIP(c, hipEventElapsedTime(&ms, c->ev_pool[0], c->ev_pool[1]));
+                for (int k = 0; k < chunk; ++k) c->last_launch_ms.push_back(ms / chunk);
+            }
             if (final_ctl(c)) break;
             if (L > max_levels) return set_err(c, BD_EHIP, "search did not terminate");
             chunk = std::min(chunk * 2, kMaxLevelsPerSync);
@@ -1496,6 +1545,7 @@ int run_search(bd_ctx *c, int64_t source, int64_t alpha, int64_t beta, int polic
     BD_HIP(c, hipEventElapsedTime(&ev_ms, c->ev[0], c->ev[1]));
     if ((rc = read_ctl(c))) return rc;
     if ((rc = finish_search(c, source, out, ev_ms))) return rc;
+    if (c->tune.time_steps == 3) step_ms = c->last_launch_ms;  // per-launch times only
     for (size_t i = 0; i < step_ms.size() && i < c->last_log.size(); ++i)
         c->last_log[i].ms = step_ms[i];
     return BD_OK;
@@ -1675,7 +1725,7 @@ extern "C" int bd_bfs_profile(bd_ctx *c, int64_t source, int64_t alpha, int64_t 
             }
             hipLaunchKernelGGL(k_scrub, dim3(c->cu_count * 4), dim3(256), 0, c->stream, scrub,
                                scrub_n, scrub + scrub_n);
-            if ((rc = launch_step(c, a, a.force_dir, 0, true))) return rc;
+            if ((rc = launch_step(c, a, a.force_dir, 0, c->ev[2], c->ev[3]))) return rc;
             BD_HIP(c, hipEventSynchronize(c->ev[3]));
             BD_HIP(c, hipEventElapsedTime(&ms[pass], c->ev[2], c->ev[3]));
             BD_HIP(c, hipMemcpy(acc.data(), c->d_seg->acc[L % 3],
@@ -1776,3 +1826,16 @@ extern "C" int bd_bfs_multi_graph(bd_ctx *c, int ngraphs, const int64_t *vbase,
     if (depths) return bd_bfs_fetch(c, depths, nullptr);
     return BD_OK;
 }
+
+// Kernel time of every step launch of the last search run with time_steps 3
+// (device-driven, one event pair per launch) or 4 (one pair per chunk of
+// launches, each launch given the chunk's mean), including the launches
+// queued after the search ended, which return at once.
+extern "C" int bd_bfs_launch_times(bd_ctx *c, double *ms, int64_t cap, int64_t *n) {
+    if (!c || !n || cap < 0 || (cap > 0 && !ms)) return BD_EINVAL;
+    const int64_t k = std::min<int64_t>(cap, (int64_t)c->last_launch_ms.size());
+    for (int64_t i = 0; i < k; ++i) ms[i] = c->last_launch_ms[i];
+    *n = (int64_t)c->last_launch_ms.size();
+    return BD_OK;
+}
+

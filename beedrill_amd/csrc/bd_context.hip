@@ -162,7 +162,13 @@ extern "C" int bd_create(bd_ctx **out, int device) {
     if (e == hipSuccess) e = hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, device);
     if (e == hipSuccess && khz > 0) c->wall_khz = khz;
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
-    for (int i = 0; i < 4 && e == hipSuccess; ++i) e = hipEventCreate(&c->ev[i]);
+    // ev[0..1] bracket whole searches; ev[2..3] time single kernels and use
+    // a device-scope release, so a kernel's time does not include writing
+    // the L2 back for the host (results are read with hipMemcpy, which
+    // orders itself)
+    for (int i = 0; i < 4 && e == hipSuccess; ++i)
+        e = i < 2 ? hipEventCreate(&c->ev[i])
+                  : hipEventCreateWithFlags(&c->ev[i], hipEventReleaseToDevice);
     if (e == hipSuccess) e = hipMalloc((void **)&c->d_ctl, 2 * sizeof(ctl_block));
     if (e == hipSuccess) e = hipMalloc((void **)&c->d_seg, sizeof(seg_counters));
     if (e == hipSuccess) e = hipHostMalloc((void **)&c->h_ctl, 2 * sizeof(ctl_block), hipHostMallocDefault);
@@ -192,6 +198,8 @@ extern "C" int bd_destroy(bd_ctx *c) {
     dev_free(c->d_mlp_ops);
     if (c->h_ctl) (void)hipHostFree(c->h_ctl);
     for (auto &e : c->ev)
+        if (e) (void)hipEventDestroy(e);
+    for (auto &e : c->ev_pool)
         if (e) (void)hipEventDestroy(e);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
@@ -256,7 +264,7 @@ extern "C" int bd_set_tuning(bd_ctx *c, const bd_tuning *t) {
     if (t->task_threshold > 0) n.task_threshold = std::max(t->task_threshold, 16);
     if (t->task_edges > 0) n.task_edges = std::max(t->task_edges, 64);
     if (t->level_driver == 1 || t->level_driver == 2) n.level_driver = t->level_driver;
-    if (t->time_steps == 1 || t->time_steps == 2) n.time_steps = t->time_steps;
+    if (t->time_steps >= 1 && t->time_steps <= 4) n.time_steps = t->time_steps;
     if (t->bu_head == 1 || t->bu_head == 2) n.bu_head = t->bu_head;
     if (t->step_kernels == 1 || t->step_kernels == 2) n.step_kernels = t->step_kernels;
     if (t->bu_grid_per_cu > 0) n.bu_grid_per_cu = std::min(t->bu_grid_per_cu, 64);

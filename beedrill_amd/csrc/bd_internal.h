@@ -152,6 +152,8 @@ struct bd_ctx {
 
     bd_tuning tune{};
     hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+    std::vector<hipEvent_t> ev_pool;     // per-launch timing events (device-scope release)
+    std::vector<double> last_launch_ms;  // kernel time of every step launch of the last search
 };
 
 namespace bd {

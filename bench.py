@@ -27,10 +27,10 @@ sys.path.insert(0, ROOT)
 METRIC = "harmonic-mean GTEPS over 64 roots, RMAT s24; learned vs Beamer policy"
 HBM_PEAK_GBS = 8000.0
 MODEL = os.path.join(ROOT, "beedrill_amd", "models", "direction_mlp_32x32.f32")
-TRAFFIC = os.path.join(ROOT, "profiles", "bottom_up_traffic_rmat_s24.json")
-# The bottom-up instance of the step kernel, as rocprofv3 names it
-# (k_step<workgroup size, 1>).
-BU_KERNEL_RE = r"k_step<\d+, 1>"
+TRAFFIC = os.path.join(ROOT, "profiles", "step_kernel_traffic_rmat_s24.json")
+# The step kernel (both directions) as rocprofv3 names it:
+# k_step<workgroup size, 2>.
+STEP_KERNEL_RE = r"k_step<\d+, 2>"
 COPY_BYTES = 1 << 30  # calibration copy: 1 GiB read + 1 GiB written
 
 
@@ -72,35 +72,61 @@ def bu_bytes(rec, nv, n_isolated):
     return nv / 8 + 16 * U + 4 * E + E / 8 + 8 * rec["next"] + nv / 8
 
 
+def td_bytes(rec):
+    """Algorithmic bytes of one top-down step: frontier ids (4 |F|), their
+    offsets (16 |F|), the neighbour ids (4 sum deg F), one visited probe per
+    neighbour (4 sum deg F), and depth, parent and queue entry of each newly
+    reached vertex (12 |N|)."""
+    F, D = rec["frontier"], rec["frontier_deg"]
+    return 4 * F + 16 * F + 4 * D + 4 * D + 12 * rec["next"]
+
+
 def roofline(ctx, roots, policy, nv, n_isolated):
-    """Bottom-up kernel against the HBM roofline.  The searches run with the
-    host reading the state between levels, so every bottom-up step is one
-    launch of the bottom-up kernel alone, timed with HIP events on the
-    library's stream; achieved = algorithmic bytes / launch time, summed over
-    the bottom-up launches of these searches."""
+    """The step kernel against the HBM roofline.  Searches run device-driven,
+    as in the timed region, with HIP events on the library's stream spanning
+    each chunk of back-to-back step launches (bd_tuning.time_steps = 4): the
+    mean launch is the quantity rocprofv3 reports for the kernel.  achieved =
+    the algorithmic bytes of all steps / the time of all launches (the
+    launches queued after a search ended count, with no bytes).  bu_levels:
+    a second pass timing every launch alone (time_steps = 3) gives the
+    bottom-up steps' own rate; those per-launch events add a few
+    microseconds each, so it is a lower bound."""
     t = ctx.get_tuning()
-    ctx.set_tuning(time_steps=1)
-    tot_b, tot_ms, n = 0.0, 0.0, 0
+    tot_b = tot_ms = bu_b = bu_ms = 0.0
+    launches = bu_n = 0
     try:
+        ctx.set_tuning(time_steps=4)
         for r in roots:
             ctx.bfs(int(r), 15, 18, policy)
+            lt = ctx.launch_times()
+            launches += len(lt)
+            tot_ms += float(lt.sum())
             for rec in ctx.levels():
-                if rec["dir"] == 1 and rec["ms"] > 0:
-                    tot_b += bu_bytes(rec, nv, n_isolated)
-                    tot_ms += rec["ms"]
-                    n += 1
+                tot_b += bu_bytes(rec, nv, n_isolated) if rec["dir"] == 1 else td_bytes(rec)
+        ctx.set_tuning(time_steps=3)
+        for r in roots:
+            ctx.bfs(int(r), 15, 18, policy)
+            lt = ctx.launch_times()
+            for rec in ctx.levels():
+                if rec["dir"] == 1:
+                    bu_b += bu_bytes(rec, nv, n_isolated)
+                    bu_ms += lt[rec["level"]]
+                    bu_n += 1
     finally:
-        ctx.set_tuning(time_steps=2, level_driver=t["level_driver"])
+        ctx.set_tuning(time_steps=t["time_steps"])
     achieved = tot_b / (tot_ms * 1e-3) / 1e9 if tot_ms > 0 else 0.0
+    bu_ach = bu_b / (bu_ms * 1e-3) / 1e9 if bu_ms > 0 else 0.0
     traffic = None
     if os.path.exists(TRAFFIC):
         with open(TRAFFIC) as f:
             traffic = json.load(f).get("bytes_per_launch")
-    return {"kernel": f"k_step<{t['block']}, 1>", "bound": "hbm", "achieved": round(achieved, 1),
+    return {"kernel": f"k_step<{t['block']}, 2>", "bound": "hbm", "achieved": round(achieved, 1),
             "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": traffic, "launches": n,
-            "algorithmic_bytes_per_launch": round(tot_b / max(n, 1)),
-            "avg_launch_ms": round(tot_ms / max(n, 1), 5)}
+            "traffic": traffic, "launches": launches,
+            "algorithmic_bytes_per_launch": round(tot_b / max(launches, 1)),
+            "avg_launch_ms": round(tot_ms / max(launches, 1), 5),
+            "bu_levels": {"achieved": round(bu_ach, 1), "frac": round(bu_ach / HBM_PEAK_GBS, 4),
+                          "launches": bu_n, "avg_launch_ms": round(bu_ms / max(bu_n, 1), 5)}}
 
 
 def copy_calibration(ctx):
@@ -111,7 +137,7 @@ def copy_calibration(ctx):
 
 
 def summarize_pmc(fetch_csv, write_csv, out_path):
-    """Traffic per bottom-up launch from two rocprofv3 --pmc runs of the
+    """Traffic per step-kernel launch from two rocprofv3 --pmc runs of the
     roofline leg (FETCH_SIZE, then WRITE_SIZE; KiB per dispatch), each scaled
     by what the counter reads for the calibration copy of known size."""
     import csv
@@ -130,17 +156,17 @@ def summarize_pmc(fetch_csv, write_csv, out_path):
     for counter, path, copy_bytes in (("FETCH_SIZE", fetch_csv, COPY_BYTES),
                                       ("WRITE_SIZE", write_csv, COPY_BYTES)):
         rows = per_kernel(path, counter)
-        bu = [v for k, vs in rows.items() if re.search(BU_KERNEL_RE, k) for v in vs]
+        bu = [v for k, vs in rows.items() if re.search(STEP_KERNEL_RE, k) for v in vs]
         cp = [v for k, vs in rows.items() if "k_copy" in k for v in vs]
         if not bu or not cp:
-            raise SystemExit(f"{path}: no bottom-up or copy dispatches for {counter}")
+            raise SystemExit(f"{path}: no step-kernel or copy dispatches for {counter}")
         scale = copy_bytes / (float(np.median(cp)) * 1024.0)
         out[counter] = {"kib_per_launch": float(np.mean(bu)), "launches": len(bu),
                         "copy_kib": float(np.median(cp)), "scale": scale,
                         "bytes_per_launch": float(np.mean(bu)) * 1024.0 * scale}
     out["bytes_per_launch"] = round(out["FETCH_SIZE"]["bytes_per_launch"] +
                                     out["WRITE_SIZE"]["bytes_per_launch"])
-    out["kernel"] = sorted(k for k in rows if re.search(BU_KERNEL_RE, k))
+    out["kernel"] = sorted(k for k in rows if re.search(STEP_KERNEL_RE, k))
     with open(out_path, "w") as f:
         json.dump(out, f, indent=1)
     print(json.dumps(out))
@@ -228,9 +254,10 @@ def main():
     policy = "learned" if have_model else "beamer"
 
     if args.leg == "roofline":
-        for r in roots[:args.warmup]:
-            ctx.bfs(int(r), 15, 18, policy)
-        roof = roofline(ctx, roots[:args.roofline_roots], policy, nv, n_isolated)
+        # warm-up searches are timed the same way, so that every step launch
+        # of this process is in the average (as in rocprofv3's)
+        roof = roofline(ctx, list(roots[:args.warmup]) + list(roots[:args.roofline_roots]),
+                        policy, nv, n_isolated)
         copy_gbs = copy_calibration(ctx)
         print(json.dumps({"leg": "roofline", "roofline": roof, "copy_gbs": round(copy_gbs, 1)}))
         return
@@ -244,7 +271,8 @@ def main():
     batch = batched_roots(ctx, roots, results["beamer"]["mean_ms"] * len(roots))
     log(f"64 roots batched: {batch['ms']:.3f} ms vs {batch['sequential_ms']:.3f} ms sequential")
     roof = roofline(ctx, roots[:args.roofline_roots], policy, nv, n_isolated)
-    log(f"roofline: {roof['achieved']} GB/s over {roof['launches']} bottom-up launches")
+    log(f"roofline: {roof['achieved']} GB/s over {roof['launches']} step launches; bottom-up "
+        f"steps {roof['bu_levels']['achieved']} GB/s")
     cpu = None if args.no_cpu else cpu_baseline(ctx, roots, args.cpu_budget)
 
     out = {

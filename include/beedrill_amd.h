@@ -178,6 +178,14 @@ int bd_bfs_check(bd_ctx *ctx, int64_t source, int *ok);
 int bd_bfs_profile(bd_ctx *ctx, int64_t source, int64_t alpha, int64_t beta, int policy,
                    bd_run_stats *out);
 
+/* Kernel time of every step launch of the last search run with
+ * bd_tuning.time_steps = 3 (each launch timed) or 4 (each chunk of launches
+ * timed, its launches given the chunk's mean), in launch order (level L is
+ * launch L; launches queued after the search ended return at once and are
+ * listed too).  No
+ * reference counterpart: the measurement behind the step kernel's roofline. */
+int bd_bfs_launch_times(bd_ctx *ctx, double *ms, int64_t cap, int64_t *n);
+
 /* Launch tuning (autotuner output).  Any field <= 0 keeps the default. */
 typedef struct {
     int32_t block;           /* threads per workgroup of the step kernel:
@@ -197,8 +205,12 @@ typedef struct {
     int32_t level_driver;    /* 1 = direction chosen on device and steps
                                 enqueued without a per-step host sync;
                                 2 = host reads the counters after each step */
-    int32_t time_steps;      /* 1 = time each step's kernels with HIP events
-                                (implies the host-driven loop), 2 = off */
+    int32_t time_steps;      /* 1 = time each step's kernel with HIP events
+                                (implies the host-driven loop); 2 = off;
+                                3 = device-driven levels, every step launch
+                                timed by events stamped at its dispatch;
+                                4 = device-driven levels, one event pair per
+                                chunk of launches (bd_bfs_launch_times)   */
     int32_t bu_head;         /* bottom-up: 1 = test each vertex's head index
                                 (its highest-degree neighbour) before its
                                 list; 2 = scan lists only, in CSR order   */
