@@ -1181,10 +1181,12 @@ __device__ __forceinline__ unsigned long long head_key(const int64_t *off, int32
 // shorter than 64, longer lists are done by the whole wave.
 __global__ __launch_bounds__(256) void k_head_index(const int64_t *__restrict__ off,
                                                     const int32_t *__restrict__ cols, int64_t nv,
-                                                    uint64_t *__restrict__ head) {
+                                                    uint64_t *__restrict__ head,
+                                                    unsigned long long *__restrict__ max_deg) {
     const int lane = threadIdx.x & 63;
     const int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
     const int64_t nwaves = ((int64_t)gridDim.x * blockDim.x) >> 6;
+    unsigned long long dmax = 0;
     for (int64_t base = wave * 64; base < nv; base += nwaves * 64) {
         const int64_t v = base + lane;
         const bool valid = v < nv;
@@ -1192,6 +1194,7 @@ __global__ __launch_bounds__(256) void k_head_index(const int64_t *__restrict__ 
         if (valid) {
             b = off[v];
             e = off[v + 1];
+            dmax = (unsigned long long)(e - b) > dmax ? (unsigned long long)(e - b) : dmax;
         }
         const bool alone = valid && e - b < 64;
         if (alone) {
@@ -1221,6 +1224,8 @@ __global__ __launch_bounds__(256) void k_head_index(const int64_t *__restrict__ 
                           (uint32_t)cols[kb + (int64_t)(0xffffffffULL - (best & 0xffffffffULL))];
         }
     }
+    dmax = wave_max(dmax);
+    if (lane == 0 && dmax) atomicMax(max_deg, dmax);
 }
 
 // Streams a buffer larger than the last-level cache so a timed step starts
@@ -1282,7 +1287,11 @@ step_args make_args(bd_ctx *c, int32_t level, int64_t alpha, int64_t beta, int p
     a.log_cap = (int32_t)c->log_cap;
     a.mlp_h1 = c->mlp_h1;
     a.mlp_h2 = c->mlp_h2;
-    a.has_mlp = c->has_mlp ? 1 : 0;
+    // The network was trained on graphs with hubs; on a graph whose largest
+    // degree is under learned_min_skew times the mean, LEARNED runs Beamer.
+    const bool skewed = (double)c->max_deg * (double)c->nv >=
+                        (double)std::max(1, c->tune.learned_min_skew) * (double)c->nnz;
+    a.has_mlp = (c->has_mlp && skewed) ? 1 : 0;
     a.mlp_margin = (float)c->tune.learned_margin_milli * 1e-3f;
     return a;
 }
@@ -1455,10 +1464,20 @@ int build_head_index(bd_ctx *c) {
     if (c->d_head || c->nv == 0) return BD_OK;
     int rc = dev_alloc(c, (void **)&c->d_head, sizeof(uint64_t) * c->nv, "head index");
     if (rc) return rc;
+    unsigned long long *d_max = nullptr;
+    if ((rc = dev_alloc(c, (void **)&d_max, sizeof(unsigned long long), "max degree"))) return rc;
+    struct guard {
+        void *p;
+        ~guard() { dev_free(p); }
+    } g{d_max};
+    BD_HIP(c, hipMemsetAsync(d_max, 0, sizeof(unsigned long long), c->stream));
     hipLaunchKernelGGL(k_head_index, dim3(grid_for(c, c->nv, 256)), dim3(256), 0, c->stream,
-                       c->d_off, c->d_cols, c->nv, c->d_head);
+                       c->d_off, c->d_cols, c->nv, c->d_head, d_max);
     BD_HIP(c, hipGetLastError());
+    unsigned long long h = 0;
+    BD_HIP(c, hipMemcpyAsync(&h, d_max, sizeof(h), hipMemcpyDeviceToHost, c->stream));
     BD_HIP(c, hipStreamSynchronize(c->stream));
+    c->max_deg = (int64_t)h;
     return BD_OK;
 }
 

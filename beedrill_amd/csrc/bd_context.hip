@@ -69,6 +69,7 @@ void free_graph(bd_ctx *c) {
     dev_free(c->d_head);
     dev_free(c->d_el);
     c->d_head = nullptr;
+    c->max_deg = 0;
     c->d_off = nullptr;
     c->d_cols = nullptr;
     c->d_dead = nullptr;
@@ -152,6 +153,7 @@ extern "C" int bd_create(bd_ctx **out, int device) {
     c->tune.bu_grid_per_cu = 8;
     c->tune.learned_margin_milli = 250;
     c->tune.ms_alpha = 4;
+    c->tune.learned_min_skew = 8;
     hipError_t e = hipSetDevice(device);
     if (e == hipSuccess) {
         hipDeviceProp_t prop;
@@ -270,6 +272,7 @@ extern "C" int bd_set_tuning(bd_ctx *c, const bd_tuning *t) {
     if (t->bu_grid_per_cu > 0) n.bu_grid_per_cu = std::min(t->bu_grid_per_cu, 64);
     if (t->learned_margin_milli > 0) n.learned_margin_milli = t->learned_margin_milli;
     if (t->ms_alpha > 0) n.ms_alpha = t->ms_alpha;
+    if (t->learned_min_skew > 0) n.learned_min_skew = t->learned_min_skew;
     bool realloc = (n.task_edges != c->tune.task_edges || n.task_threshold != c->tune.task_threshold);
     c->tune = n;
     if (realloc && c->d_off) return alloc_search(c);

@@ -108,6 +108,7 @@ struct bd_ctx {
     int32_t *d_cols = nullptr;
     uint64_t *d_dead = nullptr;   // 1 = vertex of degree 0, or padding bit
     uint64_t *d_head = nullptr;   // bottom-up head index (built with the graph)
+    int64_t max_deg = 0;          // largest degree (found with the head index)
     uint64_t *d_el = nullptr;     // el64 order, packed (src << 32) | dst
     int64_t el_m = 0;
 

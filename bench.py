@@ -183,6 +183,84 @@ def batched_roots(ctx, roots, sequential_ms):
             "speedup_vs_sequential": round(sequential_ms / ms, 3) if ms > 0 else None}
 
 
+def hmean_policy(ctx, roots, policy):
+    g = []
+    for r in roots:
+        st = ctx.bfs(int(r), 15, 18, policy)
+        g.append(st["traversed_edges"] / (st["ms"] * 1e-3) / 1e9)
+    return hmean(g)
+
+
+def leg_batches(ctx, args):
+    """Batched searches: (a) 64 roots of RMAT scale 23 in one bit-parallel
+    search against the 64 searches one by one; (b) 256 RMAT graphs of uneven
+    size (scale uniform in 10-16, edge factor uniform in 8-32, generator
+    seeds 1000+g) searched from one root each at once, against one search
+    per graph."""
+    out = []
+    ctx.generate_rmat(23, 16, seed=args.seed)
+    roots = ctx.pick_roots(64)
+    for r in roots[:2]:
+        ctx.bfs(int(r))
+    seq = sum(ctx.bfs(int(r))["ms"] for r in roots)
+    ctx.multi_root(roots[:8], want_depths=False)
+    _, ms, lv = ctx.multi_root(roots, want_depths=True)
+    out.append({"batch": "64 roots, RMAT scale 23, bit-parallel", "ms": round(ms, 3),
+                "sequential_ms": round(seq, 3), "speedup": round(seq / ms, 3), "levels": lv})
+    log(f"64 roots s23: {ms:.2f} ms batched vs {seq:.2f} ms sequential")
+    rng = np.random.default_rng(7)
+    graphs, roots, seq_ms = [], [], 0.0
+    for g in range(256):
+        scale, ef = int(rng.integers(10, 17)), int(rng.integers(8, 33))
+        ctx.generate_rmat(scale, ef, seed=1000 + g)
+        off, cols = ctx.csr()
+        _, _, m = ctx.info()
+        root = int(ctx.pick_roots(1)[0])
+        ctx.bfs(root)
+        seq_ms += ctx.bfs(root)["ms"]
+        graphs.append((off, cols, m))
+        roots.append(root)
+    vbase = ctx.upload_union(graphs)
+    ctx.multi_graph(vbase, roots, want_depths=False)
+    _, ms = ctx.multi_graph(vbase, roots, want_depths=True)
+    edges = ctx.count_traversed()
+    out.append({"batch": "256 RMAT graphs, scales 10-16, edge factors 8-32, one root each",
+                "vertices": int(vbase[-1]), "edges": int(sum(m for _, _, m in graphs)),
+                "ms": round(ms, 3), "sequential_ms": round(seq_ms, 3),
+                "speedup": round(seq_ms / ms, 3), "gteps": round(edges / (ms * 1e-3) / 1e9, 2)})
+    log(f"256 graphs: {ms:.2f} ms at once vs {seq_ms:.2f} ms one by one")
+    return out
+
+
+def leg_stress(ctx, args, have_model):
+    """Same-size stress graphs with the learned policy judged against Beamer:
+    RMAT scale 26, an Erdos-Renyi graph of the same size (uniform endpoints,
+    16 * 2^26 edges) and the 4096 x 4096 four-neighbour lattice.  One root of
+    each is checked in full (hybrid_bfs::check on the host)."""
+    out = []
+    for name, gen, nroots in (("RMAT scale 26", lambda: ctx.generate_rmat(26, 16, seed=args.seed), 16),
+                              ("Erdos-Renyi, 2^26 vertices, 2^30 edges",
+                               lambda: ctx.generate_er(1 << 26, 16 << 26, seed=args.seed), 16),
+                              ("lattice 4096 x 4096", lambda: ctx.generate_lattice(4096, 4096), 2)):
+        t0 = time.perf_counter()
+        m = gen()
+        build = time.perf_counter() - t0
+        roots = ctx.pick_roots(nroots)
+        ctx.bfs(int(roots[0]))
+        rec = {"graph": name, "num_vertices": ctx.info()[0], "num_edges": m,
+               "build_s": round(build, 2), "roots": nroots,
+               "beamer_gteps": round(hmean_policy(ctx, roots, "beamer"), 3)}
+        st = ctx.bfs(int(roots[0]))
+        rec["levels"] = int(st["steps"])
+        rec["check_root0"] = ctx.check(int(roots[0]))
+        if have_model:
+            rec["learned_gteps"] = round(hmean_policy(ctx, roots, "learned"), 3)
+            rec["learned_vs_beamer"] = round(rec["learned_gteps"] / rec["beamer_gteps"], 4)
+        log(json.dumps(rec))
+        out.append(rec)
+    return out
+
+
 def cpu_baseline(ctx, roots, budget_s):
     import oracle
     off, adj = ctx.csr()
@@ -214,9 +292,10 @@ def main():
     ap.add_argument("--cpu-budget", type=float, default=20.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--model", default=MODEL, help="direction network parameters (float32)")
-    ap.add_argument("--leg", choices=("all", "roofline"), default="all",
-                    help="roofline: only the bottom-up roofline searches and the calibration "
-                         "copy (the command profiled with rocprofv3)")
+    ap.add_argument("--leg", choices=("all", "roofline", "batches", "stress"), default="all",
+                    help="roofline: only the roofline searches and the calibration copy (the "
+                         "command profiled with rocprofv3); batches: 64 roots at scale 23 and "
+                         "256 uneven graphs; stress: RMAT scale 26, Erdos-Renyi and lattice")
     ap.add_argument("--summarize-pmc", nargs=2, metavar=("FETCH_CSV", "WRITE_CSV"),
                     help="write the bottom-up traffic file from two rocprofv3 --pmc runs of "
                          "--leg roofline, and exit")
@@ -231,6 +310,19 @@ def main():
 
     import beedrill_amd as bd
     ctx = bd.Context(0)
+    if args.leg in ("batches", "stress"):
+        have_model = os.path.exists(args.model)
+        if have_model:
+            ctx.mlp_set(np.fromfile(args.model, np.float32), 32, 32)
+            meta = os.path.splitext(args.model)[0] + ".json"
+            if os.path.exists(meta):
+                with open(meta) as f:
+                    mm = json.load(f).get("margin_milli")
+                if mm:
+                    ctx.set_tuning(learned_margin_milli=int(mm))
+        res = leg_batches(ctx, args) if args.leg == "batches" else leg_stress(ctx, args, have_model)
+        print(json.dumps({"leg": args.leg, "results": res}))
+        return
     t0 = time.perf_counter()
     m = ctx.generate_rmat(args.scale, args.edge_factor, seed=args.seed)
     gen_s = time.perf_counter() - t0

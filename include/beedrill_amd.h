@@ -231,7 +231,11 @@ typedef struct {
                                 frontier's degree sum * ms_alpha is below
                                 the degree sum of the vertices still open,
                                 else it pulls                             */
-    int32_t reserved[2];
+    int32_t learned_min_skew; /* LEARNED consults the network only on graphs
+                                whose largest degree is at least this many
+                                times the mean degree (the graphs it was
+                                trained on have hubs); otherwise Beamer   */
+    int32_t reserved[1];
 } bd_tuning;
 int bd_set_tuning(bd_ctx *ctx, const bd_tuning *t);
 int bd_get_tuning(bd_ctx *ctx, bd_tuning *t);

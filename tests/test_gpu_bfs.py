@@ -429,3 +429,33 @@ def test_learned_decisions_follow_the_network(ctx, orc, s16):
     finally:
         ctx.set_tuning(**t0)
     assert checked >= 10
+
+
+def test_learned_policy_is_beamer_on_graphs_without_hubs(ctx, orc):
+    """The network only overrides Beamer on graphs whose largest degree is at
+    least learned_min_skew times the mean (it was trained on graphs with
+    hubs).  A network that always prefers bottom-up changes an RMAT search
+    but leaves a lattice search exactly Beamer's."""
+    params = np.zeros(32 * 8 + 32 + 32 * 32 + 32 + 2 * 32 + 2, np.float32)
+    params[-2:] = [10.0, -10.0]  # predicted log2 cost: top-down 2^10, bottom-up 2^-10
+    ctx.mlp_set(params, 32, 32)
+    t0 = ctx.get_tuning()
+    try:
+        ctx.set_tuning(learned_margin_milli=1)
+        ctx.generate_lattice(30, 40, seed=3)
+        off, adj = ctx.csr()
+        for root in (0, 611):
+            ctx.bfs(root, 15, 18, "beamer")
+            beamer = [(r["dir"], r["next"]) for r in ctx.levels()]
+            ctx.bfs(root, 15, 18, "learned")
+            assert [(r["dir"], r["next"]) for r in ctx.levels()] == beamer
+        ctx.generate_rmat(12, 16, seed=5)
+        off, adj = ctx.csr()
+        m = ctx.info()[2]
+        root = int(ctx.pick_roots(1)[0])
+        ctx.bfs(root, 15, 18, "learned")
+        lv = ctx.levels()
+        assert all(r["dir"] == bd.DIR_BU for r in lv)
+        np.testing.assert_array_equal(ctx.fetch()[0], orc.bfs_depth(off, adj, root))
+    finally:
+        ctx.set_tuning(**t0)
