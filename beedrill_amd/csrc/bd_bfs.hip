@@ -54,6 +54,7 @@ namespace {
 
 constexpr int kMaxLevelsPerSync = 64;
 constexpr int kQCap = 2048;  // LDS-staged queue entries per workgroup
+constexpr int64_t kPrecheckEdges = 1 << 16;  // top-down steps this heavy pre-check claims
 
 struct step_args {
     const int64_t *off;
@@ -301,8 +302,13 @@ __device__ __forceinline__ void td_visit(const step_args &a, step_smem<BLOCK> &s
     const uint64_t bit = 1ULL << (u & 63);
     if (valid) {
         uint64_t *wp = a.vis + (u >> 6);
-        const uint64_t seen = __hip_atomic_load((unsigned long long *)wp, __ATOMIC_RELAXED,
-                                                __HIP_MEMORY_SCOPE_AGENT);
+        // heavy steps pre-check the word (a popular vertex's word would
+        // otherwise take an atomic per edge); light steps go straight to it
+        const uint64_t seen =
+            s.st.frontier_deg >= kPrecheckEdges
+                ? __hip_atomic_load((unsigned long long *)wp, __ATOMIC_RELAXED,
+                                    __HIP_MEMORY_SCOPE_AGENT)
+                : 0ULL;
         if (!(seen & bit)) {
             const unsigned long long old =
                 atomicOr((unsigned long long *)wp, (unsigned long long)bit);
@@ -807,9 +813,10 @@ static_assert(kCtlWords <= 3 * 64, "ctl_block is copied as three words per lane"
 // its direction.  Every workgroup of the step runs this on its first wave
 // and gets the same result, so no workgroup waits for another; `publish`
 // (workgroup 0) also writes the step record.  All global loads are issued
-// before any is used, so the rebuild costs one memory round trip.
+// before any is used, so the rebuild costs one memory round trip.  Returns
+// true, having changed nothing, when the search ended before this level.
 template <int BLOCK>
-__device__ void rebuild_state(const step_args &a, step_smem<BLOCK> &s, bool publish) {
+__device__ bool rebuild_state(const step_args &a, step_smem<BLOCK> &s, bool publish) {
     const int L = a.level;
     const int l = lane_id();
     seg_counters *g = a.seg;
@@ -831,6 +838,10 @@ __device__ void rebuild_state(const step_args &a, step_smem<BLOCK> &s, bool publ
         for (int i = 0; i < 3; ++i)
             if (l + 64 * i < kCtlWords) dst[l + 64 * i] = cw[i];
     }
+    wave_lds_sync();
+    // the search ended before this level: leave everything as it is (the
+    // launch only has to see the flag, which is part of the state it loaded)
+    if (s.st.done) return true;
     const unsigned long long nn = wave_sum(ac[0]);
     const unsigned long long nd = wave_sum(ac[1]);
     const unsigned long long ex = wave_sum(ac[2]);
@@ -886,6 +897,7 @@ __device__ void rebuild_state(const step_args &a, step_smem<BLOCK> &s, bool publ
     seg_prefix(tl, to, a.tseg_cap, a.task_cap, c->tpref);
     wave_lds_sync();
     if (s.s_new > 0) choose_dir(a, s, c, s.s_dir, s.s_new, s.s_new_deg, w);
+    return false;
 }
 
 // Workgroup 0, first wave: publish the state before step L as copy L&1 (both
@@ -967,7 +979,6 @@ constexpr int kStepTD = BD_DIR_TD, kStepBU = BD_DIR_BU, kStepBoth = 2;
 template <int BLOCK, int DIRS>
 __global__ __launch_bounds__(BLOCK) void k_step(step_args a) {
     const int L = a.level;
-    if (L > 0 && a.c[(L - 1) & 1].done) return;
     __shared__ step_smem<BLOCK> s;
     if (threadIdx.x < 64) {
         if (L == 0) {
@@ -975,8 +986,10 @@ __global__ __launch_bounds__(BLOCK) void k_step(step_args a) {
             uint64_t *dst = (uint64_t *)&s.st;
             for (int i = threadIdx.x; i < (int)(sizeof(ctl_block) / 8); i += 64) dst[i] = src[i];
         } else {
-            rebuild_state(a, s, blockIdx.x == 0);
-            if (blockIdx.x == 0) publish_state(a, s);
+            // one round trip: the previous state (done flag included) and the
+            // counters are loaded together; a finished search stops there
+            const bool was_done = rebuild_state(a, s, blockIdx.x == 0);
+            if (blockIdx.x == 0 && !was_done) publish_state(a, s);
         }
     }
     if (threadIdx.x == 0) s.qn = 0;
