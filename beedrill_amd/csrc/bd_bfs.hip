@@ -2,7 +2,11 @@
 // hybrid_bfs.cc:152-191).
 //
 // Device state per search (V vertices, W = ceil(V/64) 64-bit words):
-//   depth[V], parent[V]  int32     results (-1 = unreached)
+//   parent[V]            int32     result (-1 = unreached); as in the
+//                                  reference (parent_ only, hybrid_bfs.h:15)
+//                                  the steps write parents alone, and depths
+//                                  are derived from the parent tree on
+//                                  readout (k_tree_*)
 //   vis[W]               bitmap    visited; degree-0 vertices are pre-set
 //   bm[3][W]             bitmaps   bm[L%3] = depth-L set (bottom-up input),
 //                                  bm[(L+1)%3] = set found by step L,
@@ -55,6 +59,7 @@ namespace {
 constexpr int kMaxLevelsPerSync = 64;
 constexpr int kQCap = 2048;  // LDS-staged queue entries per workgroup
 constexpr int64_t kPrecheckEdges = 1 << 16;  // top-down steps this heavy pre-check claims
+constexpr int kBuBatch = 1;  // bottom-up rounds of 64 vertices loaded together
 
 struct step_args {
     const int64_t *off;
@@ -68,7 +73,6 @@ struct step_args {
     int32_t *qout;
     const td_task *tin;
     td_task *tout;
-    int32_t *depth;
     int32_t *parent;
     ctl_block *c;
     seg_counters *seg;
@@ -317,7 +321,6 @@ __device__ __forceinline__ void td_visit(const step_args &a, step_smem<BLOCK> &s
     }
     int64_t b = 0, e = 0;
     if (won) {
-        a.depth[u] = a.level + 1;
         a.parent[u] = src;
         b = a.off[u];
         e = a.off[u + 1];
@@ -503,7 +506,7 @@ __device__ void td_bitmap_body(const step_args &a, step_smem<BLOCK> &s, unsigned
 // a hit records (position << 32 | neighbour) with an LDS atomic minimum, so
 // each vertex keeps its first hit in CSR order, as a sequential scan would.
 // Lists that hold no hit in their first bu_cap neighbours are finished by
-// the whole wave.  Found vertices get depth and parent and their bit in the
+// the whole wave.  Found vertices get their parent and their bit in the
 // wave's found words (chunk word (v >> 6) - w0).
 template <int BLOCK>
 __device__ void bu_pending(const step_args &a, step_smem<BLOCK> &s, const int32_t *pend, int n,
@@ -573,7 +576,6 @@ __device__ void bu_pending(const step_args &a, step_smem<BLOCK> &s, const int32_
         if (lane == src_lane) par = found;
     }
     if (par >= 0) {
-        a.depth[v] = a.level + 1;
         a.parent[v] = par;
         const unsigned long long d = (unsigned long long)(e - b);
         ++cnt;
@@ -629,60 +631,77 @@ __device__ void bu_body(const step_args &a, step_smem<BLOCK> &s, unsigned long l
             const uint64_t bits = (uint64_t)__shfl((long long)unv, j);
             bpos = select_bit(bits, gg - __shfl(excl, j));
         };
-        // Rounds of 64 vertices, software-pipelined: the next round's head
-        // entry is loaded while this round's frontier bit is in flight.
-        int j, bpos;
-        locate(lane, j, bpos);
-        bool valid = lane < total;
-        uint64_t hd =
-            (valid && a.bu_head) ? __builtin_nontemporal_load(a.head + (w0 + j) * 64 + bpos) : 0ULL;
-        for (int r = 0; r < total; r += 64) {
-            const int64_t v = (w0 + j) * 64 + bpos;
-            const int32_t h = (int32_t)(uint32_t)hd;
-            const bool probe = valid && a.bu_head && h >= 0;
-            const uint64_t fw = probe ? a.front[h >> 6] : 0ULL;
-            int jn, bposn;
-            locate(r + 64 + lane, jn, bposn);
-            const bool validn = r + 64 + lane < total;
-            const uint64_t hdn = (validn && a.bu_head)
-                                     ? __builtin_nontemporal_load(a.head + (w0 + jn) * 64 + bposn)
-                                     : 0ULL;
-            bool miss = valid;
-            if (probe) {
-                ++exam;
-                if ((hd >> 32) == 1 && !((fw >> (h & 63)) & 1)) {
-                    // the only neighbour is the head, which missed: the list
-                    // scan would read it again and find nothing
-                    miss = false;
+        // Batches of kBuBatch rounds of 64 vertices.  A round's head entry
+        // is one dependent load away from its frontier word, so one round
+        // at a time leaves a wave with a single 512-byte load in flight;
+        // here the next batch's heads are loaded while this batch's
+        // frontier words are in flight.  vb[k] is the vertex of round k
+        // (-1 past the chunk's last unvisited vertex).
+        auto head_of = [&](int g, int32_t &v, uint64_t &hd) {
+            int j, bpos;
+            locate(g, j, bpos);
+            v = g < total ? (int32_t)((w0 + j) * 64 + bpos) : -1;
+            hd = (v >= 0 && a.bu_head) ? __builtin_nontemporal_load(a.head + v) : 0ULL;
+        };
+        int32_t vb[kBuBatch];
+        uint64_t hb[kBuBatch];
+#pragma unroll
+        for (int k = 0; k < kBuBatch; ++k) head_of(64 * k + lane, vb[k], hb[k]);
+        for (int r0 = 0; r0 < total; r0 += 64 * kBuBatch) {
+            uint64_t fw[kBuBatch];
+#pragma unroll
+            for (int k = 0; k < kBuBatch; ++k) {
+                const int32_t h = (int32_t)(uint32_t)hb[k];
+                fw[k] = (vb[k] >= 0 && a.bu_head && h >= 0) ? a.front[h >> 6] : 0ULL;
+            }
+            int32_t vn[kBuBatch];
+            uint64_t hn[kBuBatch];
+#pragma unroll
+            for (int k = 0; k < kBuBatch; ++k) head_of(r0 + 64 * (kBuBatch + k) + lane, vn[k], hn[k]);
+#pragma unroll
+            for (int k = 0; k < kBuBatch; ++k) {
+                if (r0 + 64 * k >= total) break;  // wave-uniform
+                const int32_t v = vb[k];
+                const uint64_t hd = hb[k];
+                const int32_t h = (int32_t)(uint32_t)hd;
+                bool miss = v >= 0;
+                if (v >= 0 && a.bu_head && h >= 0) {
+                    const bool hit = (fw[k] >> (h & 63)) & 1;
                     ++exam;
-                } else if ((fw >> (h & 63)) & 1) {
-                    miss = false;
-                    a.depth[v] = a.level + 1;
-                    a.parent[v] = h;
-                    const unsigned long long d = hd >> 32;
-                    ++cnt;
-                    dsum += d;
-                    dmax = d > dmax ? d : dmax;
-                    atomicOr((unsigned long long *)&s.b_found[wid][j],
-                             (unsigned long long)(1ULL << bpos));
+                    if ((hd >> 32) == 1 && !hit) {
+                        // the only neighbour is the head, which missed: the
+                        // list scan would read it again and find nothing
+                        miss = false;
+                        ++exam;
+                    } else if (hit) {
+                        miss = false;
+                        a.parent[v] = h;
+                        const unsigned long long d = hd >> 32;
+                        ++cnt;
+                        dsum += d;
+                        dmax = d > dmax ? d : dmax;
+                        atomicOr((unsigned long long *)&s.b_found[wid][(v >> 6) - w0],
+                                 (unsigned long long)(1ULL << (v & 63)));
+                    }
+                }
+                const uint64_t mm = __ballot(miss);
+                if (miss) pend[np + __popcll(mm & below)] = v;
+                np += __popcll(mm);
+                if (np >= 64) {
+                    wave_lds_sync();
+                    bu_pending(a, s, pend, 64, w0, cnt, dsum, dmax, exam);
+                    const int rest = np - 64;
+                    const int32_t t = lane < rest ? pend[64 + lane] : 0;
+                    wave_lds_sync();
+                    if (lane < rest) pend[lane] = t;
+                    np = rest;
+                    wave_lds_sync();
                 }
             }
-            j = jn;
-            bpos = bposn;
-            valid = validn;
-            hd = hdn;
-            const uint64_t mm = __ballot(miss);
-            if (miss) pend[np + __popcll(mm & below)] = (int32_t)v;
-            np += __popcll(mm);
-            if (np >= 64) {
-                wave_lds_sync();
-                bu_pending(a, s, pend, 64, w0, cnt, dsum, dmax, exam);
-                const int rest = np - 64;
-                const int32_t t = lane < rest ? pend[64 + lane] : 0;
-                wave_lds_sync();
-                if (lane < rest) pend[lane] = t;
-                np = rest;
-                wave_lds_sync();
+#pragma unroll
+            for (int k = 0; k < kBuBatch; ++k) {
+                vb[k] = vn[k];
+                hb[k] = hn[k];
             }
         }
         if (np > 0) {
@@ -1056,7 +1075,6 @@ __global__ __launch_bounds__(64) void k_init(step_args a, int64_t source,
             won = !(old & bit);
             if (won) {
                 atomicOr((unsigned long long *)&a.vis[v >> 6], (unsigned long long)bit);
-                a.depth[v] = 0;
                 a.parent[v] = (int32_t)v;
                 b = a.off[v];
                 e = a.off[v + 1];
@@ -1133,15 +1151,12 @@ __global__ __launch_bounds__(64) void k_init(step_args a, int64_t source,
 }
 
 // clear() (hybrid_bfs.cc:433-443) for the device layout.
-__global__ void k_clear(int32_t *__restrict__ depth, int32_t *__restrict__ parent, int64_t nv,
-                        uint64_t *__restrict__ vis, const uint64_t *__restrict__ dead,
-                        uint64_t *__restrict__ b0, uint64_t *__restrict__ b1,
-                        uint64_t *__restrict__ b2, int64_t nwords) {
+__global__ void k_clear(int32_t *__restrict__ parent, int64_t nv, uint64_t *__restrict__ vis,
+                        const uint64_t *__restrict__ dead, uint64_t *__restrict__ b0,
+                        uint64_t *__restrict__ b1, uint64_t *__restrict__ b2, int64_t nwords) {
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nv; i += stride) {
-        depth[i] = -1;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nv; i += stride)
         parent[i] = -1;
-    }
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nwords; i += stride) {
         vis[i] = dead[i];
         b0[i] = 0;
@@ -1165,15 +1180,74 @@ __global__ __launch_bounds__(64) void k_reset_step(seg_counters *g, int32_t leve
 
 // count_num_traversed_edges (hybrid_bfs.cc:401-414): sum of degrees over
 // reached vertices (one atomic per wave, spread over kSlots lines).
-__global__ void k_count_traversed(const int32_t *__restrict__ depth,
+__global__ void k_count_traversed(const int32_t *__restrict__ parent,
                                   const int64_t *__restrict__ off, int64_t nv,
                                   unsigned long long *__restrict__ out) {
     unsigned long long s = 0;
     for (int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v < nv;
          v += (int64_t)gridDim.x * blockDim.x)
-        if (depth[v] >= 0) s += (unsigned long long)(off[v + 1] - off[v]);
+        if (parent[v] >= 0) s += (unsigned long long)(off[v + 1] - off[v]);
     s = wave_sum(s);
     if ((threadIdx.x & 63) == 0 && s) atomicAdd(&out[(blockIdx.x % kSlots) * kLine], s);
+}
+
+// Depths from the parent tree.  The steps write parents only, as
+// hybrid_bfs::run_beamer does; a vertex's depth is its distance to the
+// source its parent chain ends at (a source is its own parent).
+// t[v] = (k << 32) | x names an ancestor x of v, k levels up, starting from
+// the parent (k = 1, or k = 0 for a source).  Each round of pointer jumping
+// replaces (x, k) by (y, k + j) where t[x] = (j << 32) | y, until every x is
+// a source.  Entries are single 8-byte words, read and written whole, so a
+// jump that reads a word another thread has already advanced in this round
+// still gets an ancestor and its distance; such reads only shorten the
+// rounds, which are no more than with separate buffers (log2 of the depth).
+// err[1] is set when a parent is itself unreached, err[2] when a chain
+// closes on a vertex that is not a source.
+constexpr uint64_t kNoTree = ~0ULL;
+
+__global__ void k_tree_init(const int32_t *__restrict__ parent, uint64_t *__restrict__ t,
+                            int64_t nv) {
+    for (int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v < nv;
+         v += (int64_t)gridDim.x * blockDim.x) {
+        const int32_t p = parent[v];
+        t[v] = p < 0 ? kNoTree : ((uint64_t)(p == v ? 0u : 1u) << 32) | (uint32_t)p;
+    }
+}
+
+__global__ void k_tree_jump(uint64_t *t, int64_t nv, int32_t *__restrict__ err) {
+    for (int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v < nv;
+         v += (int64_t)gridDim.x * blockDim.x) {
+        const uint64_t x = __hip_atomic_load(t + v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        if (x == kNoTree) continue;
+        const uint32_t a = (uint32_t)x;
+        if ((int64_t)a >= nv) {
+            err[1] = 1;
+            continue;
+        }
+        const uint64_t y = __hip_atomic_load(t + a, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        if (y == kNoTree) {
+            err[1] = 1;
+            continue;
+        }
+        if ((uint32_t)y == a) continue;  // a is a source (or closes a cycle)
+        __hip_atomic_store(t + v, (((x >> 32) + (y >> 32)) << 32) | (uint32_t)y, __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_WORKGROUP);
+        err[0] = 1;  // a jump was made: another round
+    }
+}
+
+__global__ void k_tree_depth(const uint64_t *__restrict__ t, int32_t *__restrict__ depth,
+                             int64_t nv, int32_t *__restrict__ err) {
+    for (int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v < nv;
+         v += (int64_t)gridDim.x * blockDim.x) {
+        const uint64_t x = t[v];
+        if (x == kNoTree) {
+            depth[v] = -1;
+            continue;
+        }
+        if (t[(uint32_t)x] != (uint64_t)(uint32_t)x) err[2] = 1;  // ends at a non-source
+        depth[v] = (int32_t)(x >> 32);
+    }
 }
 
 // Head index of the bottom-up step: head[v] = (deg(v) << 32) | the
@@ -1269,7 +1343,6 @@ step_args make_args(bd_ctx *c, int32_t level, int64_t alpha, int64_t beta, int p
     a.qout = c->d_q[(level + 1) & 1];
     a.tin = c->d_tasks[level & 1];
     a.tout = c->d_tasks[(level + 1) & 1];
-    a.depth = c->d_depth;
     a.parent = c->d_parent;
     a.c = c->d_ctl;
     a.seg = c->d_seg;
@@ -1415,9 +1488,9 @@ int check_run_args(bd_ctx *c, int64_t source, int64_t alpha, int64_t beta, int p
 int start_search(bd_ctx *c, int64_t source, int64_t alpha, int64_t beta, int policy,
                  const int64_t *srcs = nullptr, int64_t nsrc = 1, hipEvent_t init_end = nullptr) {
     const int64_t n = std::max(c->nv, c->nwords);
-    hipLaunchKernelGGL(k_clear, dim3(grid_for(c, n, 256)), dim3(256), 0, c->stream, c->d_depth,
-                       c->d_parent, c->nv, c->d_vis, c->d_dead, c->d_bm[0], c->d_bm[1],
-                       c->d_bm[2], c->nwords);
+    hipLaunchKernelGGL(k_clear, dim3(grid_for(c, n, 256)), dim3(256), 0, c->stream, c->d_parent,
+                       c->nv, c->d_vis, c->d_dead, c->d_bm[0], c->d_bm[1], c->d_bm[2], c->nwords);
+    c->depth_valid = false;
     BD_HIP(c, hipGetLastError());
     // frontier 0: bitmap bm[0], queue q[0], tasks tasks[0]
     step_args a = make_args(c, 0, alpha, beta, policy);
@@ -1608,12 +1681,52 @@ extern "C" int bd_bfs_set_script(bd_ctx *c, const int32_t *dirs, int64_t n) {
     return BD_OK;
 }
 
+// Depths of the last search into d_depth, from its parent tree (k_tree_*).
+int derive_depth(bd_ctx *c) {
+    if (c->depth_valid) return BD_OK;
+    const int64_t nv = c->nv;
+    uint64_t *t = nullptr;
+    // nv tree words, then err[4]
+    int rc = dev_alloc(c, (void **)&t, sizeof(uint64_t) * (nv + 2), "parent tree");
+    if (rc) return rc;
+    struct guard {
+        uint64_t *p;
+        ~guard() { dev_free(p); }
+    } g{t};
+    int32_t *err = (int32_t *)(t + nv);
+    const dim3 grid(grid_for(c, nv, 256)), block(256);
+    hipLaunchKernelGGL(k_tree_init, grid, block, 0, c->stream, c->d_parent, t, nv);
+    BD_HIP(c, hipGetLastError());
+    int32_t h[4] = {1, 0, 0, 0};
+    // a chain of length D takes ceil(log2 D) + 1 rounds; ids are 31 bits
+    for (int r = 0; r < 40 && h[0]; ++r) {
+        BD_HIP(c, hipMemsetAsync(err, 0, sizeof(h), c->stream));
+        hipLaunchKernelGGL(k_tree_jump, grid, block, 0, c->stream, t, nv, err);
+        BD_HIP(c, hipGetLastError());
+        BD_HIP(c, hipMemcpyAsync(h, err, sizeof(h), hipMemcpyDeviceToHost, c->stream));
+        BD_HIP(c, hipStreamSynchronize(c->stream));
+        if (h[1])
+            return set_err(c, BD_EHIP, "parent tree: a reached vertex has an unreached parent");
+    }
+    if (h[0]) return set_err(c, BD_EHIP, "parent tree: chains do not end");
+    hipLaunchKernelGGL(k_tree_depth, grid, block, 0, c->stream, t, c->d_depth, nv, err);
+    BD_HIP(c, hipGetLastError());
+    BD_HIP(c, hipMemcpyAsync(h, err, sizeof(h), hipMemcpyDeviceToHost, c->stream));
+    BD_HIP(c, hipStreamSynchronize(c->stream));
+    if (h[2]) return set_err(c, BD_EHIP, "parent tree: a chain closes on a non-source");
+    c->depth_valid = true;
+    return BD_OK;
+}
+
 extern "C" int bd_bfs_fetch(bd_ctx *c, int32_t *depth, int64_t *parent) {
     if (!c) return BD_EINVAL;
     if (!c->have_result) return set_err(c, BD_ESTATE, "no search has run");
     BD_HIP(c, hipSetDevice(c->device));
-    if (depth)
+    if (depth) {
+        int rc = derive_depth(c);
+        if (rc) return rc;
         BD_HIP(c, hipMemcpy(depth, c->d_depth, sizeof(int32_t) * c->nv, hipMemcpyDeviceToHost));
+    }
     if (parent) {
         std::vector<int32_t> p(c->nv);
         BD_HIP(c, hipMemcpy(p.data(), c->d_parent, sizeof(int32_t) * c->nv, hipMemcpyDeviceToHost));
@@ -1642,7 +1755,7 @@ extern "C" int bd_bfs_count_traversed(bd_ctx *c, int64_t *out) {
     hipError_t e = hipMemsetAsync(d, 0, bytes, c->stream);
     if (e == hipSuccess) {
         hipLaunchKernelGGL(k_count_traversed, dim3(grid_for(c, c->nv, 256)), dim3(256), 0,
-                           c->stream, c->d_depth, c->d_off, c->nv, d);
+                           c->stream, c->d_parent, c->d_off, c->nv, d);
         e = hipGetLastError();
     }
     if (e == hipSuccess) e = hipMemcpyAsync(h.data(), d, bytes, hipMemcpyDeviceToHost, c->stream);

@@ -134,6 +134,7 @@ struct bd_ctx {
 
     // results of the last run
     bool have_result = false;
+    bool depth_valid = false;  // d_depth holds the last search's depths
     int64_t last_source = -1;
     int64_t last_steps = 0;
     std::vector<bd_level_rec> last_log;

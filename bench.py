@@ -65,20 +65,20 @@ def bu_bytes(rec, nv, n_isolated):
     """Algorithmic bytes of one bottom-up step: visited-bitmap scan (V/8),
     both offsets of every unvisited vertex with edges (16 U), the neighbour ids
     read up to the first hit (4 E), one frontier bit per id read (E/8),
-    depth and parent of each newly reached vertex (8 N) and the next-frontier
+    the parent of each newly reached vertex (4 N) and the next-frontier
     bitmap (V/8)."""
     U = rec["unvisited"] - n_isolated
     E = rec["edges_examined"]
-    return nv / 8 + 16 * U + 4 * E + E / 8 + 8 * rec["next"] + nv / 8
+    return nv / 8 + 16 * U + 4 * E + E / 8 + 4 * rec["next"] + nv / 8
 
 
 def td_bytes(rec):
     """Algorithmic bytes of one top-down step: frontier ids (4 |F|), their
     offsets (16 |F|), the neighbour ids (4 sum deg F), one visited probe per
-    neighbour (4 sum deg F), and depth, parent and queue entry of each newly
-    reached vertex (12 |N|)."""
+    neighbour (4 sum deg F), and parent and queue entry of each newly
+    reached vertex (8 |N|)."""
     F, D = rec["frontier"], rec["frontier_deg"]
-    return 4 * F + 16 * F + 4 * D + 4 * D + 12 * rec["next"]
+    return 4 * F + 16 * F + 4 * D + 4 * D + 8 * rec["next"]
 
 
 def roofline(ctx, roots, policy, nv, n_isolated):

@@ -92,6 +92,21 @@ def test_small_graphs_match_oracle(ctx, orc, g, root):
     assert ctx.check(root)
 
 
+def test_depths_from_deep_parent_chains(ctx, orc):
+    # The steps write parents only and depths come from the parent tree on
+    # readout: a 20000-vertex path under a random relabelling has parent
+    # chains of up to 19999 links running both ways through the id space.
+    n = 20000
+    perm = np.random.default_rng(7).permutation(n).astype(np.int64)
+    s, d = perm[:-1], perm[1:]
+    ctx.from_edges(n, s, d)
+    off, adj, m = oracle_csr(orc, n, s, d)
+    for root in (int(perm[0]), int(perm[n // 3])):
+        run_and_compare(ctx, orc, off, adj, m, root)
+        depth, _ = ctx.fetch()
+        assert depth.max() == n - 1 - (0 if root == perm[0] else n // 3)
+
+
 def test_isolated_source_and_self_loops(ctx, orc):
     # vertex 5 isolated; a self-loop on 2 and a duplicate edge stay as given
     nv = 6
