@@ -24,7 +24,8 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libbeedrill_amd.so")
 
 POLICY_BEAMER, POLICY_LEARNED, POLICY_TD_ONLY, POLICY_BU_ONLY, POLICY_SCRIPT = range(5)
-POLICIES = {"beamer": 0, "learned": 1, "td_only": 2, "bu_only": 3, "script": 4}
+POLICIES = {"beamer": 0, "learned": 1, "td_only": 2, "bu_only": 3, "script": 4,
+            "remote_writes": 5, "remote_writes_hybrid": 6}
 DIR_TD, DIR_BU = 0, 1
 GEN_RMAT, GEN_ER, GEN_LATTICE = 0, 1, 2
 STATUS = {0: "OK", -1: "EINVAL", -2: "ENOMEM", -3: "EHIP", -4: "ESTATE", -5: "EIO",
@@ -40,7 +41,8 @@ class LevelRec(ctypes.Structure):
         "level", "dir", "frontier", "frontier_deg", "unvisited", "unvisited_deg",
         "edges_to_check", "result", "next", "next_deg", "edges_examined",
         "max_frontier_deg", "tasks", "queued")] + [
-        ("ms", ctypes.c_double), ("ms_other", ctypes.c_double)]
+        ("ms", ctypes.c_double), ("ms_other", ctypes.c_double),
+        ("launch", ctypes.c_int64), ("launches", ctypes.c_int64)]
 
 
 class RunStats(ctypes.Structure):
@@ -54,7 +56,8 @@ class RunStats(ctypes.Structure):
 
 TUNING_FIELDS = ("block", "grid_per_cu", "wave_threshold", "task_threshold", "task_edges",
                  "bu_words", "bu_cap", "level_driver", "time_steps", "bu_head", "step_kernels",
-                 "bu_grid_per_cu", "learned_margin_milli", "ms_alpha", "learned_min_skew")
+                 "bu_grid_per_cu", "learned_margin_milli", "ms_alpha", "learned_min_skew",
+                 "td_split_edges")
 
 
 class Tuning(ctypes.Structure):

@@ -59,7 +59,21 @@ namespace {
 constexpr int kMaxLevelsPerSync = 64;
 constexpr int kQCap = 2048;  // LDS-staged queue entries per workgroup
 constexpr int64_t kPrecheckEdges = 1 << 16;  // top-down steps this heavy pre-check claims
-constexpr int kBuBatch = 1;  // bottom-up rounds of 64 vertices loaded together
+
+// A top-down step run with remote writes (hybrid_bfs.cc:41-67) takes two
+// launches: kDirTDWrite writes each frontier edge's source as the candidate
+// parent of its unvisited end (plain stores, no claims), kDirTDClaim sweeps
+// the visited words and gives every unvisited vertex with a candidate that
+// parent.  Records, counts and the direction policy see one top-down step.
+constexpr int32_t kDirTDWrite = 4, kDirTDClaim = 5;
+// the step a direction belongs to
+__host__ __device__ constexpr int32_t step_dir(int32_t d) {
+    return (d == kDirTDWrite || d == kDirTDClaim) ? BD_DIR_TD : d;
+}
+// the kernel that runs it: the claim sweep is bitmap work, like bottom-up
+__host__ __device__ constexpr int32_t kernel_dir(int32_t d) {
+    return d == kDirTDWrite ? BD_DIR_TD : (d == kDirTDClaim ? BD_DIR_BU : d);
+}
 
 struct step_args {
     const int64_t *off;
@@ -74,6 +88,7 @@ struct step_args {
     const td_task *tin;
     td_task *tout;
     int32_t *parent;
+    int32_t *cand;         // candidate parents (-1 = none)
     ctl_block *c;
     seg_counters *seg;
     bd_level_rec *log;
@@ -85,6 +100,7 @@ struct step_args {
     int64_t qseg_cap, tseg_cap;
     int64_t nv, m, nnz;
     int64_t alpha, beta;
+    int64_t td_split;      // frontier edges from which top-down writes remotely
     int32_t level;
     int32_t close_only;    // only publish the state before this step
     int32_t force_dir;     // -1: ctl->dir; 0/1: run this direction (profiling)
@@ -116,6 +132,7 @@ struct step_smem {
     ctl_block st;  // the search state before this step
     long long s_new, s_new_deg;
     int s_dir;
+    int s_split;  // the step closed was a top-down step with remote writes
     int use_mlp;
     float x[8];
     unsigned long long red[4][BLOCK / 64];
@@ -297,11 +314,33 @@ __device__ void flush_queue(const step_args &a, step_smem<BLOCK> &s) {
 // Visit edge (src -> u).  Wave-collective.  The visited word is first read
 // with a device-scope load, which bypasses the CU's L1: a stale L1 copy would
 // send every later edge into a popular vertex to the atomic, and atomics on
-// one word serialize (~12 ns each).
+// one word serialize (~12 ns each).  With tdw (the write half of a top-down
+// step with remote writes) the edge only writes a candidate parent.
 template <int BLOCK>
-__device__ __forceinline__ void td_visit(const step_args &a, step_smem<BLOCK> &s, bool valid, int32_t u,
+__device__ __forceinline__ void td_visit(const step_args &a, step_smem<BLOCK> &s, bool tdw, bool valid, int32_t u,
                                          int32_t src, unsigned long long &cnt,
                                          unsigned long long &dsum, unsigned long long &dmax) {
+    if (tdw) {
+        // remote write (hybrid_bfs.cc:47-54): a plain store of the source as
+        // the candidate parent of an unvisited end; nothing sets visited
+        // bits in this launch, so the cached word is current
+        // The next-frontier bitmap, zero and otherwise unused during this
+        // launch, marks the ends already written: a plain read-modify-write
+        // that each L2 keeps for its own XCD, so most repeated ends cost no
+        // second scattered store.  A lost or unseen mark only costs a
+        // repeated store of another valid candidate; the bitmap is cleared
+        // before it next holds a frontier.
+        if (valid && !((a.vis[u >> 6] >> (u & 63)) & 1)) {
+            uint64_t *mw = a.next_bm + (u >> 6);
+            const uint64_t w = *mw;
+            const uint64_t bit = 1ULL << (u & 63);
+            if (!(w & bit)) {
+                a.cand[u] = src;
+                *mw = w | bit;
+            }
+        }
+        return;
+    }
     bool won = false;
     const uint64_t bit = 1ULL << (u & 63);
     if (valid) {
@@ -337,7 +376,7 @@ __device__ __forceinline__ void td_visit(const step_args &a, step_smem<BLOCK> &s
 // degree >= wave_t get a wave each; the edges of the others are concatenated
 // by a block-wide scan and every thread takes one.  Whole workgroup.
 template <int BLOCK>
-__device__ void td_tile(const step_args &a, step_smem<BLOCK> &s, int32_t v,
+__device__ void td_tile(const step_args &a, step_smem<BLOCK> &s, bool tdw, int32_t v,
                         unsigned long long &cnt, unsigned long long &dsum,
                         unsigned long long &dmax) {
     const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
@@ -382,7 +421,7 @@ __device__ void td_tile(const step_args &a, step_smem<BLOCK> &s, int32_t v,
             const int64_t k = k0 + lane;
             const bool valid = k < me;
             const int32_t u = valid ? a.cols[k] : 0;
-            td_visit(a, s, valid, u, src, cnt, dsum, dmax);
+            td_visit(a, s, tdw, valid, u, src, cnt, dsum, dmax);
         }
     }
     for (int32_t i0 = 0; i0 < total; i0 += BLOCK) {
@@ -401,7 +440,7 @@ __device__ void td_tile(const step_args &a, step_smem<BLOCK> &s, int32_t v,
             src = s.t_src[lo];
             u = a.cols[s.t_beg[lo] + (i - s.t_pref[lo])];
         }
-        td_visit(a, s, valid, u, src, cnt, dsum, dmax);
+        td_visit(a, s, tdw, valid, u, src, cnt, dsum, dmax);
     }
     __syncthreads();
     if (s.qn >= kQCap / 2) flush_queue<BLOCK>(a, s);
@@ -410,7 +449,7 @@ __device__ void td_tile(const step_args &a, step_smem<BLOCK> &s, int32_t v,
 // Task slices of the frontier's high-degree vertices, one wave each, in
 // workgroup-synchronous rounds.
 template <int BLOCK>
-__device__ void td_tasks(const step_args &a, step_smem<BLOCK> &s, unsigned long long &cnt,
+__device__ void td_tasks(const step_args &a, step_smem<BLOCK> &s, bool tdw, unsigned long long &cnt,
                          unsigned long long &dsum, unsigned long long &dmax) {
     const int lane = lane_id(), wid = threadIdx.x >> 6;
     const int64_t ntasks = s.st.tpref[kSeg + 1];
@@ -424,7 +463,7 @@ __device__ void td_tasks(const step_args &a, step_smem<BLOCK> &s, unsigned long 
                 const int64_t k = k0 + lane;
                 const bool valid = k < tk.end;
                 const int32_t u = valid ? a.cols[k] : 0;
-                td_visit(a, s, valid, u, tk.src, cnt, dsum, dmax);
+                td_visit(a, s, tdw, valid, u, tk.src, cnt, dsum, dmax);
             }
         }
         __syncthreads();
@@ -434,7 +473,7 @@ __device__ void td_tasks(const step_args &a, step_smem<BLOCK> &s, unsigned long 
 
 // Top-down step over the frontier lists (queue tiles, then task slices).
 template <int BLOCK>
-__device__ void td_body(const step_args &a, step_smem<BLOCK> &s, unsigned long long &cnt,
+__device__ void td_body(const step_args &a, step_smem<BLOCK> &s, bool tdw, unsigned long long &cnt,
                         unsigned long long &dsum, unsigned long long &dmax) {
     const int64_t nitems = s.st.qpref[kSeg + 1];
     for (int64_t tile = blockIdx.x; tile * BLOCK < nitems; tile += gridDim.x) {
@@ -444,9 +483,9 @@ __device__ void td_body(const step_args &a, step_smem<BLOCK> &s, unsigned long l
             const int sg = seg_of(s.st.qpref, slot);
             v = a.qin[(int64_t)sg * a.qseg_cap + (slot - s.st.qpref[sg])];
         }
-        td_tile<BLOCK>(a, s, v, cnt, dsum, dmax);
+        td_tile<BLOCK>(a, s, tdw, v, cnt, dsum, dmax);
     }
-    td_tasks<BLOCK>(a, s, cnt, dsum, dmax);
+    td_tasks<BLOCK>(a, s, tdw, cnt, dsum, dmax);
 }
 
 // Top-down step whose frontier is only in the depth-L bitmap (the previous
@@ -454,7 +493,7 @@ __device__ void td_body(const step_args &a, step_smem<BLOCK> &s, unsigned long l
 // ranges of BLOCK frontier words, ranks their set bits with a block-wide
 // scan, and feeds them to td_tile BLOCK vertices at a time.
 template <int BLOCK>
-__device__ void td_bitmap_body(const step_args &a, step_smem<BLOCK> &s, unsigned long long &cnt,
+__device__ void td_bitmap_body(const step_args &a, step_smem<BLOCK> &s, bool tdw, unsigned long long &cnt,
                                unsigned long long &dsum, unsigned long long &dmax) {
     const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
     const int64_t nranges = (a.nwords + BLOCK - 1) / BLOCK;
@@ -492,7 +531,7 @@ __device__ void td_bitmap_body(const step_args &a, step_smem<BLOCK> &s, unsigned
                 v = (int32_t)((r * BLOCK + lo) * 64 +
                               select_bit(s.w_bits[lo], i - s.w_pref[lo]));
             }
-            td_tile<BLOCK>(a, s, v, cnt, dsum, dmax);
+            td_tile<BLOCK>(a, s, tdw, v, cnt, dsum, dmax);
         }
         __syncthreads();
     }
@@ -587,6 +626,76 @@ __device__ void bu_pending(const step_args &a, step_smem<BLOCK> &s, const int32_
     wave_lds_sync();  // hit[] is rewritten by the next batch
 }
 
+// Rank g of a chunk's unvisited vertices -> word j, bit bpos.  Lane i holds
+// word i's unvisited bits `unv` and their exclusive rank prefix `excl`;
+// `total` is the chunk's count (ranks past it give the last vertex).
+__device__ __forceinline__ void rank_locate(int g, int total, int excl, uint64_t unv, int &j,
+                                            int &bpos) {
+    const int gg = g < total ? g : total - 1;
+    j = 0;
+#pragma unroll
+    for (int st = 32; st > 0; st >>= 1) {
+        const int mid = j + st;
+        const int em = __shfl(excl, mid < 64 ? mid : 63);
+        if (mid < 64 && em <= gg) j = mid;
+    }
+    const uint64_t bits = (uint64_t)__shfl((long long)unv, j);
+    bpos = select_bit(bits, gg - __shfl(excl, j));
+}
+
+// Claim half of a top-down step run with remote writes (hybrid_bfs.cc:56-66):
+// every unvisited vertex with a candidate takes it as its parent and joins
+// the next frontier, which is left in the bitmap only, as after a bottom-up
+// step.  Waves own chunks of bu_words visited words and rank their unvisited
+// vertices as the bottom-up step does; each consumed candidate is reset, so
+// the array is all -1 again when the step ends.
+template <int BLOCK>
+__device__ void td_claim_body(const step_args &a, step_smem<BLOCK> &s, unsigned long long &cnt,
+                              unsigned long long &dsum, unsigned long long &dmax) {
+    const int lane = lane_id(), wid = threadIdx.x >> 6;
+    const int CW = a.bu_words;
+    const int64_t nchunks = (a.nwords + CW - 1) / CW;
+    for (int64_t chunk = (int64_t)blockIdx.x * (BLOCK / 64) + wid; chunk < nchunks;
+         chunk += (int64_t)gridDim.x * (BLOCK / 64)) {
+        const int64_t w0 = chunk * CW;
+        const int64_t wi = w0 + lane;
+        const bool own = lane < CW && wi < a.nwords;
+        const uint64_t vw = own ? a.vis[wi] : ~0ULL;
+        if (own) a.clr[wi] = 0;
+        const uint64_t unv = ~vw;
+        const int cntw = __popcll(unv);
+        int excl = (int)wave_incl_scan(cntw);
+        const int total = __shfl(excl, 63);
+        excl -= cntw;
+        s.b_found[wid][lane] = 0;
+        wave_lds_sync();
+        for (int r = 0; r < total; r += 64) {
+            int j, bpos;
+            rank_locate(r + lane, total, excl, unv, j, bpos);
+            const int64_t v = (w0 + j) * 64 + bpos;
+            const int32_t p = r + lane < total ? __builtin_nontemporal_load(a.cand + v) : -1;
+            if (p >= 0) {
+                a.cand[v] = -1;
+                a.parent[v] = p;
+                const unsigned long long d =
+                    a.bu_head ? (unsigned long long)(a.head[v] >> 32)
+                              : (unsigned long long)(a.off[v + 1] - a.off[v]);
+                ++cnt;
+                dsum += d;
+                dmax = d > dmax ? d : dmax;
+                atomicOr((unsigned long long *)&s.b_found[wid][j], (unsigned long long)(1ULL << bpos));
+            }
+        }
+        wave_lds_sync();
+        if (own) {
+            const uint64_t f = s.b_found[wid][lane];
+            a.next_bm[wi] = f;
+            if (f) a.vis[wi] = vw | f;
+        }
+        wave_lds_sync();
+    }
+}
+
 // Bottom-up step.  Each wave owns chunks of bu_words visited words and ranks
 // their unvisited vertices with a wave prefix sum; per round of 64 vertices
 // each lane reads its vertex's head index (the neighbour most likely to be
@@ -618,90 +727,60 @@ __device__ void bu_body(const step_args &a, step_smem<BLOCK> &s, unsigned long l
         s.b_found[wid][lane] = 0;
         int np = 0;  // pending vertices (wave-uniform)
         wave_lds_sync();
-        // rank g of the chunk's unvisited vertices -> word j, bit bpos
-        auto locate = [&](int g, int &j, int &bpos) {
-            const int gg = g < total ? g : total - 1;
-            j = 0;
-#pragma unroll
-            for (int st = 32; st > 0; st >>= 1) {
-                const int mid = j + st;
-                const int em = __shfl(excl, mid < 64 ? mid : 63);
-                if (mid < 64 && em <= gg) j = mid;
-            }
-            const uint64_t bits = (uint64_t)__shfl((long long)unv, j);
-            bpos = select_bit(bits, gg - __shfl(excl, j));
-        };
-        // Batches of kBuBatch rounds of 64 vertices.  A round's head entry
-        // is one dependent load away from its frontier word, so one round
-        // at a time leaves a wave with a single 512-byte load in flight;
-        // here the next batch's heads are loaded while this batch's
-        // frontier words are in flight.  vb[k] is the vertex of round k
-        // (-1 past the chunk's last unvisited vertex).
-        auto head_of = [&](int g, int32_t &v, uint64_t &hd) {
-            int j, bpos;
-            locate(g, j, bpos);
-            v = g < total ? (int32_t)((w0 + j) * 64 + bpos) : -1;
-            hd = (v >= 0 && a.bu_head) ? __builtin_nontemporal_load(a.head + v) : 0ULL;
-        };
-        int32_t vb[kBuBatch];
-        uint64_t hb[kBuBatch];
-#pragma unroll
-        for (int k = 0; k < kBuBatch; ++k) head_of(64 * k + lane, vb[k], hb[k]);
-        for (int r0 = 0; r0 < total; r0 += 64 * kBuBatch) {
-            uint64_t fw[kBuBatch];
-#pragma unroll
-            for (int k = 0; k < kBuBatch; ++k) {
-                const int32_t h = (int32_t)(uint32_t)hb[k];
-                fw[k] = (vb[k] >= 0 && a.bu_head && h >= 0) ? a.front[h >> 6] : 0ULL;
-            }
-            int32_t vn[kBuBatch];
-            uint64_t hn[kBuBatch];
-#pragma unroll
-            for (int k = 0; k < kBuBatch; ++k) head_of(r0 + 64 * (kBuBatch + k) + lane, vn[k], hn[k]);
-#pragma unroll
-            for (int k = 0; k < kBuBatch; ++k) {
-                if (r0 + 64 * k >= total) break;  // wave-uniform
-                const int32_t v = vb[k];
-                const uint64_t hd = hb[k];
-                const int32_t h = (int32_t)(uint32_t)hd;
-                bool miss = v >= 0;
-                if (v >= 0 && a.bu_head && h >= 0) {
-                    const bool hit = (fw[k] >> (h & 63)) & 1;
+        auto locate = [&](int g, int &j, int &bpos) { rank_locate(g, total, excl, unv, j, bpos); };
+        // Rounds of 64 vertices, software-pipelined: the next round's head
+        // entry is loaded while this round's frontier word is in flight.
+        int j, bpos;
+        locate(lane, j, bpos);
+        bool valid = lane < total;
+        uint64_t hd =
+            (valid && a.bu_head) ? __builtin_nontemporal_load(a.head + (w0 + j) * 64 + bpos) : 0ULL;
+        for (int r = 0; r < total; r += 64) {
+            const int64_t v = (w0 + j) * 64 + bpos;
+            const int32_t h = (int32_t)(uint32_t)hd;
+            const bool probe = valid && a.bu_head && h >= 0;
+            const uint64_t fw = probe ? a.front[h >> 6] : 0ULL;
+            int jn, bposn;
+            locate(r + 64 + lane, jn, bposn);
+            const bool validn = r + 64 + lane < total;
+            const uint64_t hdn = (validn && a.bu_head)
+                                     ? __builtin_nontemporal_load(a.head + (w0 + jn) * 64 + bposn)
+                                     : 0ULL;
+            bool miss = valid;
+            if (probe) {
+                ++exam;
+                if ((hd >> 32) == 1 && !((fw >> (h & 63)) & 1)) {
+                    // the only neighbour is the head, which missed: the list
+                    // scan would read it again and find nothing
+                    miss = false;
                     ++exam;
-                    if ((hd >> 32) == 1 && !hit) {
-                        // the only neighbour is the head, which missed: the
-                        // list scan would read it again and find nothing
-                        miss = false;
-                        ++exam;
-                    } else if (hit) {
-                        miss = false;
-                        a.parent[v] = h;
-                        const unsigned long long d = hd >> 32;
-                        ++cnt;
-                        dsum += d;
-                        dmax = d > dmax ? d : dmax;
-                        atomicOr((unsigned long long *)&s.b_found[wid][(v >> 6) - w0],
-                                 (unsigned long long)(1ULL << (v & 63)));
-                    }
-                }
-                const uint64_t mm = __ballot(miss);
-                if (miss) pend[np + __popcll(mm & below)] = v;
-                np += __popcll(mm);
-                if (np >= 64) {
-                    wave_lds_sync();
-                    bu_pending(a, s, pend, 64, w0, cnt, dsum, dmax, exam);
-                    const int rest = np - 64;
-                    const int32_t t = lane < rest ? pend[64 + lane] : 0;
-                    wave_lds_sync();
-                    if (lane < rest) pend[lane] = t;
-                    np = rest;
-                    wave_lds_sync();
+                } else if ((fw >> (h & 63)) & 1) {
+                    miss = false;
+                    a.parent[v] = h;
+                    const unsigned long long d = hd >> 32;
+                    ++cnt;
+                    dsum += d;
+                    dmax = d > dmax ? d : dmax;
+                    atomicOr((unsigned long long *)&s.b_found[wid][j],
+                             (unsigned long long)(1ULL << bpos));
                 }
             }
-#pragma unroll
-            for (int k = 0; k < kBuBatch; ++k) {
-                vb[k] = vn[k];
-                hb[k] = hn[k];
+            j = jn;
+            bpos = bposn;
+            valid = validn;
+            hd = hdn;
+            const uint64_t mm = __ballot(miss);
+            if (miss) pend[np + __popcll(mm & below)] = (int32_t)v;
+            np += __popcll(mm);
+            if (np >= 64) {
+                wave_lds_sync();
+                bu_pending(a, s, pend, 64, w0, cnt, dsum, dmax, exam);
+                const int rest = np - 64;
+                const int32_t t = lane < rest ? pend[64 + lane] : 0;
+                wave_lds_sync();
+                if (lane < rest) pend[lane] = t;
+                np = rest;
+                wave_lds_sync();
             }
         }
         if (np > 0) {
@@ -762,6 +841,14 @@ __device__ void choose_dir(const step_args &a, step_smem<BLOCK> &s, ctl_block *c
         case BD_POLICY_BU_ONLY:
             c->dir = BD_DIR_BU;
             break;
+        case BD_POLICY_REMOTE_WRITES:
+            c->dir = kDirTDWrite;
+            break;
+        case BD_POLICY_REMOTE_WRITES_HYBRID:
+            // Beamer's state machine; its bottom-up phase writes remotely
+            beamer_next(c, last_dir, nnew, nnew_deg);
+            if (c->dir == BD_DIR_BU) c->dir = kDirTDWrite;
+            break;
         case BD_POLICY_SCRIPT: {
             const int32_t i = c->level < a.script_len ? c->level : a.script_len - 1;
             c->dir = (a.script_len > 0 && a.script[i] == BD_DIR_BU) ? BD_DIR_BU : BD_DIR_TD;
@@ -806,6 +893,10 @@ __device__ void choose_dir(const step_args &a, step_smem<BLOCK> &s, ctl_block *c
             }
         }
     }
+    if (lane_id() == 0 && c->dir == BD_DIR_TD && a.policy != BD_POLICY_TD_ONLY &&
+        a.policy != BD_POLICY_REMOTE_WRITES_HYBRID && a.td_split > 0 &&
+        c->frontier_deg >= a.td_split)
+        c->dir = kDirTDWrite;
     wave_lds_sync();
 }
 
@@ -868,54 +959,77 @@ __device__ bool rebuild_state(const step_args &a, step_smem<BLOCK> &s, bool publ
     wave_lds_sync();
     ctl_block *c = &s.st;
     if (l == 0) {
-        const int64_t nnew = (int64_t)nn;
-        const int64_t nnew_deg = (int64_t)nd;
         const int32_t dir = c->dir;
-        if (publish && L - 1 < a.log_cap) {
-            bd_level_rec r;
-            r.level = L - 1;
-            r.dir = dir;
-            r.frontier = c->frontier;
-            r.frontier_deg = c->frontier_deg;
-            r.unvisited = c->unvisited;
-            r.unvisited_deg = c->unvisited_deg;
-            r.edges_to_check = c->edges_to_check;
-            r.result = dir == BD_DIR_TD ? nnew_deg : nnew;
-            r.next = nnew;
-            r.next_deg = nnew_deg;
-            r.edges_examined = dir == BD_DIR_TD ? c->frontier_deg : (int64_t)ex;
-            r.max_frontier_deg = c->max_frontier_deg;
-            r.tasks = c->tpref[kSeg + 1];
-            r.queued = c->qpref[kSeg + 1];
-            r.ms = -1.0;
-            r.ms_other = -1.0;
-            a.log[L - 1] = r;
-        }
-        c->steps += 1;
-        if (dir == BD_DIR_TD)
-            c->td_steps += 1;
-        else
-            c->bu_steps += 1;
-        c->reached += nnew;
-        c->reached_deg += nnew_deg;
-        c->unvisited -= nnew;
-        c->unvisited_deg -= nnew_deg;
-        c->frontier = nnew;
-        c->frontier_deg = nnew_deg;
-        c->max_frontier_deg = (int64_t)mx;
-        c->prev_dir = dir;
-        c->td_bitmap = dir == BD_DIR_BU ? 1 : 0;
-        c->level = L;
-        if (nnew == 0) c->done = 1;
-        s.s_new = nnew;
-        s.s_new_deg = nnew_deg;
         s.s_dir = dir;
+        s.s_new = 0;
+        s.s_split = dir == kDirTDClaim;
+        if (dir == kDirTDWrite) {
+            // the write half found nothing by itself: its claim half runs
+            // next, at the same level and from the same state (the frontier
+            // lists stay as they were, for the step's record)
+            c->dir = kDirTDClaim;
+        } else {
+            const int64_t nnew = (int64_t)nn;
+            const int64_t nnew_deg = (int64_t)nd;
+            const int32_t sd = step_dir(dir);
+            const int64_t idx = c->steps;  // records are per level
+            if (publish && idx < a.log_cap) {
+                bd_level_rec r;
+                r.level = idx;
+                r.dir = sd;
+                r.frontier = c->frontier;
+                r.frontier_deg = c->frontier_deg;
+                r.unvisited = c->unvisited;
+                r.unvisited_deg = c->unvisited_deg;
+                r.edges_to_check = c->edges_to_check;
+                r.result = sd == BD_DIR_TD ? nnew_deg : nnew;
+                r.next = nnew;
+                r.next_deg = nnew_deg;
+                r.edges_examined = sd == BD_DIR_TD ? c->frontier_deg : (int64_t)ex;
+                r.max_frontier_deg = c->max_frontier_deg;
+                r.tasks = c->tpref[kSeg + 1];
+                r.queued = c->qpref[kSeg + 1];
+                r.ms = -1.0;
+                r.ms_other = -1.0;
+                r.launches = dir == kDirTDClaim ? 2 : 1;
+                r.launch = L - r.launches;
+                a.log[idx] = r;
+            }
+            c->steps += 1;
+            if (sd == BD_DIR_TD)
+                c->td_steps += 1;
+            else
+                c->bu_steps += 1;
+            c->reached += nnew;
+            c->reached_deg += nnew_deg;
+            c->unvisited -= nnew;
+            c->unvisited_deg -= nnew_deg;
+            c->frontier = nnew;
+            c->frontier_deg = nnew_deg;
+            c->max_frontier_deg = (int64_t)mx;
+            c->prev_dir = sd;
+            // after a bottom-up step or a claim sweep the frontier is only
+            // in the bitmap
+            c->td_bitmap = kernel_dir(dir) == BD_DIR_BU ? 1 : 0;
+            c->level = (int32_t)c->steps;
+            if (nnew == 0) c->done = 1;
+            s.s_new = nnew;
+            s.s_new_deg = nnew_deg;
+            s.s_dir = sd;
+        }
     }
     wave_lds_sync();
-    seg_prefix(ql, qo, a.qseg_cap, a.nv, c->qpref);
-    seg_prefix(tl, to, a.tseg_cap, a.task_cap, c->tpref);
-    wave_lds_sync();
-    if (s.s_new > 0) choose_dir(a, s, c, s.s_dir, s.s_new, s.s_new_deg, w);
+    if (s.s_dir != kDirTDWrite) {
+        seg_prefix(ql, qo, a.qseg_cap, a.nv, c->qpref);
+        seg_prefix(tl, to, a.tseg_cap, a.task_cap, c->tpref);
+        wave_lds_sync();
+        // the remote-write steps of REMOTE_WRITES_HYBRID are its bottom-up
+        // phase (hybrid_bfs.cc:262-268)
+        const int32_t last = (a.policy == BD_POLICY_REMOTE_WRITES_HYBRID && s.s_split)
+                                 ? BD_DIR_BU
+                                 : s.s_dir;
+        if (s.s_new > 0) choose_dir(a, s, c, last, s.s_new, s.s_new_deg, w);
+    }
     return false;
 }
 
@@ -1015,19 +1129,24 @@ __global__ __launch_bounds__(BLOCK) void k_step(step_args a) {
     __syncthreads();
     if (s.st.done || a.close_only) return;
     const int dir = a.force_dir >= 0 ? a.force_dir : s.st.dir;
-    if (DIRS != kStepBoth && dir != DIRS) return;
+    const int kdir = kernel_dir(dir);
+    if (DIRS != kStepBoth && kdir != DIRS) return;
     unsigned long long cnt = 0, dsum = 0, exam = 0, dmax = 0;
-    if (DIRS != kStepBU && dir == BD_DIR_TD) {
+    if (DIRS != kStepBU && kdir == BD_DIR_TD) {
+        const bool tdw = dir == kDirTDWrite;
         for (int64_t i = (int64_t)blockIdx.x * BLOCK + threadIdx.x; i < a.nwords;
              i += (int64_t)gridDim.x * BLOCK)
             a.clr[i] = 0;
         if (s.st.td_bitmap)
-            td_bitmap_body<BLOCK>(a, s, cnt, dsum, dmax);
+            td_bitmap_body<BLOCK>(a, s, tdw, cnt, dsum, dmax);
         else
-            td_body<BLOCK>(a, s, cnt, dsum, dmax);
+            td_body<BLOCK>(a, s, tdw, cnt, dsum, dmax);
         flush_queue<BLOCK>(a, s);
     } else if (DIRS != kStepTD) {
-        bu_body<BLOCK>(a, s, cnt, dsum, dmax, exam);
+        if (dir == kDirTDClaim)
+            td_claim_body<BLOCK>(a, s, cnt, dsum, dmax);
+        else
+            bu_body<BLOCK>(a, s, cnt, dsum, dmax, exam);
     }
     block_flush_counters<BLOCK>(a, s, cnt, dsum, exam, dmax);
 }
@@ -1163,6 +1282,16 @@ __global__ void k_clear(int32_t *__restrict__ parent, int64_t nv, uint64_t *__re
         b1[i] = 0;
         b2[i] = 0;
     }
+}
+
+// Candidate parents to -1.  A search leaves them so (each claim sweep resets
+// what its write half wrote), so this runs only after a search that did not
+// finish, and after allocation: clearing them with every search would push
+// another V-sized array through the caches just before the timed steps.
+__global__ void k_fill_cand(int32_t *__restrict__ cand, int64_t nv) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nv;
+         i += (int64_t)gridDim.x * blockDim.x)
+        cand[i] = -1;
 }
 
 // Profiling: undo the first direction's step at level L (its counters and
@@ -1344,6 +1473,7 @@ step_args make_args(bd_ctx *c, int32_t level, int64_t alpha, int64_t beta, int p
     a.tin = c->d_tasks[level & 1];
     a.tout = c->d_tasks[(level + 1) & 1];
     a.parent = c->d_parent;
+    a.cand = c->d_cand;
     a.c = c->d_ctl;
     a.seg = c->d_seg;
     a.log = c->d_log;
@@ -1359,6 +1489,7 @@ step_args make_args(bd_ctx *c, int32_t level, int64_t alpha, int64_t beta, int p
     a.nnz = c->nnz;
     a.alpha = alpha;
     a.beta = beta;
+    a.td_split = c->split_off ? 0 : (int64_t)c->tune.td_split_edges;
     a.level = level;
     a.close_only = 0;
     a.force_dir = -1;
@@ -1478,7 +1609,7 @@ int check_run_args(bd_ctx *c, int64_t source, int64_t alpha, int64_t beta, int p
     if (source < 0 || source >= c->nv) return set_err(c, BD_ERANGE, "Source vertex out of range.");
     if (alpha <= 0) return set_err(c, BD_EINVAL, "alpha must be > 0");
     if (beta <= 0) return set_err(c, BD_EINVAL, "beta must be > 0");
-    if (policy < BD_POLICY_BEAMER || policy > BD_POLICY_SCRIPT)
+    if (policy < BD_POLICY_BEAMER || policy > BD_POLICY_REMOTE_WRITES_HYBRID)
         return set_err(c, BD_EINVAL, "unknown policy");
     if (policy == BD_POLICY_SCRIPT && c->script_len <= 0)
         return set_err(c, BD_ESTATE, "no script set");
@@ -1490,6 +1621,10 @@ int start_search(bd_ctx *c, int64_t source, int64_t alpha, int64_t beta, int pol
     const int64_t n = std::max(c->nv, c->nwords);
     hipLaunchKernelGGL(k_clear, dim3(grid_for(c, n, 256)), dim3(256), 0, c->stream, c->d_parent,
                        c->nv, c->d_vis, c->d_dead, c->d_bm[0], c->d_bm[1], c->d_bm[2], c->nwords);
+    if (!c->cand_clean)
+        hipLaunchKernelGGL(k_fill_cand, dim3(grid_for(c, c->nv, 256)), dim3(256), 0, c->stream,
+                           c->d_cand, c->nv);
+    c->cand_clean = false;  // until this search finishes
     c->depth_valid = false;
     BD_HIP(c, hipGetLastError());
     // frontier 0: bitmap bm[0], queue q[0], tasks tasks[0]
@@ -1526,6 +1661,7 @@ int finish_search(bd_ctx *c, int64_t source, bd_run_stats *out, double event_ms)
     c->last_steps = h.steps;
     c->last_source = source;
     c->have_result = true;
+    c->cand_clean = true;  // every write half was followed by its claim sweep
     const int64_t n = std::min<int64_t>(h.steps, c->log_cap);
     c->last_log.resize(n);
     if (n)
@@ -1583,7 +1719,8 @@ int run_search(bd_ctx *c, int64_t source, int64_t alpha, int64_t beta, int polic
                            per_chunk ? c->ev_pool[0] : nullptr)))
         return rc;
     BD_HIP(c, hipEventRecord(c->ev[0], c->stream));
-    const int64_t max_levels = c->nv + 2;
+    // launches: a level takes one, or two when it writes remotely
+    const int64_t max_levels = 2 * c->nv + 4;
     int32_t L = 0;
     const bool timed = c->tune.time_steps == 1;
     std::vector<double> step_ms;
@@ -1593,8 +1730,8 @@ int run_search(bd_ctx *c, int64_t source, int64_t alpha, int64_t beta, int polic
             const step_args a = make_args(c, L, alpha, beta, policy);
             if ((rc = close_level(c, a))) return rc;
             if (c->h_ctl[L & 1].done) break;
-            // the host knows the direction: launch that direction's kernel
-            const int dirs = c->h_ctl[L & 1].dir;
+            // the host knows the direction: launch the kernel that runs it
+            const int dirs = kernel_dir(c->h_ctl[L & 1].dir);
             if ((rc = launch_step(c, a, dirs, 0, timed ? c->ev[2] : nullptr,
                                   timed ? c->ev[3] : nullptr)))
                 return rc;
@@ -1651,8 +1788,11 @@ int run_search(bd_ctx *c, int64_t source, int64_t alpha, int64_t beta, int polic
     if ((rc = read_ctl(c))) return rc;
     if ((rc = finish_search(c, source, out, ev_ms))) return rc;
     if (c->tune.time_steps == 3) step_ms = c->last_launch_ms;  // per-launch times only
-    for (size_t i = 0; i < step_ms.size() && i < c->last_log.size(); ++i)
-        c->last_log[i].ms = step_ms[i];
+    for (auto &r : c->last_log) {
+        if (r.launch < 0 || r.launch + r.launches > (int64_t)step_ms.size()) continue;
+        r.ms = 0;
+        for (int64_t k = r.launch; k < r.launch + r.launches; ++k) r.ms += step_ms[k];
+    }
     return BD_OK;
 }
 
@@ -1831,7 +1971,16 @@ extern "C" int bd_bfs_profile(bd_ctx *c, int64_t source, int64_t alpha, int64_t 
     if (!c) return BD_EINVAL;
     int rc;
     if ((rc = check_run_args(c, source, alpha, beta, policy))) return rc;
+    if (policy == BD_POLICY_REMOTE_WRITES || policy == BD_POLICY_REMOTE_WRITES_HYBRID)
+        return set_err(c, BD_EINVAL, "profile times one-launch steps: not for remote-write policies");
     BD_HIP(c, hipSetDevice(c->device));
+    // each level is timed in both directions as one launch each: top-down
+    // steps run without remote writes here
+    c->split_off = true;
+    struct split_reset {
+        bd_ctx *c;
+        ~split_reset() { c->split_off = false; }
+    } sr{c};
     const int64_t scrub_n = ((int64_t)512 << 20) / 16;
     uint4 *scrub = nullptr;
     if ((rc = dev_alloc(c, (void **)&scrub, sizeof(uint4) * (scrub_n + 1), "scrub"))) return rc;

@@ -40,6 +40,7 @@ void dev_free(void *p) {
 static void free_search(bd_ctx *c) {
     dev_free(c->d_depth);
     dev_free(c->d_parent);
+    dev_free(c->d_cand);
     dev_free(c->d_vis);
     dev_free(c->d_vis_save);
     for (auto &b : c->d_bm) dev_free(b);
@@ -48,7 +49,7 @@ static void free_search(bd_ctx *c) {
     for (auto &b : c->d_ms) dev_free(b);
     for (auto &b : c->d_ms_bits) dev_free(b);
     dev_free(c->d_ms_depth);
-    c->d_depth = c->d_parent = nullptr;
+    c->d_depth = c->d_parent = c->d_cand = nullptr;
     c->d_vis = c->d_vis_save = nullptr;
     for (auto &b : c->d_bm) b = nullptr;
     for (auto &q : c->d_q) q = nullptr;
@@ -87,10 +88,13 @@ int64_t grid_for(const bd_ctx *c, int64_t work_items, int block) {
 
 int alloc_search(bd_ctx *c) {
     free_search(c);
+    c->cand_clean = false;  // the first search fills the candidate array
     int rc;
     const int64_t nv = c->nv, nw = c->nwords;
     if ((rc = dev_alloc(c, (void **)&c->d_depth, sizeof(int32_t) * nv, "depth"))) return rc;
     if ((rc = dev_alloc(c, (void **)&c->d_parent, sizeof(int32_t) * nv, "parent"))) return rc;
+    if ((rc = dev_alloc(c, (void **)&c->d_cand, sizeof(int32_t) * nv, "candidate parents")))
+        return rc;
     if ((rc = dev_alloc(c, (void **)&c->d_vis, sizeof(uint64_t) * nw, "visited"))) return rc;
     if ((rc = dev_alloc(c, (void **)&c->d_vis_save, sizeof(uint64_t) * nw, "visited copy")))
         return rc;
@@ -154,6 +158,7 @@ extern "C" int bd_create(bd_ctx **out, int device) {
     c->tune.learned_margin_milli = 250;
     c->tune.ms_alpha = 4;
     c->tune.learned_min_skew = 8;
+    c->tune.td_split_edges = 1 << 21;
     hipError_t e = hipSetDevice(device);
     if (e == hipSuccess) {
         hipDeviceProp_t prop;
@@ -273,6 +278,7 @@ extern "C" int bd_set_tuning(bd_ctx *c, const bd_tuning *t) {
     if (t->learned_margin_milli > 0) n.learned_margin_milli = t->learned_margin_milli;
     if (t->ms_alpha > 0) n.ms_alpha = t->ms_alpha;
     if (t->learned_min_skew > 0) n.learned_min_skew = t->learned_min_skew;
+    if (t->td_split_edges > 0) n.td_split_edges = t->td_split_edges;
     bool realloc = (n.task_edges != c->tune.task_edges || n.task_threshold != c->tune.task_threshold);
     c->tune = n;
     if (realloc && c->d_off) return alloc_search(c);

@@ -115,6 +115,7 @@ struct bd_ctx {
     // search state
     int32_t *d_depth = nullptr;
     int32_t *d_parent = nullptr;
+    int32_t *d_cand = nullptr;    // candidate parents of a remote-write top-down step (-1 = none)
     uint64_t *d_vis = nullptr;
     uint64_t *d_vis_save = nullptr;
     uint64_t *d_bm[3] = {nullptr, nullptr, nullptr};
@@ -135,6 +136,8 @@ struct bd_ctx {
     // results of the last run
     bool have_result = false;
     bool depth_valid = false;  // d_depth holds the last search's depths
+    bool split_off = false;    // profiling: top-down steps never run with remote writes
+    bool cand_clean = false;   // d_cand is all -1 (set when a search finishes)
     int64_t last_source = -1;
     int64_t last_steps = 0;
     std::vector<bd_level_rec> last_log;

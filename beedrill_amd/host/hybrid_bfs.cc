@@ -23,12 +23,12 @@ void hybrid_bfs::run_with_migrating_threads(long source) {
 
 void hybrid_bfs::run_with_remote_writes(long source) {
     assert(source < g_->num_vertices());
-    run(source, 15, 18, BD_POLICY_TD_ONLY);
+    run(source, 15, 18, BD_POLICY_REMOTE_WRITES);
 }
 
 void hybrid_bfs::run_with_remote_writes_hybrid(long source, long alpha, long beta) {
     assert(source < g_->num_vertices());
-    run(source, alpha, beta, BD_POLICY_TD_ONLY);
+    run(source, alpha, beta, BD_POLICY_REMOTE_WRITES_HYBRID);
 }
 
 void hybrid_bfs::clear() { have_ = false; }

@@ -106,11 +106,10 @@ def roofline(ctx, roots, policy, nv, n_isolated):
         ctx.set_tuning(time_steps=3)
         for r in roots:
             ctx.bfs(int(r), 15, 18, policy)
-            lt = ctx.launch_times()
-            for rec in ctx.levels():
+            for rec in ctx.levels():  # rec["ms"]: its launches' times
                 if rec["dir"] == 1:
                     bu_b += bu_bytes(rec, nv, n_isolated)
-                    bu_ms += lt[rec["level"]]
+                    bu_ms += rec["ms"]
                     bu_n += 1
     finally:
         ctx.set_tuning(time_steps=t["time_steps"])

@@ -46,14 +46,20 @@ typedef enum {
  * evaluates the MLP cost model beside the two predicates at
  * hybrid_bfs.cc:167 and :178-179 and takes its direction when it predicts a
  * gain larger than bd_tuning.learned_margin_milli; with no model loaded it
- * is Beamer;
+ * is Beamer; REMOTE_WRITES is run_with_remote_writes (hybrid_bfs.cc:218-236),
+ * every step a top-down step with remote writes; REMOTE_WRITES_HYBRID is
+ * run_with_remote_writes_hybrid (hybrid_bfs.cc:243-275), Beamer's state
+ * machine with remote-write top-down steps where Beamer goes bottom-up.
+ * TD_ONLY and the two REMOTE policies ignore bd_tuning.td_split_edges.
  * BU_ONLY and SCRIPT exist for testing and for timing both directions. */
 typedef enum {
     BD_POLICY_BEAMER = 0,
     BD_POLICY_LEARNED = 1,
     BD_POLICY_TD_ONLY = 2,
     BD_POLICY_BU_ONLY = 3,
-    BD_POLICY_SCRIPT = 4
+    BD_POLICY_SCRIPT = 4,
+    BD_POLICY_REMOTE_WRITES = 5,
+    BD_POLICY_REMOTE_WRITES_HYBRID = 6
 } bd_policy;
 
 enum { BD_DIR_TD = 0, BD_DIR_BU = 1 };
@@ -61,7 +67,9 @@ enum { BD_DIR_TD = 0, BD_DIR_BU = 1 };
 /* Per-step record (one per top-down or bottom-up step).  The first fields
  * mirror the reference's step-level quantities: |F| (sliding_queue
  * combined_size, sliding_queue.h:126-136), scout/awake returned by the step
- * (hybrid_bfs.cc:99, :142) and edges_to_check (:162, :182). */
+ * (hybrid_bfs.cc:99, :142) and edges_to_check (:162, :182).  A top-down step
+ * run with remote writes (bd_tuning.td_split_edges) takes two launches and
+ * gets one record. */
 typedef struct {
     int64_t level;            /* depth of the frontier expanded            */
     int64_t dir;              /* BD_DIR_TD or BD_DIR_BU                    */
@@ -79,6 +87,9 @@ typedef struct {
     int64_t queued;           /* F's other vertices, read from the queue   */
     double ms;                /* device time of the step (timed runs only) */
     double ms_other;          /* profile runs: time of the other direction */
+    int64_t launch;           /* index of the step's first kernel launch   */
+    int64_t launches;         /* its launches: 1, or 2 for a top-down step
+                                 run with remote writes                    */
 } bd_level_rec;
 
 typedef struct {
@@ -180,9 +191,9 @@ int bd_bfs_profile(bd_ctx *ctx, int64_t source, int64_t alpha, int64_t beta, int
 
 /* Kernel time of every step launch of the last search run with
  * bd_tuning.time_steps = 3 (each launch timed) or 4 (each chunk of launches
- * timed, its launches given the chunk's mean), in launch order (level L is
- * launch L; launches queued after the search ended return at once and are
- * listed too).  No
+ * timed, its launches given the chunk's mean), in launch order (a level's
+ * launches are its record's launch .. launch + launches - 1; launches queued
+ * after the search ended return at once and are listed too).  No
  * reference counterpart: the measurement behind the step kernel's roofline. */
 int bd_bfs_launch_times(bd_ctx *ctx, double *ms, int64_t cap, int64_t *n);
 
@@ -235,6 +246,12 @@ typedef struct {
                                 whose largest degree is at least this many
                                 times the mean degree (the graphs it was
                                 trained on have hubs); otherwise Beamer   */
+    int32_t td_split_edges;  /* a top-down step whose frontier has at least
+                                this many edges runs with remote writes
+                                (hybrid_bfs.cc:41-67): one launch writes
+                                each edge's source as the candidate parent
+                                of its unvisited end, the next claims one
+                                candidate per vertex; 2^31-1 = never       */
     int32_t reserved[1];
 } bd_tuning;
 int bd_set_tuning(bd_ctx *ctx, const bd_tuning *t);

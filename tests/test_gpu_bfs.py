@@ -194,6 +194,113 @@ def test_s16_host_driven_levels_identical(ctx, orc, s16):
         ctx.set_tuning(**t0)
 
 
+def _launches_follow(lv):
+    # a level's launches start where the previous level's end
+    starts = np.cumsum([0] + [r["launches"] for r in lv])[:-1]
+    return [r["launch"] for r in lv] == starts.tolist()
+
+
+def test_s16_remote_write_topdown(ctx, orc, s16):
+    # Top-down steps with remote writes take two launches (candidate writes,
+    # then the claim sweep) and give the oracle's depths and records: on the
+    # fused kernel, one kernel per direction, host-driven and per-launch
+    # timed, for the top-down-only policy, and with a threshold that splits
+    # only the heavy steps.
+    ctx.generate_rmat(16, 16)
+    off, adj, m = s16
+    roots = [int(r) for r in ctx.pick_roots(3)]
+    t0 = ctx.get_tuning()
+    try:
+        ctx.set_tuning(td_split_edges=1)
+        for root in roots:
+            run_and_compare(ctx, orc, off, adj, m, root)
+            lv = ctx.levels()
+            assert all(r["launches"] == (2 if r["dir"] == bd.DIR_TD else 1) for r in lv)
+            assert _launches_follow(lv)
+        ctx.set_tuning(step_kernels=1)
+        run_and_compare(ctx, orc, off, adj, m, roots[0])
+        ctx.set_tuning(level_driver=2)
+        run_and_compare(ctx, orc, off, adj, m, roots[0])
+        ctx.set_tuning(level_driver=1, step_kernels=2, time_steps=3)
+        run_and_compare(ctx, orc, off, adj, m, roots[1])
+        lv = ctx.levels()
+        lt = ctx.launch_times()
+        for r in lv:
+            assert r["ms"] == pytest.approx(lt[r["launch"]:r["launch"] + r["launches"]].sum())
+        ctx.set_tuning(time_steps=2)
+        run_and_compare(ctx, orc, off, adj, m, roots[1], policy="td_only", levels=False)
+        _, ref_depth = orc.td_only(off, adj, m, roots[1])
+        np.testing.assert_array_equal(ctx.fetch()[0], ref_depth)
+        split = 20000
+        ctx.set_tuning(td_split_edges=split)
+        for root in roots:
+            run_and_compare(ctx, orc, off, adj, m, root)
+            lv = ctx.levels()
+            for r in lv:
+                heavy = r["dir"] == bd.DIR_TD and r["frontier_deg"] >= split
+                assert r["launches"] == (2 if heavy else 1)
+            assert _launches_follow(lv)
+    finally:
+        ctx.set_tuning(**t0)
+
+
+def test_s16_remote_write_policies(ctx, orc, s16):
+    # run_with_remote_writes: every step top-down with remote writes, the
+    # levels of the top-down-only search.  run_with_remote_writes_hybrid:
+    # Beamer's state machine with remote-write steps exactly where Beamer
+    # goes bottom-up, so its records carry Beamer's level quantities.
+    ctx.generate_rmat(16, 16)
+    off, adj, m = s16
+    head = orc.head_index(off, adj)
+    same = ("level", "frontier", "frontier_deg", "unvisited", "unvisited_deg",
+            "edges_to_check", "next", "next_deg", "max_frontier_deg")
+    for root in [int(r) for r in ctx.pick_roots(3)]:
+        run_and_compare(ctx, orc, off, adj, m, root, policy="remote_writes", levels=False)
+        _, ref_depth = orc.td_only(off, adj, m, root)
+        np.testing.assert_array_equal(ctx.fetch()[0], ref_depth)
+        lv = ctx.levels()
+        assert all(r["dir"] == bd.DIR_TD and r["launches"] == 2 for r in lv)
+        assert _launches_follow(lv)
+        run_and_compare(ctx, orc, off, adj, m, root, policy="remote_writes_hybrid", levels=False)
+        lv = ctx.levels()
+        _, _, ref_lv = orc.beamer(off, adj, m, root, 15, 18, head=head)
+        assert_same_levels(lv, ref_lv, keys=same)
+        assert [r["launches"] for r in lv] == [2 if o["dir"] == bd.DIR_BU else 1 for o in ref_lv]
+        assert all(r["dir"] == bd.DIR_TD for r in lv)
+    with pytest.raises(bd.BeedrillError):
+        ctx.profile(int(ctx.pick_roots(1)[0]), policy="remote_writes")
+
+
+@pytest.mark.parametrize("alg", ["beamer_hybrid", "remote_writes", "remote_writes_hybrid",
+                                 "migrating_threads"])
+def test_cli_algorithms_check(alg):
+    exe = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))),
+                       "beedrill_amd", "bin", "hybrid_bfs")
+    import subprocess
+    r = subprocess.run([exe, "--generate", "graph500-scale12", "--num_trials", "3",
+                        "--algorithm", alg, "--check_results"],
+                       capture_output=True, text=True, timeout=120)
+    out = r.stdout + r.stderr
+    assert r.returncode == 0, out
+    assert out.count("PASS") >= 3 and "FAIL" not in out, out
+
+
+@pytest.mark.parametrize("g,root", [(path_graph(100), 0), (star_graph(100), 1),
+                                    (complete_graph(20), 0), (two_components(), 3)])
+def test_small_graphs_remote_write_topdown(ctx, orc, g, root):
+    nv, s, d = g
+    ctx.from_edges(nv, s, d)
+    off, adj, m = oracle_csr(orc, nv, s, d)
+    t0 = ctx.get_tuning()
+    try:
+        ctx.set_tuning(td_split_edges=1)
+        run_and_compare(ctx, orc, off, adj, m, root)
+        run_and_compare(ctx, orc, off, adj, m, root, policy="td_only", levels=False)
+        assert ctx.check(root)
+    finally:
+        ctx.set_tuning(**t0)
+
+
 def test_s16_alpha_beta_sweep(ctx, orc, s16):
     ctx.generate_rmat(16, 16)
     off, adj, m = s16
