@@ -9,35 +9,49 @@ namespace bd {
 
 __device__ __forceinline__ int lane_id() { return threadIdx.x & 63; }
 
-__device__ __forceinline__ unsigned long long wave_sum(unsigned long long x) {
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) x += (unsigned long long)__shfl_xor((long long)x, o);
+// Wave-wide scans and reductions with DPP lane moves (no LDS round trips):
+// row_shr 1, 2, 4, 8 scan each 16-lane row, then row_bcast 15 and 31 carry
+// the row totals into the rows above.  Lanes a move cannot reach read 0.
+// Every lane of the wave must be active; results are inclusive (lane i
+// holds the combination of lanes 0..i), reductions read lane 63.
+template <int CTRL, int ROW_MASK>
+__device__ __forceinline__ uint64_t dpp_move64(uint64_t x) {
+    const int lo = __builtin_amdgcn_update_dpp(0, (int)(uint32_t)x, CTRL, ROW_MASK, 0xf, true);
+    const int hi = __builtin_amdgcn_update_dpp(0, (int)(uint32_t)(x >> 32), CTRL, ROW_MASK, 0xf, true);
+    return ((uint64_t)(uint32_t)hi << 32) | (uint32_t)lo;
+}
+
+template <typename Op>
+__device__ __forceinline__ uint64_t wave_scan64(uint64_t x, Op op) {
+    x = op(x, dpp_move64<0x111, 0xf>(x));  // row_shr:1
+    x = op(x, dpp_move64<0x112, 0xf>(x));  // row_shr:2
+    x = op(x, dpp_move64<0x114, 0xf>(x));  // row_shr:4
+    x = op(x, dpp_move64<0x118, 0xf>(x));  // row_shr:8
+    x = op(x, dpp_move64<0x142, 0xa>(x));  // row_bcast:15 into rows 1, 3
+    x = op(x, dpp_move64<0x143, 0xc>(x));  // row_bcast:31 into rows 2, 3
     return x;
 }
 
-__device__ __forceinline__ unsigned long long wave_max(unsigned long long x) {
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) {
-        const unsigned long long y = (unsigned long long)__shfl_xor((long long)x, o);
-        x = y > x ? y : x;
-    }
-    return x;
-}
-
-__device__ __forceinline__ unsigned long long wave_or(unsigned long long x) {
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) x |= (unsigned long long)__shfl_xor((long long)x, o);
-    return x;
+__device__ __forceinline__ uint64_t lane63(uint64_t x) {
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)x, 63);
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(x >> 32), 63);
+    return ((uint64_t)hi << 32) | lo;
 }
 
 __device__ __forceinline__ long long wave_incl_scan(long long x) {
-    const int lane = lane_id();
-#pragma unroll
-    for (int o = 1; o < 64; o <<= 1) {
-        const long long y = __shfl_up(x, o);
-        if (lane >= o) x += y;
-    }
-    return x;
+    return (long long)wave_scan64((uint64_t)x, [](uint64_t p, uint64_t q) { return p + q; });
+}
+
+__device__ __forceinline__ unsigned long long wave_sum(unsigned long long x) {
+    return lane63(wave_scan64(x, [](uint64_t p, uint64_t q) { return p + q; }));
+}
+
+__device__ __forceinline__ unsigned long long wave_max(unsigned long long x) {
+    return lane63(wave_scan64(x, [](uint64_t p, uint64_t q) { return p > q ? p : q; }));
+}
+
+__device__ __forceinline__ unsigned long long wave_or(unsigned long long x) {
+    return lane63(wave_scan64(x, [](uint64_t p, uint64_t q) { return p | q; }));
 }
 
 // Position of the k-th set bit (k from 0) of x.
