@@ -162,8 +162,8 @@ __global__ __launch_bounds__(kMsBlock) void k_ms_push(ms_args a) {
         while (lm) {
             const int src = __ffsll((unsigned long long)lm) - 1;
             lm &= lm - 1;
-            const int64_t kb = __shfl(b, src), ke = __shfl(e, src);
-            const uint64_t fs = (uint64_t)__shfl((long long)f, src);
+            const int64_t kb = lane_bcast(b, src), ke = lane_bcast(e, src);
+            const uint64_t fs = (uint64_t)lane_bcast((int64_t)f, src);
             for (int64_t k = kb + lane; k < ke; k += 64) ms_push_to(a, ms_col(a, k), fs);
         }
     }
@@ -238,8 +238,8 @@ __global__ __launch_bounds__(kMsBlock) void k_ms_pull(ms_args a) {
         while (lm) {
             const int src = __ffsll((unsigned long long)lm) - 1;
             lm &= lm - 1;
-            const int64_t kb = __shfl(b, src), ke = __shfl(e, src);
-            const uint64_t ws = (uint64_t)__shfl((long long)want, src);
+            const int64_t kb = lane_bcast(b, src), ke = lane_bcast(e, src);
+            const uint64_t ws = (uint64_t)lane_bcast((int64_t)want, src);
             uint64_t got = 0;
             for (int64_t k0 = kb; k0 < ke && (got & ws) != ws; k0 += 64) {
                 const int64_t k = k0 + lane;

@@ -189,7 +189,7 @@ __device__ void emit_tasks(const step_args &a, bool huge, int32_t u, int64_t b, 
     const int lane = lane_id();
     const long long nt = huge ? (e - b + a.task_edges - 1) / a.task_edges : 0;
     const long long incl = wave_incl_scan(nt);
-    const long long total = __shfl(incl, 63);
+    const long long total = lane_bcast((int64_t)incl, 63);
     const long long excl = incl - nt;
     long long pos = 0, opos = 0, fit = 0;
     if (lane == 0) {
@@ -198,9 +198,9 @@ __device__ void emit_tasks(const step_args &a, bool huge, int32_t u, int64_t b, 
         pos = ps;
         opos = ops;
     }
-    pos = __shfl(pos, 0);
-    opos = __shfl(opos, 0);
-    fit = __shfl(fit, 0);
+    pos = lane_bcast((int64_t)pos, 0);
+    opos = lane_bcast((int64_t)opos, 0);
+    fit = lane_bcast((int64_t)fit, 0);
     for (long long j0 = 0; j0 < total; j0 += 64) {
         const long long j = j0 + lane;
         const long long jj = j < total ? j : total - 1;
@@ -249,7 +249,7 @@ __device__ __forceinline__ void emit_next(const step_args &a, step_smem<BLOCK> &
     const int cnt = __popcll(m);
     int p = 0;
     if (lane == leader) p = atomicAdd(&s.qn, cnt);
-    p = __shfl(p, leader);
+    p = lane_bcast((int32_t)p, leader);
     const int rank = __popcll(m & ((1ULL << lane) - 1));
     // Slots below kQCap are staged in LDS (the flush writes exactly those);
     // the part of the batch past the end goes straight to HBM.
@@ -262,9 +262,9 @@ __device__ __forceinline__ void emit_next(const step_args &a, step_smem<BLOCK> &
             pos = ps;
             opos = ops;
         }
-        pos = __shfl(pos, leader);
-        opos = __shfl(opos, leader);
-        fit = __shfl(fit, leader);
+        pos = lane_bcast((int64_t)pos, leader);
+        opos = lane_bcast((int64_t)opos, leader);
+        fit = lane_bcast((int64_t)fit, leader);
     }
     if (q) {
         if (rank < in_lds) {
@@ -398,12 +398,7 @@ __device__ void td_tile(const step_args &a, step_smem<BLOCK> &s, bool tdw, int32
     s.t_beg[tid] = b;
     s.t_end[tid] = e;
     s.t_src[tid] = v;
-    int32_t incl = small;
-#pragma unroll
-    for (int o = 1; o < 64; o <<= 1) {
-        const int32_t y = __shfl_up(incl, o);
-        if (lane >= o) incl += y;
-    }
+    const int32_t incl = wave_incl_scan32(small);
     if (lane == 63) s.t_wsum[wid] = incl;
     __syncthreads();
     int32_t wbase = 0;
@@ -502,12 +497,7 @@ __device__ void td_bitmap_body(const step_args &a, step_smem<BLOCK> &s, bool tdw
         const uint64_t w = wi < a.nwords ? a.front[wi] : 0ULL;
         const int32_t c = __popcll(w);
         s.w_bits[tid] = w;
-        int32_t incl = c;
-#pragma unroll
-        for (int o = 1; o < 64; o <<= 1) {
-            const int32_t y = __shfl_up(incl, o);
-            if (lane >= o) incl += y;
-        }
+        const int32_t incl = wave_incl_scan32(c);
         if (lane == 63) s.w_wsum[wid] = incl;
         __syncthreads();
         int32_t wbase = 0;
@@ -560,8 +550,8 @@ __device__ void bu_pending(const step_args &a, step_smem<BLOCK> &s, const int32_
         e = a.off[v + 1];
     }
     const int32_t len = (int32_t)(e - b < a.bu_cap ? e - b : a.bu_cap);
-    int32_t excl = (int32_t)wave_incl_scan(len);
-    const int32_t total = __shfl(excl, 63);
+    int32_t excl = wave_incl_scan32(len);
+    const int32_t total = lane_bcast(excl, 63);
     excl -= len;
     unsigned long long *hit = s.b_hit[wid];
     hit[lane] = ~0ULL;
@@ -600,7 +590,8 @@ __device__ void bu_pending(const step_args &a, step_smem<BLOCK> &s, const int32_
     while (lm) {
         const int src_lane = __ffsll((unsigned long long)lm) - 1;
         lm &= lm - 1;
-        const int64_t kb = __shfl(b, src_lane) + __shfl(len, src_lane), ke = __shfl(e, src_lane);
+        const int64_t kb = lane_bcast(b, src_lane) + lane_bcast(len, src_lane),
+                      ke = lane_bcast(e, src_lane);
         int32_t found = -1;
         for (int64_t k0 = kb; k0 < ke && found < 0; k0 += 64) {
             const int64_t kk = k0 + lane;
@@ -610,7 +601,7 @@ __device__ void bu_pending(const step_args &a, step_smem<BLOCK> &s, const int32_
             const int64_t rest = ke - k0 < 64 ? ke - k0 : 64;
             const int64_t seen = hm ? (int64_t)__ffsll((unsigned long long)hm) : rest;
             if (lane == src_lane) exam += (unsigned long long)seen;
-            if (hm) found = __shfl(u, __ffsll((unsigned long long)hm) - 1);
+            if (hm) found = lane_bcast(u, __ffsll((unsigned long long)hm) - 1);
         }
         if (lane == src_lane) par = found;
     }
@@ -664,8 +655,8 @@ __device__ void td_claim_body(const step_args &a, step_smem<BLOCK> &s, unsigned 
         if (own) a.clr[wi] = 0;
         const uint64_t unv = ~vw;
         const int cntw = __popcll(unv);
-        int excl = (int)wave_incl_scan(cntw);
-        const int total = __shfl(excl, 63);
+        int excl = wave_incl_scan32(cntw);
+        const int total = lane_bcast(excl, 63);
         excl -= cntw;
         s.b_found[wid][lane] = 0;
         wave_lds_sync();
@@ -721,8 +712,8 @@ __device__ void bu_body(const step_args &a, step_smem<BLOCK> &s, unsigned long l
         if (own) a.clr[wi] = 0;
         const uint64_t unv = ~vw;
         const int cntw = __popcll(unv);
-        int excl = (int)wave_incl_scan(cntw);
-        const int total = __shfl(excl, 63);
+        int excl = wave_incl_scan32(cntw);
+        const int total = lane_bcast(excl, 63);
         excl -= cntw;
         s.b_found[wid][lane] = 0;
         int np = 0;  // pending vertices (wave-uniform)
@@ -1227,7 +1218,7 @@ __global__ __launch_bounds__(64) void k_init(step_args a, int64_t source,
             else
                 ++lost;
         }
-        nt += __shfl(incl, 63);
+        nt += lane_bcast((int64_t)incl, 63);
     }
     cnt = wave_sum(cnt);
     dsum = wave_sum(dsum);
@@ -1428,7 +1419,7 @@ __global__ __launch_bounds__(256) void k_head_index(const int64_t *__restrict__ 
         while (lm) {
             const int src = __ffsll((unsigned long long)lm) - 1;
             lm &= lm - 1;
-            const int64_t kb = __shfl(b, src), ke = __shfl(e, src);
+            const int64_t kb = lane_bcast(b, src), ke = lane_bcast(e, src);
             unsigned long long best = 0;
             for (int64_t k = kb + lane; k < ke; k += 64) {
                 const unsigned long long key = head_key(off, cols[k], k - kb);

@@ -38,6 +38,27 @@ __device__ __forceinline__ uint64_t lane63(uint64_t x) {
     return ((uint64_t)hi << 32) | lo;
 }
 
+// x from lane `src`, which must be the same in every lane (a scalar read,
+// no LDS round trip).
+__device__ __forceinline__ int32_t lane_bcast(int32_t x, int src) {
+    return __builtin_amdgcn_readlane(x, src);
+}
+__device__ __forceinline__ int64_t lane_bcast(int64_t x, int src) {
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(uint64_t)x, src);
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)((uint64_t)x >> 32), src);
+    return (int64_t)(((uint64_t)hi << 32) | lo);
+}
+
+__device__ __forceinline__ int32_t wave_incl_scan32(int32_t x) {
+    x += __builtin_amdgcn_update_dpp(0, x, 0x111, 0xf, 0xf, true);
+    x += __builtin_amdgcn_update_dpp(0, x, 0x112, 0xf, 0xf, true);
+    x += __builtin_amdgcn_update_dpp(0, x, 0x114, 0xf, 0xf, true);
+    x += __builtin_amdgcn_update_dpp(0, x, 0x118, 0xf, 0xf, true);
+    x += __builtin_amdgcn_update_dpp(0, x, 0x142, 0xa, 0xf, true);
+    x += __builtin_amdgcn_update_dpp(0, x, 0x143, 0xc, 0xf, true);
+    return x;
+}
+
 __device__ __forceinline__ long long wave_incl_scan(long long x) {
     return (long long)wave_scan64((uint64_t)x, [](uint64_t p, uint64_t q) { return p + q; });
 }
