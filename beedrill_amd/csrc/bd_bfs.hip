@@ -59,6 +59,8 @@ namespace {
 constexpr int kMaxLevelsPerSync = 64;
 constexpr int kQCap = 2048;  // LDS-staged queue entries per workgroup
 constexpr int64_t kPrecheckEdges = 1 << 16;  // top-down steps this heavy pre-check claims
+// Top-down rounds of 64 edges a wave visits together (see td_visit_n).
+constexpr int kTdRounds = 2;
 
 // A top-down step run with remote writes (hybrid_bfs.cc:41-67) takes two
 // launches: kDirTDWrite writes each frontier edge's source as the candidate
@@ -372,6 +374,71 @@ __device__ __forceinline__ void td_visit(const step_args &a, step_smem<BLOCK> &s
     emit_next(a, s, won, u, b, e);
 }
 
+// kTdRounds rounds of edges (u[r] on each lane, all from `src` or each
+// with its own source) visited together: each round's claim is a chain of
+// dependent memory round trips (pre-check, claim, the new vertex's
+// offsets), so the rounds' loads and claims are issued back to back and a
+// wave waits one chain for up to 64 * kTdRounds edges.  Same effect as
+// td_visit on each round in turn.
+template <int BLOCK>
+__device__ __forceinline__ void td_visit_n(const step_args &a, step_smem<BLOCK> &s, bool tdw,
+                                           const bool (&valid)[kTdRounds],
+                                           const int32_t (&u)[kTdRounds],
+                                           const int32_t (&src)[kTdRounds], unsigned long long &cnt,
+                                           unsigned long long &dsum, unsigned long long &dmax) {
+    if (tdw) {
+#pragma unroll
+        for (int r = 0; r < kTdRounds; ++r)
+            td_visit(a, s, true, valid[r], u[r], src[r], cnt, dsum, dmax);
+        return;
+    }
+    // Loads and claims are issued for every round before any result is
+    // used, with in-range addresses for idle lanes (their results are
+    // dropped), so the compiler keeps them in flight together.
+    uint64_t seen[kTdRounds];
+    if (s.st.frontier_deg >= kPrecheckEdges) {
+#pragma unroll
+        for (int r = 0; r < kTdRounds; ++r)
+            seen[r] = __hip_atomic_load((unsigned long long *)(a.vis + (u[r] >> 6)),
+                                        __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    } else {
+#pragma unroll
+        for (int r = 0; r < kTdRounds; ++r) seen[r] = 0;
+    }
+    unsigned long long old[kTdRounds];
+#pragma unroll
+    for (int r = 0; r < kTdRounds; ++r) {
+        const uint64_t bit = 1ULL << (u[r] & 63);
+        old[r] = ~0ULL;  // no claim
+        if (valid[r] && !(seen[r] & bit))
+            old[r] = atomicOr((unsigned long long *)(a.vis + (u[r] >> 6)), (unsigned long long)bit);
+    }
+    bool won[kTdRounds];
+    int64_t b[kTdRounds], e[kTdRounds];
+#pragma unroll
+    for (int r = 0; r < kTdRounds; ++r) {
+        won[r] = !(old[r] & (1ULL << (u[r] & 63)));
+        const int32_t w = won[r] ? u[r] : 0;
+        b[r] = a.off[w];
+        e[r] = a.off[w + 1];
+    }
+#pragma unroll
+    for (int r = 0; r < kTdRounds; ++r) {
+        if (won[r]) {
+            a.parent[u[r]] = src[r];
+            const unsigned long long d = (unsigned long long)(e[r] - b[r]);
+            ++cnt;
+            dsum += d;
+            dmax = d > dmax ? d : dmax;
+            atomicOr((unsigned long long *)(a.next_bm + (u[r] >> 6)),
+                     (unsigned long long)(1ULL << (u[r] & 63)));
+        } else {
+            b[r] = e[r] = 0;
+        }
+        emit_next(a, s, won[r], u[r], b[r], e[r]);
+    }
+}
+
 // One tile of frontier vertices, one per thread (v < 0: none).  Vertices of
 // degree >= wave_t get a wave each; the edges of the others are concatenated
 // by a block-wide scan and every thread takes one.  Whole workgroup.
@@ -412,30 +479,42 @@ __device__ void td_tile(const step_args &a, step_smem<BLOCK> &s, bool tdw, int32
         const int t = s.t_med[i];
         const int64_t mb = s.t_beg[t], me = s.t_end[t];
         const int32_t src = s.t_src[t];
-        for (int64_t k0 = mb; k0 < me; k0 += 64) {
-            const int64_t k = k0 + lane;
-            const bool valid = k < me;
-            const int32_t u = valid ? a.cols[k] : 0;
-            td_visit(a, s, tdw, valid, u, src, cnt, dsum, dmax);
+        for (int64_t k0 = mb; k0 < me; k0 += 64 * kTdRounds) {
+            bool valid[kTdRounds];
+            int32_t u[kTdRounds], sr[kTdRounds];
+#pragma unroll
+            for (int r = 0; r < kTdRounds; ++r) {
+                const int64_t k = k0 + 64 * r + lane;
+                valid[r] = k < me;
+                u[r] = valid[r] ? __builtin_nontemporal_load(a.cols + k) : 0;
+                sr[r] = src;
+            }
+            td_visit_n(a, s, tdw, valid, u, sr, cnt, dsum, dmax);
         }
     }
-    for (int32_t i0 = 0; i0 < total; i0 += BLOCK) {
-        const int32_t i = i0 + tid;
-        const bool valid = i < total;
-        int32_t u = 0, src = 0;
-        if (valid) {
-            int lo = 0, hi = BLOCK;
-            while (hi - lo > 1) {
-                const int mid = (lo + hi) >> 1;
-                if (s.t_pref[mid] <= i)
-                    lo = mid;
-                else
-                    hi = mid;
+    for (int32_t i0 = 0; i0 < total; i0 += BLOCK * kTdRounds) {
+        bool valid[kTdRounds];
+        int32_t u[kTdRounds], sr[kTdRounds];
+#pragma unroll
+        for (int r = 0; r < kTdRounds; ++r) {
+            const int32_t i = i0 + BLOCK * r + tid;
+            valid[r] = i < total;
+            u[r] = 0;
+            sr[r] = 0;
+            if (valid[r]) {
+                int lo = 0, hi = BLOCK;
+                while (hi - lo > 1) {
+                    const int mid = (lo + hi) >> 1;
+                    if (s.t_pref[mid] <= i)
+                        lo = mid;
+                    else
+                        hi = mid;
+                }
+                sr[r] = s.t_src[lo];
+                u[r] = a.cols[s.t_beg[lo] + (i - s.t_pref[lo])];
             }
-            src = s.t_src[lo];
-            u = a.cols[s.t_beg[lo] + (i - s.t_pref[lo])];
         }
-        td_visit(a, s, tdw, valid, u, src, cnt, dsum, dmax);
+        td_visit_n(a, s, tdw, valid, u, sr, cnt, dsum, dmax);
     }
     __syncthreads();
     if (s.qn >= kQCap / 2) flush_queue<BLOCK>(a, s);
@@ -448,17 +527,32 @@ __device__ void td_tasks(const step_args &a, step_smem<BLOCK> &s, bool tdw, unsi
                          unsigned long long &dsum, unsigned long long &dmax) {
     const int lane = lane_id(), wid = threadIdx.x >> 6;
     const int64_t ntasks = s.st.tpref[kSeg + 1];
-    for (int64_t t0 = (int64_t)blockIdx.x * (BLOCK / 64); t0 < ntasks;
-         t0 += (int64_t)gridDim.x * (BLOCK / 64)) {
+    const int64_t waves = (int64_t)gridDim.x * (BLOCK / 64);
+    // A frontier of a few high-degree vertices has too few slices to occupy
+    // the grid, and each slice is a chain of dependent claims: then every
+    // slice is cut into 64-edge pieces, each taken by its own wave.
+    const int64_t piece = ntasks * ((a.task_edges + 63) / 64) <= 4 * waves ? 64 : a.task_edges;
+    const int64_t per = (a.task_edges + piece - 1) / piece;  // pieces per slice
+    const int64_t npieces = ntasks * per;
+    for (int64_t t0 = (int64_t)blockIdx.x * (BLOCK / 64); t0 < npieces; t0 += waves) {
         const int64_t t = t0 + wid;
-        if (t < ntasks) {
-            const int sg = seg_of(s.st.tpref, t);
-            const td_task tk = a.tin[(int64_t)sg * a.tseg_cap + (t - s.st.tpref[sg])];
-            for (int64_t k0 = tk.beg; k0 < tk.end; k0 += 64) {
-                const int64_t k = k0 + lane;
-                const bool valid = k < tk.end;
-                const int32_t u = valid ? a.cols[k] : 0;
-                td_visit(a, s, tdw, valid, u, tk.src, cnt, dsum, dmax);
+        if (t < npieces) {
+            const int64_t ti = t / per;
+            const int sg = seg_of(s.st.tpref, ti);
+            const td_task tk = a.tin[(int64_t)sg * a.tseg_cap + (ti - s.st.tpref[sg])];
+            const int64_t beg = tk.beg + (t - ti * per) * piece;
+            const int64_t end = beg + piece < tk.end ? beg + piece : tk.end;
+            for (int64_t k0 = beg; k0 < end; k0 += 64 * kTdRounds) {
+                bool valid[kTdRounds];
+                int32_t u[kTdRounds], sr[kTdRounds];
+#pragma unroll
+                for (int r = 0; r < kTdRounds; ++r) {
+                    const int64_t k = k0 + 64 * r + lane;
+                    valid[r] = k < end;
+                    u[r] = valid[r] ? __builtin_nontemporal_load(a.cols + k) : 0;
+                    sr[r] = tk.src;
+                }
+                td_visit_n(a, s, tdw, valid, u, sr, cnt, dsum, dmax);
             }
         }
         __syncthreads();
@@ -466,7 +560,6 @@ __device__ void td_tasks(const step_args &a, step_smem<BLOCK> &s, bool tdw, unsi
     }
 }
 
-// Top-down step over the frontier lists (queue tiles, then task slices).
 template <int BLOCK>
 __device__ void td_body(const step_args &a, step_smem<BLOCK> &s, bool tdw, unsigned long long &cnt,
                         unsigned long long &dsum, unsigned long long &dmax) {
