@@ -4,8 +4,8 @@ The network predicts, for the state before a BFS step, the log2 cost in
 microseconds of a top-down step and of a bottom-up step; the learned policy
 takes the cheaper one (it replaces the predicates at hybrid_bfs.cc:167 and
 :178-179).  Training data are per-step timings the search logs itself (HIP
-events around each step; every step is also run once in the other direction
-inside an otherwise identical search, see log_steps) on
+events stamped at each step launch's dispatch; every step is also run once in
+the other direction inside an otherwise identical search, see log_steps) on
 seeded RMAT graphs of scales 18-22 with A in [0.45, 0.60] (B = C =
 (0.95 - A) / 2, D = 0.05), on generator seeds disjoint from the benchmark's
 (seed 0).  Training is full-batch Adam on the device (bd_mlp_train).
@@ -54,9 +54,11 @@ def log_steps(ctx, graph, roots_per_graph):
 
     Each root is searched once with Beamer's choices, then once per step with
     only that step's direction flipped (the frontier sequence does not depend
-    on the directions, so every run visits the same states).  Step times come
-    from HIP events around the step kernel with the caches as the previous
-    step left them.  A flipped run also times the following step after a
+    on the directions, so every run visits the same states).  The searches
+    run device-driven, as timed searches do, with an event pair stamped at
+    the dispatch of every step launch (bd_tuning.time_steps = 3); a step's
+    time is that of its launches, with the caches as the previous step left
+    them.  A flipped run also times the following step after a
     step of the other direction: that sample has one known target (masked).
     """
     ctx.generate_rmat(graph["scale"], graph["edge_factor"], graph["a"], graph["b"], graph["c"],
@@ -64,7 +66,7 @@ def log_steps(ctx, graph, roots_per_graph):
     nv, _, m = ctx.info()
     xs, ys, masks, meta = [], [], [], []
     tune = ctx.get_tuning()
-    ctx.set_tuning(time_steps=1)
+    ctx.set_tuning(time_steps=3)
     try:
         for root in ctx.pick_roots(roots_per_graph):
             root = int(root)
