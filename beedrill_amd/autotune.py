@@ -42,6 +42,7 @@ SPACE = {
     "block": [128, 256, 512],
     "step_kernels": [1, 2],
     "bu_head": [1, 2],
+    "td_split_edges": [1 << 20, 1 << 21, 1 << 22, 1 << 23, 2147483647],
 }
 
 
