@@ -1833,7 +1833,10 @@ int run_search(bd_ctx *c, int64_t source, int64_t alpha, int64_t beta, int polic
         const bool per_launch = c->tune.time_steps == 3;
         if (per_launch && (rc = ensure_event_pool(c, 2 * kMaxLevelsPerSync))) return rc;
         c->last_launch_ms.clear();
-        int chunk = 16;  // most searches end within the first chunk
+        // the first chunk: as many launches as the last search on this
+        // graph took, and a few more (16 for the first search); launches
+        // past the end return at once but still occupy the device
+        int chunk = c->chunk_hint > 0 ? c->chunk_hint : 16;
         for (;;) {
             for (int k = 0; k < chunk; ++k, ++L) {
                 const step_args a = make_args(c, L, alpha, beta, policy);
@@ -1871,6 +1874,12 @@ int run_search(bd_ctx *c, int64_t source, int64_t alpha, int64_t beta, int polic
     BD_HIP(c, hipEventElapsedTime(&ev_ms, c->ev[0], c->ev[1]));
     if ((rc = read_ctl(c))) return rc;
     if ((rc = finish_search(c, source, out, ev_ms))) return rc;
+    if (!c->last_log.empty() && (int64_t)c->last_log.size() == c->last_steps) {
+        const bd_level_rec &r = c->last_log.back();
+        // its level launches and the launch that found the frontier empty
+        const int64_t used = r.launch + r.launches + 1;
+        c->chunk_hint = (int)std::min<int64_t>(std::max<int64_t>(used + 3, 4), kMaxLevelsPerSync);
+    }
     if (c->tune.time_steps == 3) step_ms = c->last_launch_ms;  // per-launch times only
     for (auto &r : c->last_log) {
         if (r.launch < 0 || r.launch + r.launches > (int64_t)step_ms.size()) continue;

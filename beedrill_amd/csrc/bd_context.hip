@@ -60,6 +60,7 @@ static void free_search(bd_ctx *c) {
     c->ms_depth_rows = 0;
     c->task_cap = 0;
     c->have_result = false;
+    c->chunk_hint = 0;
 }
 
 void free_graph(bd_ctx *c) {

@@ -138,6 +138,7 @@ struct bd_ctx {
     bool depth_valid = false;  // d_depth holds the last search's depths
     bool split_off = false;    // profiling: top-down steps never run with remote writes
     bool cand_clean = false;   // d_cand is all -1 (set when a search finishes)
+    int chunk_hint = 0;        // first chunk of device-driven launches (0: 16)
     int64_t last_source = -1;
     int64_t last_steps = 0;
     std::vector<bd_level_rec> last_log;
