@@ -38,7 +38,7 @@ SPACE = {
     "grid_per_cu": [2, 3, 4, 5, 6, 8, 12],
     "wave_threshold": [16, 32, 64, 128, 256],
     "task_threshold": [16, 32, 48, 64, 96, 128, 256],
-    "task_edges": [64, 128, 192, 256, 384, 512],
+    "task_edges": [128, 256, 384, 768, 1536, 2048, 3072, 4096],
     "block": [128, 256, 512],
     "step_kernels": [1, 2],
     "bu_head": [1, 2],

@@ -528,10 +528,12 @@ __device__ void td_tasks(const step_args &a, step_smem<BLOCK> &s, bool tdw, unsi
     const int lane = lane_id(), wid = threadIdx.x >> 6;
     const int64_t ntasks = s.st.tpref[kSeg + 1];
     const int64_t waves = (int64_t)gridDim.x * (BLOCK / 64);
-    // A frontier of a few high-degree vertices has too few slices to occupy
-    // the grid, and each slice is a chain of dependent claims: then every
-    // slice is cut into 64-edge pieces, each taken by its own wave.
-    const int64_t piece = ntasks * ((a.task_edges + 63) / 64) <= 4 * waves ? 64 : a.task_edges;
+    // Slices are cut into pieces of a multiple of 64 edges, each taken by
+    // its own wave, sized so the frontier's edges make about four pieces
+    // per wave: a frontier of a few high-degree vertices has too few slices
+    // to occupy the grid, and each slice is a chain of dependent claims.
+    int64_t piece = (s.st.frontier_deg / (4 * waves) + 63) / 64 * 64;
+    piece = piece < 64 ? 64 : (piece > a.task_edges ? a.task_edges : piece);
     const int64_t per = (a.task_edges + piece - 1) / piece;  // pieces per slice
     const int64_t npieces = ntasks * per;
     for (int64_t t0 = (int64_t)blockIdx.x * (BLOCK / 64); t0 < npieces; t0 += waves) {

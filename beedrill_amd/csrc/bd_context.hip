@@ -150,7 +150,7 @@ extern "C" int bd_create(bd_ctx **out, int device) {
     c->tune.bu_cap = 32;
     c->tune.wave_threshold = 256;
     c->tune.task_threshold = 16;
-    c->tune.task_edges = 384;
+    c->tune.task_edges = 768;
     c->tune.level_driver = 1;
     c->tune.time_steps = 2;
     c->tune.bu_head = 1;
