@@ -59,6 +59,14 @@ namespace {
 constexpr int kMaxLevelsPerSync = 64;
 constexpr int kQCap = 2048;  // LDS-staged queue entries per workgroup
 constexpr int64_t kPrecheckEdges = 1 << 16;  // top-down steps this heavy pre-check claims
+// A top-down step over at least this many frontier edges builds no queue or
+// task list: its frontier is large enough that the next step, bottom-up as a
+// rule, reads the bitmap, and a top-down step after it ranks the bitmap.  On
+// a graph whose largest degree is under wave_t no step builds lists: the
+// bitmap's tiles take every vertex (there are no hubs to slice), and ranking
+// the bitmap costs less than staging, flushing and reading the lists (a
+// 4096 x 4096 lattice, 6278 levels: 68 ms against 92 ms).
+constexpr int64_t kListEdges = 1 << 14;
 // Top-down rounds of 64 edges a wave visits together (see td_visit_n).
 constexpr int kTdRounds = 2;
 
@@ -103,6 +111,7 @@ struct step_args {
     int64_t nv, m, nnz;
     int64_t alpha, beta;
     int64_t td_split;      // frontier edges from which top-down writes remotely
+    int64_t max_deg;       // the graph's largest degree
     int32_t level;
     int32_t close_only;    // only publish the state before this step
     int32_t force_dir;     // -1: ctl->dir; 0/1: run this direction (profiling)
@@ -241,6 +250,7 @@ __device__ void emit_tasks(const step_args &a, bool huge, int32_t u, int64_t b, 
 template <int BLOCK>
 __device__ __forceinline__ void emit_next(const step_args &a, step_smem<BLOCK> &s, bool emit, int32_t u,
                                           int64_t b, int64_t e) {
+    if (s.st.td_nolist) return;  // the frontier stays in the bitmap only
     const bool huge = emit && (e - b) >= a.task_t;
     if (__ballot(huge)) emit_tasks(a, huge, u, b, e);
     const bool q = emit && !huge;
@@ -979,10 +989,15 @@ __device__ void choose_dir(const step_args &a, step_smem<BLOCK> &s, ctl_block *c
             }
         }
     }
-    if (lane_id() == 0 && c->dir == BD_DIR_TD && a.policy != BD_POLICY_TD_ONLY &&
-        a.policy != BD_POLICY_REMOTE_WRITES_HYBRID && a.td_split > 0 &&
-        c->frontier_deg >= a.td_split)
-        c->dir = kDirTDWrite;
+    if (lane_id() == 0) {
+        if (c->dir == BD_DIR_TD && a.policy != BD_POLICY_TD_ONLY &&
+            a.policy != BD_POLICY_REMOTE_WRITES_HYBRID && a.td_split > 0 &&
+            c->frontier_deg >= a.td_split)
+            c->dir = kDirTDWrite;
+        c->td_nolist =
+            (c->dir == BD_DIR_TD && (c->frontier_deg >= kListEdges || a.max_deg < a.wave_t)) ? 1
+                                                                                             : 0;
+    }
     wave_lds_sync();
 }
 
@@ -1096,7 +1111,7 @@ __device__ bool rebuild_state(const step_args &a, step_smem<BLOCK> &s, bool publ
             c->prev_dir = sd;
             // after a bottom-up step or a claim sweep the frontier is only
             // in the bitmap
-            c->td_bitmap = kernel_dir(dir) == BD_DIR_BU ? 1 : 0;
+            c->td_bitmap = (kernel_dir(dir) == BD_DIR_BU || c->td_nolist) ? 1 : 0;
             c->level = (int32_t)c->steps;
             if (nnew == 0) c->done = 1;
             s.s_new = nnew;
@@ -1576,6 +1591,7 @@ step_args make_args(bd_ctx *c, int32_t level, int64_t alpha, int64_t beta, int p
     a.alpha = alpha;
     a.beta = beta;
     a.td_split = c->split_off ? 0 : (int64_t)c->tune.td_split_edges;
+    a.max_deg = c->max_deg;
     a.level = level;
     a.close_only = 0;
     a.force_dir = -1;

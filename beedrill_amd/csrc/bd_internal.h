@@ -76,6 +76,7 @@ struct ctl_block {
     int32_t done;
     int32_t td_bitmap;   // frontier L is only in bm[L%3] (step L-1 was BU)
     int32_t prev_dir;    // -1 before the first step
+    int32_t td_nolist;   // this top-down step leaves its frontier in the bitmap only
     int64_t frontier;
     int64_t frontier_deg;
     int64_t max_frontier_deg;
