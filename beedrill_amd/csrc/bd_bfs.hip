@@ -144,7 +144,6 @@ struct step_smem {
     long long s_new, s_new_deg;
     int s_dir;
     int s_split;  // the step closed was a top-down step with remote writes
-    int use_mlp;
     float x[8];
     unsigned long long red[4][BLOCK / 64];
     int32_t qn;
@@ -929,7 +928,6 @@ template <int BLOCK>
 __device__ void choose_dir(const step_args &a, step_smem<BLOCK> &s, ctl_block *c, int32_t last_dir,
                            int64_t nnew, int64_t nnew_deg, const mlp_wave_regs &w) {
     if (lane_id() == 0) {
-        s.use_mlp = 0;
         switch (a.policy) {
         case BD_POLICY_TD_ONLY:
             c->dir = BD_DIR_TD;
@@ -954,7 +952,6 @@ __device__ void choose_dir(const step_args &a, step_smem<BLOCK> &s, ctl_block *c
             // Beamer's choice and state, which the network may override
             beamer_next(c, last_dir, nnew, nnew_deg);
             if (a.has_mlp) {
-                s.use_mlp = 1;
                 mlp_features_dev(c->frontier, c->frontier_deg, c->max_frontier_deg, c->unvisited,
                                  c->unvisited_deg, c->level, c->nv, a.m, last_dir, s.x);
             }
@@ -963,8 +960,10 @@ __device__ void choose_dir(const step_args &a, step_smem<BLOCK> &s, ctl_block *c
             beamer_next(c, last_dir, nnew, nnew_deg);
         }
     }
-    wave_lds_sync();
-    if (s.use_mlp) {
+    // the network runs only for LEARNED with a network loaded (kernel
+    // arguments, the same in every lane: no LDS flag to wait for)
+    if (a.policy == BD_POLICY_LEARNED && a.has_mlp) {
+        wave_lds_sync();
         float y[2];
         mlp_eval(w, s.x, y);
         // y = predicted log2 cost of (top-down, bottom-up).  The network
