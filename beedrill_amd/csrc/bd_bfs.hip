@@ -110,6 +110,7 @@ struct step_args {
     int64_t qseg_cap, tseg_cap;
     int64_t nv, m, nnz;
     int64_t alpha, beta;
+    int64_t nv_beta;       // num_vertices() / beta (hybrid_bfs.cc:179)
     int64_t td_split;      // frontier edges from which top-down writes remotely
     int64_t max_deg;       // the graph's largest degree
     int32_t level;
@@ -899,11 +900,21 @@ __device__ void bu_body(const step_args &a, step_smem<BLOCK> &s, unsigned long l
 // a BU step the do-while test (:178-179) decides whether to stay bottom-up;
 // leaving sets scout = 1 (:180).  Then the outer test (:167) picks the
 // direction, charging scout to edges_to_check before a TD step (:182).
-__device__ void beamer_next(ctl_block *c, int32_t last_dir, int64_t nnew, int64_t nnew_deg) {
+// s > e / alpha with C division (alpha > 0), without a 64-bit divide when
+// s, e >= 0: floor(e / alpha) < s exactly when e < s * alpha.
+__device__ __forceinline__ bool gt_quotient(int64_t s, int64_t e, int64_t alpha) {
+    if (s < 0 || e < 0) return s > e / alpha;
+    long long p;
+    if (__builtin_mul_overflow((long long)s, (long long)alpha, &p)) return true;
+    return p > e;
+}
+
+__device__ void beamer_next(const step_args &a, ctl_block *c, int32_t last_dir, int64_t nnew,
+                            int64_t nnew_deg) {
     if (last_dir == BD_DIR_TD) {
         c->scout = nnew_deg;
     } else if (c->in_bu_loop) {
-        if (nnew >= c->old_awake || nnew > c->nv / c->beta) {
+        if (nnew >= c->old_awake || nnew > a.nv_beta) {
             c->old_awake = nnew;
             c->dir = BD_DIR_BU;
             return;
@@ -911,7 +922,7 @@ __device__ void beamer_next(ctl_block *c, int32_t last_dir, int64_t nnew, int64_
         c->in_bu_loop = 0;
         c->scout = 1;
     }
-    if (c->scout > c->edges_to_check / c->alpha) {
+    if (gt_quotient(c->scout, c->edges_to_check, c->alpha)) {
         c->in_bu_loop = 1;
         c->old_awake = c->frontier;
         c->dir = BD_DIR_BU;
@@ -940,7 +951,7 @@ __device__ void choose_dir(const step_args &a, step_smem<BLOCK> &s, ctl_block *c
             break;
         case BD_POLICY_REMOTE_WRITES_HYBRID:
             // Beamer's state machine; its bottom-up phase writes remotely
-            beamer_next(c, last_dir, nnew, nnew_deg);
+            beamer_next(a, c, last_dir, nnew, nnew_deg);
             if (c->dir == BD_DIR_BU) c->dir = kDirTDWrite;
             break;
         case BD_POLICY_SCRIPT: {
@@ -950,14 +961,14 @@ __device__ void choose_dir(const step_args &a, step_smem<BLOCK> &s, ctl_block *c
         }
         case BD_POLICY_LEARNED:
             // Beamer's choice and state, which the network may override
-            beamer_next(c, last_dir, nnew, nnew_deg);
+            beamer_next(a, c, last_dir, nnew, nnew_deg);
             if (a.has_mlp) {
                 mlp_features_dev(c->frontier, c->frontier_deg, c->max_frontier_deg, c->unvisited,
                                  c->unvisited_deg, c->level, c->nv, a.m, last_dir, s.x);
             }
             break;
         default:
-            beamer_next(c, last_dir, nnew, nnew_deg);
+            beamer_next(a, c, last_dir, nnew, nnew_deg);
         }
     }
     // the network runs only for LEARNED with a network loaded (kernel
@@ -1589,6 +1600,7 @@ step_args make_args(bd_ctx *c, int32_t level, int64_t alpha, int64_t beta, int p
     a.nnz = c->nnz;
     a.alpha = alpha;
     a.beta = beta;
+    a.nv_beta = c->nv / beta;
     a.td_split = c->split_off ? 0 : (int64_t)c->tune.td_split_edges;
     a.max_deg = c->max_deg;
     a.level = level;
