@@ -45,6 +45,14 @@ class LevelRec(ctypes.Structure):
         ("launch", ctypes.c_int64), ("launches", ctypes.c_int64)]
 
 
+class CCStats(ctypes.Structure):
+    _fields_ = [("num_components", ctypes.c_int64), ("num_iters", ctypes.c_int64),
+                ("ms", ctypes.c_double)]
+
+    def as_dict(self):
+        return {f: getattr(self, f) for f, _ in self._fields_}
+
+
 class RunStats(ctypes.Structure):
     _fields_ = [(n, ctypes.c_int64) for n in (
         "source", "steps", "td_steps", "bu_steps", "reached", "traversed_edges")] + [
@@ -120,6 +128,9 @@ def lib():
                               ctypes.c_int),
         "bd_bfs_multi_graph": ([vp, ctypes.c_int, _i64p, _i64p, vp,
                                 ctypes.POINTER(ctypes.c_double)], ctypes.c_int),
+        "bd_cc_run": ([vp, ctypes.POINTER(CCStats)], ctypes.c_int),
+        "bd_cc_fetch": ([vp, _i64p], ctypes.c_int),
+        "bd_cc_check": ([vp, ctypes.POINTER(ctypes.c_int)], ctypes.c_int),
         "bd_mlp_set": ([vp, _f32p, ctypes.c_int, ctypes.c_int], ctypes.c_int),
         "bd_mlp_get": ([vp, vp, i64, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)],
                        ctypes.c_int),
@@ -311,6 +322,25 @@ class Context:
     def check(self, source):
         ok = ctypes.c_int(0)
         self._ok(self._L.bd_bfs_check(self._h, int(source), ctypes.byref(ok)))
+        return bool(ok.value)
+
+    # ------------------------------------------------- connected components
+    def components(self):
+        """components::run (components.cc:38-104) on the resident graph:
+        {num_components, num_iters, ms}."""
+        st = CCStats()
+        self._ok(self._L.bd_cc_run(self._h, ctypes.byref(st)))
+        return st.as_dict()
+
+    def component_labels(self):
+        nv, _, _ = self.info()
+        lab = np.empty(nv, np.int64)
+        self._ok(self._L.bd_cc_fetch(self._h, lab))
+        return lab
+
+    def components_check(self):
+        ok = ctypes.c_int(0)
+        self._ok(self._L.bd_cc_check(self._h, ctypes.byref(ok)))
         return bool(ok.value)
 
     def get_tuning(self):

@@ -41,6 +41,11 @@ static void free_search(bd_ctx *c) {
     dev_free(c->d_depth);
     dev_free(c->d_parent);
     dev_free(c->d_cand);
+    dev_free(c->d_comp);
+    dev_free(c->d_cc_flag);
+    c->d_comp = nullptr;
+    c->d_cc_flag = nullptr;
+    c->have_cc = false;
     dev_free(c->d_vis);
     dev_free(c->d_vis_save);
     for (auto &b : c->d_bm) dev_free(b);

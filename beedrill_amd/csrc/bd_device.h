@@ -71,6 +71,10 @@ __device__ __forceinline__ unsigned long long wave_max(unsigned long long x) {
     return lane63(wave_scan64(x, [](uint64_t p, uint64_t q) { return p > q ? p : q; }));
 }
 
+__device__ __forceinline__ unsigned long long wave_min(unsigned long long x) {
+    return lane63(wave_scan64(x, [](uint64_t p, uint64_t q) { return p < q ? p : q; }));
+}
+
 __device__ __forceinline__ unsigned long long wave_or(unsigned long long x) {
     return lane63(wave_scan64(x, [](uint64_t p, uint64_t q) { return p | q; }));
 }

@@ -117,6 +117,9 @@ struct bd_ctx {
     int32_t *d_depth = nullptr;
     int32_t *d_parent = nullptr;
     int32_t *d_cand = nullptr;    // candidate parents of a remote-write top-down step (-1 = none)
+    int32_t *d_comp = nullptr;    // connected-component labels (components_)
+    int32_t *d_cc_flag = nullptr; // [0] an edge pass changed a label, [1..] counters
+    bool have_cc = false;         // d_comp holds a finished run
     uint64_t *d_vis = nullptr;
     uint64_t *d_vis_save = nullptr;
     uint64_t *d_bm[3] = {nullptr, nullptr, nullptr};

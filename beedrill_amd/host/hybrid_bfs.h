@@ -16,13 +16,13 @@ class hybrid_bfs {
     hybrid_bfs(const hybrid_bfs &other, emu::shallow_copy)
         : g_(other.g_), policy_(other.policy_), stats_(other.stats_), have_(other.have_) {}
 
-    // hybrid_bfs.cc:197-213: top-down steps only.
+    // hybrid_bfs.cc:197-213: top-down steps only, vertices claimed by atomics.
     void run_with_migrating_threads(long source);
-    // hybrid_bfs.cc:218-235: top-down steps only (the remote-write step is an
-    // Emu memory-system variant of the same top-down expansion).
+    // hybrid_bfs.cc:218-235: top-down steps with remote writes only (each a
+    // candidate-write launch and a claim sweep).
     void run_with_remote_writes(long source);
-    // hybrid_bfs.cc:243-277: every step is top-down, so on the GPU it runs
-    // the top-down kernels throughout.
+    // hybrid_bfs.cc:243-277: Beamer's state machine, with remote-write steps
+    // where Beamer goes bottom-up.
     void run_with_remote_writes_hybrid(long source, long alpha, long beta);
     // hybrid_bfs.cc:152-191.  The direction policy is Beamer's unless
     // set_policy() chose another.

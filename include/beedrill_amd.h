@@ -197,6 +197,26 @@ int bd_bfs_profile(bd_ctx *ctx, int64_t source, int64_t alpha, int64_t beta, int
  * reference counterpart: the measurement behind the step kernel's roofline. */
 int bd_bfs_launch_times(bd_ctx *ctx, double *ms, int64_t cap, int64_t *n);
 
+/* --------------------------------------------------- connected components */
+
+/* components::stats (components.h:27-32), plus the device time of the run. */
+typedef struct {
+    int64_t num_components;   /* components found                            */
+    int64_t num_iters;        /* edge passes, the last of which changed none */
+    double ms;                /* device time of run(): passes and the count  */
+} bd_cc_stats;
+
+/* components::run (components.cc:38-104) on the resident graph: label
+ * propagation (each vertex takes its neighbours' smallest label) and
+ * shortcutting until an edge pass changes nothing.  Each component ends
+ * labelled by its smallest vertex id. */
+int bd_cc_run(bd_ctx *ctx, bd_cc_stats *out);
+/* components::component_ (components.h:13-14) of the last run: labels[nv]. */
+int bd_cc_fetch(bd_ctx *ctx, int64_t *labels);
+/* components::check (components.cc:152-202), on the host from downloaded
+ * copies.  *ok = 1 PASS, 0 FAIL. */
+int bd_cc_check(bd_ctx *ctx, int *ok);
+
 /* Launch tuning (autotuner output).  Any field <= 0 keeps the default. */
 typedef struct {
     int32_t block;           /* threads per workgroup of the step kernel:

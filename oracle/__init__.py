@@ -98,6 +98,9 @@ def lib():
         L.oracle_mlp_loss_grad.restype = ctypes.c_float
         L.oracle_mlp_adam.argtypes = [_f32p, _f32p, _f32p, _f32p, ctypes.c_int64, ctypes.c_float,
                                       ctypes.c_int64]
+        L.oracle_cc.argtypes = [ctypes.c_int64, _i64p, _i32p, _i64p, ctypes.POINTER(ctypes.c_int64)]
+        L.oracle_cc.restype = ctypes.c_int64
+        L.oracle_cc_check.argtypes = [ctypes.c_int64, _i64p, _i32p, _i64p]
         _lib = L
     return _lib
 
@@ -317,3 +320,23 @@ def mlp_loss_grad(params, x, target, mask=None, h1=32, h2=32):
 
 def mlp_adam(params, grad, m, v, lr, t):
     lib().oracle_mlp_adam(params, np.ascontiguousarray(grad, np.float32), m, v, len(params), lr, t)
+
+
+# --------------------------------------------------------- connected components
+
+def components(off, adj):
+    """components::run (components.cc:38-104): (labels, num_components,
+    num_iters); each component is labelled by its smallest vertex id."""
+    nv = len(off) - 1
+    lab = np.empty(nv, np.int64)
+    n = ctypes.c_int64(0)
+    it = lib().oracle_cc(nv, np.ascontiguousarray(off, np.int64),
+                         np.ascontiguousarray(adj, np.int32), lab, ctypes.byref(n))
+    return lab, n.value, it
+
+
+def components_check(off, adj, labels) -> bool:
+    """components::check (components.cc:152-202)."""
+    return bool(lib().oracle_cc_check(len(off) - 1, np.ascontiguousarray(off, np.int64),
+                                      np.ascontiguousarray(adj, np.int32),
+                                      np.ascontiguousarray(labels, np.int64)))

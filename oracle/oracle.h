@@ -30,6 +30,10 @@
  *          reference's std::random_shuffle under _GLIBCXX_PARALLEL is
  *          thread-count dependent); bijection and multiset tests.
  *   oracle_mlp_*         -> finite differences and numpy.
+ *   oracle_cc            -> scipy.sparse.csgraph.connected_components (the
+ *                           same partition, each part labelled by its
+ *                           smallest id) and closed forms on special graphs.
+ *   oracle_cc_check      -> planted wrong labels must FAIL.
  */
 #ifndef BEEDRILL_ORACLE_H
 #define BEEDRILL_ORACLE_H
@@ -171,6 +175,14 @@ void oracle_aggregate(const double *x, int64_t n, double out[6]);
 /* Plant a known mistake in the BFS restatement (0 = none; ids documented in
  * oracle_bfs.c).  Tests use it to show their known answers catch each one. */
 void oracle_set_mutation(int id);
+
+/* ---------------- connected components (components.cc) */
+/* components::run (components.cc:38-104): labels[nv] (each component carries
+ * its smallest vertex id), *ncomp; returns the iteration count. */
+int64_t oracle_cc(int64_t nv, const int64_t *off, const int32_t *adj, int64_t *label,
+                  int64_t *ncomp);
+/* components::check (components.cc:152-202).  1 = PASS, 0 = FAIL. */
+int oracle_cc_check(int64_t nv, const int64_t *off, const int32_t *adj, const int64_t *label);
 
 /* ---------------- direction-cost MLP (no reference counterpart)
  * 8 -> H1 -> H2 -> 2, ReLU, fp32.  Parameter layout (row-major, out x in):

@@ -39,8 +39,9 @@ def test_no_gpu_means_no_context():
         bd.Context(0)
 
 
-def test_cli_binary_built():
-    exe = os.path.join(ROOT, "beedrill_amd", "bin", "hybrid_bfs")
+@pytest.mark.parametrize("name", ["hybrid_bfs", "components"])
+def test_cli_binary_built(name):
+    exe = os.path.join(ROOT, "beedrill_amd", "bin", name)
     assert os.path.exists(exe), "CLI not built"
     r = subprocess.run([exe, "--help"], capture_output=True, text=True)
     assert r.returncode == 1 and "--graph_filename" in r.stdout
