@@ -13,6 +13,13 @@
 // iteration count includes that pass.  Labels only decrease and stay ids of
 // the vertex's own component, so each component ends labelled by its
 // smallest vertex id, whatever order the passes read the labels in.
+//
+// A pass reads a neighbour's label only if it changed in the previous pass
+// or shortcut (a bitmap of changed vertices; the first pass reads all):
+// an unchanged label was read by the vertex before, when it last changed,
+// and the vertex's own label has only gone down since, so it cannot lower
+// it.  Skipping it changes no pass's outcome, only its HBM traffic.
+#include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 
 #include <vector>
@@ -23,68 +30,198 @@
 namespace bd {
 namespace {
 
-__global__ void k_cc_init(int32_t *__restrict__ comp, int64_t nv) {
+// Every vertex its own component; every vertex "changed" for the first pass.
+__global__ void k_cc_init(int32_t *__restrict__ comp, int64_t nv, uint64_t *__restrict__ chg_all,
+                          uint64_t *__restrict__ chg_none) {
     for (int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v < nv;
-         v += (int64_t)gridDim.x * blockDim.x)
+         v += (int64_t)gridDim.x * blockDim.x) {
         comp[v] = (int32_t)v;
+        if ((v & 63) == 0) {
+            chg_all[v >> 6] = ~0ull;
+            chg_none[v >> 6] = 0;
+        }
+    }
 }
 
+// Lists of kCcHeavy edges or more are cut into slices of kCcSlice edges,
+// one wave each, merged into the label with atomicMin; shorter lists are
+// walked by the wave that owns their vertex.
+constexpr int kCcHeavy = 256;
+constexpr int kCcSlice = 2048;
+
+__device__ __forceinline__ int32_t shfl(int32_t x, int src) {
+    return __builtin_amdgcn_ds_bpermute(src << 2, x);
+}
+__device__ __forceinline__ int64_t shfl(int64_t x, int src) {
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_ds_bpermute(src << 2, (int)(uint32_t)(uint64_t)x);
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_ds_bpermute(src << 2, (int)(uint32_t)((uint64_t)x >> 32));
+    return (int64_t)(((uint64_t)hi << 32) | lo);
+}
+
+// Slices of the heavy lists; built once per graph.  *count ends as the
+// number of slices the graph needs, even past cap (then only cap are kept).
+__global__ void k_cc_tasks(const int64_t *__restrict__ off, int64_t nv, td_task *__restrict__ tasks,
+                           unsigned long long *__restrict__ count, int64_t cap) {
+    for (int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v < nv;
+         v += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t b = off[v], e = off[v + 1];
+        if (e - b < kCcHeavy) continue;
+        const int64_t n = (e - b + kCcSlice - 1) / kCcSlice;
+        const int64_t at = (int64_t)atomicAdd(count, (unsigned long long)n);
+        for (int64_t i = 0; i < n && at + i < cap; ++i) {
+            td_task t;
+            t.beg = b + i * kCcSlice;
+            t.end = t.beg + kCcSlice < e ? t.beg + kCcSlice : e;
+            t.src = (int32_t)v;
+            t.pad = 0;
+            tasks[at + i] = t;
+        }
+    }
+}
+
+// One edge pass.  Units [0, ntasks) are heavy slices; unit ntasks + g is
+// the vertex group [64g, 64g + 64), whose light lists are consecutive in
+// cols (heavy lists between them are skipped).  The group's light edges are
+// numbered q = 0..T-1 (light vertex i owns q in [s_i, s_i + d_i)) and taken
+// 64 at a time, one per lane, so every lane loads whatever the degrees:
+//   owner   o(q) = the last i with s_i <= q   (binary search over lanes)
+//   edge    k    = off[base + o] + q - s_o
+//   label   l    = comp[cols[k]]
+// Owners are non-decreasing across the lanes of a chunk, so a suffix min
+// restricted to equal owners leaves the chunk's min for owner i in the
+// lane where i's run starts, max(s_i - t, 0), which lane i reads.
+// Labels of unchanged neighbours are not read (chg_in, all ones in the
+// first pass); a vertex whose label goes down is set in chg_out.  changed[0]
+// is set if any label went down; labels_read counts the labels gathered.
 __global__ __launch_bounds__(256) void k_cc_hook(const int64_t *__restrict__ off,
                                                  const int32_t *__restrict__ cols, int32_t *comp,
-                                                 int64_t nv, int32_t *__restrict__ changed) {
+                                                 int64_t nv, const td_task *__restrict__ tasks,
+                                                 int64_t ntasks, const uint64_t *__restrict__ chg_in,
+                                                 uint64_t *__restrict__ chg_out,
+                                                 int32_t *__restrict__ changed,
+                                                 unsigned long long *__restrict__ labels_read) {
     const int lane = threadIdx.x & 63;
     const int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
     const int64_t nwaves = ((int64_t)gridDim.x * blockDim.x) >> 6;
+    const int64_t nunits = ntasks + (nv + 63) / 64;
     bool ch = false;
-    for (int64_t base = wave * 64; base < nv; base += nwaves * 64) {
-        const int64_t v = base + lane;
-        const bool valid = v < nv;
+    unsigned long long nread = 0;
+    for (int64_t u = wave; u < nunits; u += nwaves) {
+        if (u < ntasks) {
+            const int64_t beg = tasks[u].beg, end = tasks[u].end;
+            const int32_t src = tasks[u].src;
+            uint32_t m = 0xffffffffu;
+            for (int64_t k0 = beg + lane; k0 < end; k0 += 4 * 64) {
+                int32_t c[4];
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    const int64_t k = k0 + r * 64;
+                    c[r] = __builtin_nontemporal_load(cols + (k < end ? k : end - 1));
+                }
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    const bool live = k0 + r * 64 < end && ((chg_in[c[r] >> 6] >> (c[r] & 63)) & 1);
+                    if (live) {
+                        const uint32_t l = (uint32_t)comp[c[r]];
+                        m = l < m ? l : m;
+                        ++nread;
+                    }
+                }
+            }
+            m = (uint32_t)wave_min(m);
+            // m stays 0xffffffff when no neighbour in the slice changed
+            if (lane == 0 && m != 0xffffffffu && (int32_t)m < comp[src]) {
+                if ((int32_t)m < atomicMin(comp + src, (int32_t)m)) {
+                    ch = true;
+                    atomicOr((unsigned long long *)chg_out + (src >> 6), 1ull << (src & 63));
+                }
+            }
+            continue;
+        }
+        const int64_t v = (u - ntasks) * 64 + lane;
         int64_t b = 0, e = 0;
-        int32_t mine = 0;
-        if (valid) {
+        int32_t mine = 0x7fffffff;
+        if (v < nv) {
             b = off[v];
             e = off[v + 1];
             mine = comp[v];
         }
-        const bool alone = valid && e - b < 64;
-        if (alone) {
-            int32_t m = mine;
-            for (int64_t k = b; k < e; ++k) {
-                const int32_t l = comp[__builtin_nontemporal_load(cols + k)];
-                m = l < m ? l : m;
+        const int32_t d = e - b < kCcHeavy ? (int32_t)(e - b) : 0;
+        const int32_t s_end = wave_incl_scan32(d);
+        const int32_t s = s_end - d;
+        const int32_t total = __builtin_amdgcn_readlane(s_end, 63);
+        const int64_t base_k = b - s;  // k = base_k of the owner + q
+        int32_t acc = mine;
+        for (int32_t t = 0; t < total; t += 64) {
+            const int32_t q = t + lane;
+            const int32_t qa = q < total ? q : total - 1;
+            int o = 0;
+#pragma unroll
+            for (int step = 32; step; step >>= 1)
+                if (shfl(s, o + step) <= qa) o += step;
+            const int64_t k = shfl(base_k, o) + qa;
+            // one past the owner's last lane in this chunk
+            const int32_t run_end = min(shfl(s_end, o) - t, 64);
+            const int32_t c = __builtin_nontemporal_load(cols + k);
+            const bool live = q < total && ((chg_in[c >> 6] >> (c & 63)) & 1);
+            int32_t m = 0x7fffffff;
+            if (live) {
+                m = comp[c];
+                ++nread;
             }
-            if (m < mine) {
-                comp[v] = m;
-                ch = true;
+#pragma unroll
+            for (int step = 1; step < 64; step <<= 1) {
+                const int32_t other = shfl(m, (lane + step) & 63);
+                if (lane + step < run_end && other < m) m = other;
             }
+            const bool mine_here = d > 0 && s < t + 64 && s_end > t;
+            const int32_t got = shfl(m, mine_here ? (s > t ? s - t : 0) : lane);
+            if (mine_here && got < acc) acc = got;
         }
-        uint64_t lm = __ballot(valid && !alone);
-        while (lm) {
-            const int src = __ffsll((unsigned long long)lm) - 1;
-            lm &= lm - 1;
-            const int64_t kb = lane_bcast(b, src), ke = lane_bcast(e, src);
-            unsigned long long m = 0xffffffffull;
-            for (int64_t k = kb + lane; k < ke; k += 64) {
-                const unsigned long long l =
-                    (uint32_t)comp[__builtin_nontemporal_load(cols + k)];
-                m = l < m ? l : m;
-            }
-            m = wave_min(m);
-            if (lane == src && (int32_t)m < mine) {
-                comp[v] = (int32_t)m;
-                ch = true;
-            }
+        const bool down = acc < mine;
+        if (down) comp[v] = acc;
+        const uint64_t downs = __ballot(down);
+        if (downs && lane == 0) {
+            atomicOr((unsigned long long *)chg_out + (u - ntasks), (unsigned long long)downs);
+            ch = true;
         }
     }
     if (__ballot(ch) && lane == 0) changed[0] = 1;
+    nread = wave_sum(nread);
+    if (lane == 0 && nread) atomicAdd(&labels_read[(blockIdx.x % kSlots) * kLine], nread);
 }
 
-__global__ void k_cc_shortcut(int32_t *comp, int64_t nv) {
+// Shortcut (:74-78): each label becomes its root.  Vertices whose label
+// moves are added to chg (the next pass's input), and chg_done (this pass's
+// input, read by nobody any more) is cleared for the pass after.  Waves
+// cover 64 aligned vertices, one bitmap word.  A label is never above its
+// vertex's id, so chains strictly descend to a root; a label outside
+// [0, v] stops the chase and sets *bad instead of being followed.
+__global__ __launch_bounds__(256) void k_cc_shortcut(int32_t *comp, int64_t nv,
+                                                     uint64_t *__restrict__ chg,
+                                                     uint64_t *__restrict__ chg_done,
+                                                     int32_t *__restrict__ bad) {
     for (int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v < nv;
          v += (int64_t)gridDim.x * blockDim.x) {
-        int32_t c = comp[v];
-        for (int32_t cc = comp[c]; cc != c; cc = comp[c]) c = cc;
-        comp[v] = c;
+        const int32_t c0 = comp[v];
+        int32_t c = c0;
+        bool ok = c >= 0 && c <= v;
+        while (ok) {
+            const int32_t cc = comp[c];
+            if (cc == c) break;
+            ok = cc >= 0 && cc < c;
+            if (ok) c = cc;
+        }
+        if (!ok) {
+            *bad = 1;
+            c = c0;
+        }
+        if (c != c0) comp[v] = c;
+        const uint64_t moved = __ballot(c != c0);
+        if ((v & 63) == 0) {
+            if (moved) chg[v >> 6] |= moved;
+            chg_done[v >> 6] = 0;
+        }
     }
 }
 
@@ -108,52 +245,93 @@ extern "C" int bd_cc_run(bd_ctx *c, bd_cc_stats *out) {
     if (!c) return BD_EINVAL;
     if (!c->d_off) return set_err(c, BD_ESTATE, "no graph loaded");
     BD_HIP(c, hipSetDevice(c->device));
-    const int64_t nv = c->nv;
+    const int64_t nv = c->nv, nw = (nv + 63) / 64;
     int rc;
     if (!c->d_comp && (rc = dev_alloc(c, (void **)&c->d_comp, sizeof(int32_t) * (nv + 1), "labels")))
         return rc;
-    const size_t flag_bytes = sizeof(unsigned long long) * (kSlots * kLine + 1);
+    if (!c->d_cc_chg &&
+        (rc = dev_alloc(c, (void **)&c->d_cc_chg, sizeof(uint64_t) * 2 * nw, "changed-label bitmaps")))
+        return rc;
+    // [0] changed flag (int32), then two kSlots x kLine counter blocks:
+    // components and labels read
+    const size_t flag_bytes = sizeof(unsigned long long) * (2 * kSlots * kLine + 1);
     if (!c->d_cc_flag && (rc = dev_alloc(c, (void **)&c->d_cc_flag, flag_bytes, "label flags")))
         return rc;
     c->have_cc = false;
     int32_t *changed = c->d_cc_flag;
     unsigned long long *count = (unsigned long long *)(c->d_cc_flag + 2);
+    unsigned long long *nread = count + kSlots * kLine;
+    uint64_t *chg[2] = {c->d_cc_chg, c->d_cc_chg + nw};
     const dim3 grid(grid_for(c, nv, 256)), block(256);
+    if (c->cc_ntasks < 0) {
+        // slices of the long lists, once per graph (like the graph's own
+        // indexes, outside the timed run)
+        const int64_t cap = c->nnz / kCcSlice + c->nnz / kCcHeavy + 1;
+        if ((rc = dev_alloc(c, (void **)&c->d_cc_tasks, sizeof(td_task) * cap, "component slices")))
+            return rc;
+        BD_HIP(c, hipMemsetAsync(count, 0, sizeof(unsigned long long), c->stream));
+        hipLaunchKernelGGL(k_cc_tasks, grid, block, 0, c->stream, c->d_off, nv, c->d_cc_tasks, count,
+                           cap);
+        BD_HIP(c, hipGetLastError());
+        unsigned long long n = 0;
+        BD_HIP(c, hipMemcpyAsync(&n, count, sizeof(n), hipMemcpyDeviceToHost, c->stream));
+        BD_HIP(c, hipStreamSynchronize(c->stream));
+        if ((int64_t)n > cap) return set_err(c, BD_EHIP, "component slices overflow");
+        c->cc_ntasks = (int64_t)n;
+    }
+    const int64_t nunits = c->cc_ntasks + nw;
+    const dim3 hook_grid(grid_for(c, nunits * 64, 256));
     BD_HIP(c, hipEventRecord(c->ev[0], c->stream));
-    hipLaunchKernelGGL(k_cc_init, grid, block, 0, c->stream, c->d_comp, nv);
+    BD_HIP(c, hipMemsetAsync(changed, 0, sizeof(int32_t) * 2, c->stream));
+    BD_HIP(c, hipMemsetAsync(count, 0, sizeof(unsigned long long) * 2 * kSlots * kLine, c->stream));
+    // pass i reads chg[i & 1] and sets chg[(i + 1) & 1]
+    hipLaunchKernelGGL(k_cc_init, grid, block, 0, c->stream, c->d_comp, nv, chg[1], chg[0]);
     BD_HIP(c, hipGetLastError());
     int64_t iters = 0;
+    double pass_ms = 0;
     for (;;) {
         ++iters;
         if (iters > nv + 2) return set_err(c, BD_EHIP, "components did not converge");
-        BD_HIP(c, hipMemsetAsync(changed, 0, sizeof(int32_t), c->stream));
-        hipLaunchKernelGGL(k_cc_hook, grid, block, 0, c->stream, c->d_off, c->d_cols, c->d_comp, nv,
-                           changed);
+        uint64_t *in = chg[iters & 1], *next = chg[(iters + 1) & 1];
+        BD_HIP(c, hipMemsetAsync(changed, 0, sizeof(int32_t), c->stream));  // [1] stays
+        hipExtLaunchKernelGGL(k_cc_hook, hook_grid, block, 0, c->stream, c->ev[2], c->ev[3], 0,
+                              (const int64_t *)c->d_off, (const int32_t *)c->d_cols, c->d_comp, nv,
+                              (const td_task *)c->d_cc_tasks, c->cc_ntasks, (const uint64_t *)in,
+                              next, changed, nread);
         BD_HIP(c, hipGetLastError());
-        int32_t h = 0;
-        BD_HIP(c, hipMemcpyAsync(&h, changed, sizeof(h), hipMemcpyDeviceToHost, c->stream));
+        int32_t h[2] = {0, 0};
+        BD_HIP(c, hipMemcpyAsync(h, changed, sizeof(h), hipMemcpyDeviceToHost, c->stream));
         BD_HIP(c, hipStreamSynchronize(c->stream));
-        if (!h) break;
-        hipLaunchKernelGGL(k_cc_shortcut, grid, block, 0, c->stream, c->d_comp, nv);
+        if (h[1]) return set_err(c, BD_EHIP, "component label out of range");
+        float kms = 0;
+        BD_HIP(c, hipEventElapsedTime(&kms, c->ev[2], c->ev[3]));
+        pass_ms += kms;
+        if (!h[0]) break;
+        hipLaunchKernelGGL(k_cc_shortcut, grid, block, 0, c->stream, c->d_comp, nv, next, in,
+                           changed + 1);
         BD_HIP(c, hipGetLastError());
     }
-    BD_HIP(c, hipMemsetAsync(count, 0, sizeof(unsigned long long) * kSlots * kLine, c->stream));
     hipLaunchKernelGGL(k_cc_count, grid, block, 0, c->stream, c->d_comp, nv, count);
     BD_HIP(c, hipGetLastError());
     BD_HIP(c, hipEventRecord(c->ev[1], c->stream));
-    std::vector<unsigned long long> h(kSlots * kLine);
+    std::vector<unsigned long long> h(2 * kSlots * kLine);
     BD_HIP(c, hipMemcpyAsync(h.data(), count, sizeof(unsigned long long) * h.size(),
                              hipMemcpyDeviceToHost, c->stream));
     BD_HIP(c, hipStreamSynchronize(c->stream));
     float ms = 0;
     BD_HIP(c, hipEventElapsedTime(&ms, c->ev[0], c->ev[1]));
-    unsigned long long ncomp = 0;
-    for (int s = 0; s < kSlots; ++s) ncomp += h[s * kLine];
+    unsigned long long ncomp = 0, nlab = 0;
+    for (int s = 0; s < kSlots; ++s) {
+        ncomp += h[s * kLine];
+        nlab += h[(kSlots + s) * kLine];
+    }
     c->have_cc = true;
     if (out) {
         out->num_components = (int64_t)ncomp;
         out->num_iters = iters;
         out->ms = ms;
+        out->pass_ms = pass_ms;
+        out->labels_read = (int64_t)nlab;
     }
     return BD_OK;
 }

@@ -43,8 +43,13 @@ static void free_search(bd_ctx *c) {
     dev_free(c->d_cand);
     dev_free(c->d_comp);
     dev_free(c->d_cc_flag);
+    dev_free(c->d_cc_tasks);
+    dev_free(c->d_cc_chg);
+    c->d_cc_chg = nullptr;
     c->d_comp = nullptr;
     c->d_cc_flag = nullptr;
+    c->d_cc_tasks = nullptr;
+    c->cc_ntasks = -1;
     c->have_cc = false;
     dev_free(c->d_vis);
     dev_free(c->d_vis_save);

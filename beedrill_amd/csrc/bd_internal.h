@@ -120,6 +120,9 @@ struct bd_ctx {
     int32_t *d_comp = nullptr;    // connected-component labels (components_)
     int32_t *d_cc_flag = nullptr; // [0] an edge pass changed a label, [1..] counters
     bool have_cc = false;         // d_comp holds a finished run
+    uint64_t *d_cc_chg = nullptr;       // two bitmaps of vertices whose label changed
+    bd::td_task *d_cc_tasks = nullptr;  // slices of the long lists of the edge pass
+    int64_t cc_ntasks = -1;             // -1: not built for the resident graph
     uint64_t *d_vis = nullptr;
     uint64_t *d_vis_save = nullptr;
     uint64_t *d_bm[3] = {nullptr, nullptr, nullptr};

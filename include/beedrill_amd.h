@@ -199,17 +199,21 @@ int bd_bfs_launch_times(bd_ctx *ctx, double *ms, int64_t cap, int64_t *n);
 
 /* --------------------------------------------------- connected components */
 
-/* components::stats (components.h:27-32), plus the device time of the run. */
+/* components::stats (components.h:27-32), plus the device time of the run
+ * and of its edge passes. */
 typedef struct {
     int64_t num_components;   /* components found                            */
     int64_t num_iters;        /* edge passes, the last of which changed none */
     double ms;                /* device time of run(): passes and the count  */
+    double pass_ms;           /* kernel time of the edge passes alone        */
+    int64_t labels_read;      /* neighbour labels the edge passes read       */
 } bd_cc_stats;
 
 /* components::run (components.cc:38-104) on the resident graph: label
  * propagation (each vertex takes its neighbours' smallest label) and
  * shortcutting until an edge pass changes nothing.  Each component ends
- * labelled by its smallest vertex id. */
+ * labelled by its smallest vertex id.  A pass skips the labels of
+ * neighbours that did not change since the vertex last read them. */
 int bd_cc_run(bd_ctx *ctx, bd_cc_stats *out);
 /* components::component_ (components.h:13-14) of the last run: labels[nv]. */
 int bd_cc_fetch(bd_ctx *ctx, int64_t *labels);

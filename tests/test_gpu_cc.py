@@ -20,6 +20,9 @@ def compare(ctx, orc, off, adj):
     np.testing.assert_array_equal(lab, ref)
     assert st["num_components"] == n
     assert st["num_iters"] >= 1 and st["ms"] > 0
+    assert 0 < st["pass_ms"] <= st["ms"]
+    # the first pass reads every neighbour label, later ones only changed ones
+    assert len(adj) <= st["labels_read"] <= st["num_iters"] * len(adj)
     assert ctx.components_check()
     return st
 
@@ -59,6 +62,27 @@ def test_rmat(ctx, orc, scale, ef, seed):
     ctx.generate_rmat(scale, ef, seed=seed)
     off, adj = ctx.csr()
     compare(ctx, orc, off, adj)
+
+
+def test_long_lists_and_slices(ctx, orc):
+    # lists of 255 and 256 edges (either side of the long-list cut), hubs of
+    # several 2048-edge slices, long and short lists in one 64-vertex group,
+    # and a group whose vertices all have long lists
+    rng = np.random.default_rng(11)
+    nv = 40000
+    src, dst = [], []
+    for hub, deg in [(5, 255), (6, 256), (7, 257), (70, 5000), (71, 2048), (72, 2049),
+                     (20000, 9000)] + [(128 + i, 300 + i) for i in range(64)]:
+        nb = rng.choice(np.arange(30000, nv), deg, replace=False)
+        src.append(np.full(deg, hub)); dst.append(nb)
+    a = rng.integers(1000, 30000, 6000)
+    src.append(a); dst.append(a + 1)  # short lists elsewhere
+    s = np.concatenate(src).astype(np.int64)
+    d = np.concatenate(dst).astype(np.int64)
+    ctx.from_edges(nv, s, d)
+    off, adj = orc.build_csr(nv, s, d)
+    compare(ctx, orc, off, adj)
+    compare(ctx, orc, off, adj)  # second run reuses the slices
 
 
 def test_er_and_lattice(ctx, orc):
