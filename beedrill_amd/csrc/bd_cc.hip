@@ -14,11 +14,14 @@
 // the vertex's own component, so each component ends labelled by its
 // smallest vertex id, whatever order the passes read the labels in.
 //
-// A pass reads a neighbour's label only if it changed in the previous pass
-// or shortcut (a bitmap of changed vertices; the first pass reads all):
+// Once few labels change, a pass reads a neighbour's label only if it
+// changed in the previous pass or shortcut (a bitmap of changed vertices):
 // an unchanged label was read by the vertex before, when it last changed,
 // and the vertex's own label has only gone down since, so it cannot lower
-// it.  Skipping it changes no pass's outcome, only its HBM traffic.
+// it.  Skipping it changes no pass's outcome, only its HBM traffic.  The
+// bitmap lookup costs a random L2 read per edge, so passes that follow many
+// changes (the edges of changed vertices at least half of all edges) read
+// every label without it.
 #include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 
@@ -90,22 +93,25 @@ __global__ void k_cc_tasks(const int64_t *__restrict__ off, int64_t nv, td_task 
 // Owners are non-decreasing across the lanes of a chunk, so a suffix min
 // restricted to equal owners leaves the chunk's min for owner i in the
 // lane where i's run starts, max(s_i - t, 0), which lane i reads.
-// Labels of unchanged neighbours are not read (chg_in, all ones in the
-// first pass); a vertex whose label goes down is set in chg_out.  changed[0]
-// is set if any label went down; labels_read counts the labels gathered.
+// With all == 0, labels of neighbours not in chg_in are not read.  A vertex
+// whose label goes down is set in chg_out and its degree (a slice's length
+// for a long list) added to chg_deg; changed[0] is set if any label went
+// down; labels_read counts the labels gathered.
 __global__ __launch_bounds__(256) void k_cc_hook(const int64_t *__restrict__ off,
                                                  const int32_t *__restrict__ cols, int32_t *comp,
                                                  int64_t nv, const td_task *__restrict__ tasks,
-                                                 int64_t ntasks, const uint64_t *__restrict__ chg_in,
+                                                 int64_t ntasks, int all,
+                                                 const uint64_t *__restrict__ chg_in,
                                                  uint64_t *__restrict__ chg_out,
                                                  int32_t *__restrict__ changed,
-                                                 unsigned long long *__restrict__ labels_read) {
+                                                 unsigned long long *__restrict__ labels_read,
+                                                 unsigned long long *__restrict__ chg_deg) {
     const int lane = threadIdx.x & 63;
     const int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
     const int64_t nwaves = ((int64_t)gridDim.x * blockDim.x) >> 6;
     const int64_t nunits = ntasks + (nv + 63) / 64;
     bool ch = false;
-    unsigned long long nread = 0;
+    unsigned long long nread = 0, ndeg = 0;
     for (int64_t u = wave; u < nunits; u += nwaves) {
         if (u < ntasks) {
             const int64_t beg = tasks[u].beg, end = tasks[u].end;
@@ -120,7 +126,8 @@ __global__ __launch_bounds__(256) void k_cc_hook(const int64_t *__restrict__ off
                 }
 #pragma unroll
                 for (int r = 0; r < 4; ++r) {
-                    const bool live = k0 + r * 64 < end && ((chg_in[c[r] >> 6] >> (c[r] & 63)) & 1);
+                    const bool live =
+                        k0 + r * 64 < end && (all || ((chg_in[c[r] >> 6] >> (c[r] & 63)) & 1));
                     if (live) {
                         const uint32_t l = (uint32_t)comp[c[r]];
                         m = l < m ? l : m;
@@ -133,6 +140,7 @@ __global__ __launch_bounds__(256) void k_cc_hook(const int64_t *__restrict__ off
             if (lane == 0 && m != 0xffffffffu && (int32_t)m < comp[src]) {
                 if ((int32_t)m < atomicMin(comp + src, (int32_t)m)) {
                     ch = true;
+                    ndeg += (unsigned long long)(end - beg);
                     atomicOr((unsigned long long *)chg_out + (src >> 6), 1ull << (src & 63));
                 }
             }
@@ -163,7 +171,7 @@ __global__ __launch_bounds__(256) void k_cc_hook(const int64_t *__restrict__ off
             // one past the owner's last lane in this chunk
             const int32_t run_end = min(shfl(s_end, o) - t, 64);
             const int32_t c = __builtin_nontemporal_load(cols + k);
-            const bool live = q < total && ((chg_in[c >> 6] >> (c & 63)) & 1);
+            const bool live = q < total && (all || ((chg_in[c >> 6] >> (c & 63)) & 1));
             int32_t m = 0x7fffffff;
             if (live) {
                 m = comp[c];
@@ -179,7 +187,10 @@ __global__ __launch_bounds__(256) void k_cc_hook(const int64_t *__restrict__ off
             if (mine_here && got < acc) acc = got;
         }
         const bool down = acc < mine;
-        if (down) comp[v] = acc;
+        if (down) {
+            comp[v] = acc;
+            ndeg += (unsigned long long)d;
+        }
         const uint64_t downs = __ballot(down);
         if (downs && lane == 0) {
             atomicOr((unsigned long long *)chg_out + (u - ntasks), (unsigned long long)downs);
@@ -188,7 +199,9 @@ __global__ __launch_bounds__(256) void k_cc_hook(const int64_t *__restrict__ off
     }
     if (__ballot(ch) && lane == 0) changed[0] = 1;
     nread = wave_sum(nread);
+    ndeg = wave_sum(ndeg);
     if (lane == 0 && nread) atomicAdd(&labels_read[(blockIdx.x % kSlots) * kLine], nread);
+    if (lane == 0 && ndeg) atomicAdd(&chg_deg[(blockIdx.x % kSlots) * kLine], ndeg);
 }
 
 // Shortcut (:74-78): each label becomes its root.  Vertices whose label
@@ -197,10 +210,12 @@ __global__ __launch_bounds__(256) void k_cc_hook(const int64_t *__restrict__ off
 // cover 64 aligned vertices, one bitmap word.  A label is never above its
 // vertex's id, so chains strictly descend to a root; a label outside
 // [0, v] stops the chase and sets *bad instead of being followed.
-__global__ __launch_bounds__(256) void k_cc_shortcut(int32_t *comp, int64_t nv,
-                                                     uint64_t *__restrict__ chg,
+__global__ __launch_bounds__(256) void k_cc_shortcut(const int64_t *__restrict__ off, int32_t *comp,
+                                                     int64_t nv, uint64_t *__restrict__ chg,
                                                      uint64_t *__restrict__ chg_done,
-                                                     int32_t *__restrict__ bad) {
+                                                     int32_t *__restrict__ bad,
+                                                     unsigned long long *__restrict__ chg_deg) {
+    unsigned long long ndeg = 0;
     for (int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v < nv;
          v += (int64_t)gridDim.x * blockDim.x) {
         const int32_t c0 = comp[v];
@@ -216,13 +231,18 @@ __global__ __launch_bounds__(256) void k_cc_shortcut(int32_t *comp, int64_t nv,
             *bad = 1;
             c = c0;
         }
-        if (c != c0) comp[v] = c;
+        if (c != c0) {
+            comp[v] = c;
+            ndeg += (unsigned long long)(off[v + 1] - off[v]);
+        }
         const uint64_t moved = __ballot(c != c0);
         if ((v & 63) == 0) {
             if (moved) chg[v >> 6] |= moved;
             chg_done[v >> 6] = 0;
         }
     }
+    ndeg = wave_sum(ndeg);
+    if ((threadIdx.x & 63) == 0 && ndeg) atomicAdd(&chg_deg[(blockIdx.x % kSlots) * kLine], ndeg);
 }
 
 // Roots (label[v] == v) are the components, one per label (:84-101).
@@ -252,15 +272,18 @@ extern "C" int bd_cc_run(bd_ctx *c, bd_cc_stats *out) {
     if (!c->d_cc_chg &&
         (rc = dev_alloc(c, (void **)&c->d_cc_chg, sizeof(uint64_t) * 2 * nw, "changed-label bitmaps")))
         return rc;
-    // [0] changed flag (int32), then two kSlots x kLine counter blocks:
-    // components and labels read
-    const size_t flag_bytes = sizeof(unsigned long long) * (2 * kSlots * kLine + 1);
+    // one line of flags (int32: [0] a label went down, [1] a label out of
+    // range), then three kSlots x kLine counter blocks: components, labels
+    // read, degrees of changed vertices
+    const size_t flag_bytes = sizeof(unsigned long long) * (kLine + 3 * kSlots * kLine);
     if (!c->d_cc_flag && (rc = dev_alloc(c, (void **)&c->d_cc_flag, flag_bytes, "label flags")))
         return rc;
     c->have_cc = false;
     int32_t *changed = c->d_cc_flag;
-    unsigned long long *count = (unsigned long long *)(c->d_cc_flag + 2);
+    unsigned long long *count = (unsigned long long *)c->d_cc_flag + kLine;
     unsigned long long *nread = count + kSlots * kLine;
+    unsigned long long *chg_deg = nread + kSlots * kLine;
+    std::vector<unsigned long long> h_deg(kSlots * kLine);
     uint64_t *chg[2] = {c->d_cc_chg, c->d_cc_chg + nw};
     const dim3 grid(grid_for(c, nv, 256)), block(256);
     if (c->cc_ntasks < 0) {
@@ -289,15 +312,17 @@ extern "C" int bd_cc_run(bd_ctx *c, bd_cc_stats *out) {
     BD_HIP(c, hipGetLastError());
     int64_t iters = 0;
     double pass_ms = 0;
+    bool all = true;  // read every label (first pass, or after many changes)
     for (;;) {
         ++iters;
         if (iters > nv + 2) return set_err(c, BD_EHIP, "components did not converge");
         uint64_t *in = chg[iters & 1], *next = chg[(iters + 1) & 1];
         BD_HIP(c, hipMemsetAsync(changed, 0, sizeof(int32_t), c->stream));  // [1] stays
+        BD_HIP(c, hipMemsetAsync(chg_deg, 0, sizeof(unsigned long long) * h_deg.size(), c->stream));
         hipExtLaunchKernelGGL(k_cc_hook, hook_grid, block, 0, c->stream, c->ev[2], c->ev[3], 0,
                               (const int64_t *)c->d_off, (const int32_t *)c->d_cols, c->d_comp, nv,
-                              (const td_task *)c->d_cc_tasks, c->cc_ntasks, (const uint64_t *)in,
-                              next, changed, nread);
+                              (const td_task *)c->d_cc_tasks, c->cc_ntasks, (int)all,
+                              (const uint64_t *)in, next, changed, nread, chg_deg);
         BD_HIP(c, hipGetLastError());
         int32_t h[2] = {0, 0};
         BD_HIP(c, hipMemcpyAsync(h, changed, sizeof(h), hipMemcpyDeviceToHost, c->stream));
@@ -307,9 +332,15 @@ extern "C" int bd_cc_run(bd_ctx *c, bd_cc_stats *out) {
         BD_HIP(c, hipEventElapsedTime(&kms, c->ev[2], c->ev[3]));
         pass_ms += kms;
         if (!h[0]) break;
-        hipLaunchKernelGGL(k_cc_shortcut, grid, block, 0, c->stream, c->d_comp, nv, next, in,
-                           changed + 1);
+        hipLaunchKernelGGL(k_cc_shortcut, grid, block, 0, c->stream, (const int64_t *)c->d_off,
+                           c->d_comp, nv, next, in, changed + 1, chg_deg);
         BD_HIP(c, hipGetLastError());
+        BD_HIP(c, hipMemcpyAsync(h_deg.data(), chg_deg, sizeof(unsigned long long) * h_deg.size(),
+                                 hipMemcpyDeviceToHost, c->stream));
+        BD_HIP(c, hipStreamSynchronize(c->stream));
+        unsigned long long deg = 0;
+        for (int s = 0; s < kSlots; ++s) deg += h_deg[s * kLine];
+        all = 2 * (int64_t)deg >= c->nnz;
     }
     hipLaunchKernelGGL(k_cc_count, grid, block, 0, c->stream, c->d_comp, nv, count);
     BD_HIP(c, hipGetLastError());
