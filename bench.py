@@ -31,6 +31,9 @@ TRAFFIC = os.path.join(ROOT, "profiles", "step_kernel_traffic_rmat_s24.json")
 # The step kernel (both directions) as rocprofv3 names it:
 # k_step<workgroup size, 2>.
 STEP_KERNEL_RE = r"k_step<\d+, 2>"
+# Connected components: the edge pass, and its traffic file.
+CC_KERNEL_RE = r"k_cc_hook"
+CC_TRAFFIC = os.path.join(ROOT, "profiles", "cc_edge_pass_traffic_rmat_s24.json")
 COPY_BYTES = 1 << 30  # calibration copy: 1 GiB read + 1 GiB written
 
 
@@ -135,10 +138,11 @@ def copy_calibration(ctx):
     return ctx.copy_bandwidth(COPY_BYTES, reps=3)
 
 
-def summarize_pmc(fetch_csv, write_csv, out_path):
-    """Traffic per step-kernel launch from two rocprofv3 --pmc runs of the
-    roofline leg (FETCH_SIZE, then WRITE_SIZE; KiB per dispatch), each scaled
-    by what the counter reads for the calibration copy of known size."""
+def summarize_pmc(fetch_csv, write_csv, out_path, kernel_re=STEP_KERNEL_RE):
+    """Traffic per launch of the kernel matching kernel_re from two rocprofv3
+    --pmc runs of a leg (FETCH_SIZE, then WRITE_SIZE; KiB per dispatch), each
+    scaled by what the counter reads for the calibration copy of known
+    size."""
     import csv
 
     def per_kernel(path, counter):
@@ -155,17 +159,17 @@ def summarize_pmc(fetch_csv, write_csv, out_path):
     for counter, path, copy_bytes in (("FETCH_SIZE", fetch_csv, COPY_BYTES),
                                       ("WRITE_SIZE", write_csv, COPY_BYTES)):
         rows = per_kernel(path, counter)
-        bu = [v for k, vs in rows.items() if re.search(STEP_KERNEL_RE, k) for v in vs]
+        bu = [v for k, vs in rows.items() if re.search(kernel_re, k) for v in vs]
         cp = [v for k, vs in rows.items() if "k_copy" in k for v in vs]
         if not bu or not cp:
-            raise SystemExit(f"{path}: no step-kernel or copy dispatches for {counter}")
+            raise SystemExit(f"{path}: no {kernel_re} or copy dispatches for {counter}")
         scale = copy_bytes / (float(np.median(cp)) * 1024.0)
         out[counter] = {"kib_per_launch": float(np.mean(bu)), "launches": len(bu),
                         "copy_kib": float(np.median(cp)), "scale": scale,
                         "bytes_per_launch": float(np.mean(bu)) * 1024.0 * scale}
     out["bytes_per_launch"] = round(out["FETCH_SIZE"]["bytes_per_launch"] +
                                     out["WRITE_SIZE"]["bytes_per_launch"])
-    out["kernel"] = sorted(k for k in rows if re.search(STEP_KERNEL_RE, k))
+    out["kernel"] = sorted(k for k in rows if re.search(kernel_re, k))
     with open(out_path, "w") as f:
         json.dump(out, f, indent=1)
     print(json.dumps(out))
@@ -260,6 +264,71 @@ def leg_stress(ctx, args, have_model):
     return out
 
 
+def leg_components(ctx, args):
+    """components::run (components.cc:38-104) on the same RMAT graph: device
+    time of each run (label initialisation, edge passes, shortcuts and the
+    count; the reference times run() alone, components_main.cc:170-180),
+    over args.cc_runs runs after one warm-up.  The roofline is the edge pass
+    (k_cc_hook), timed per launch with HIP events on the library's stream:
+    its algorithmic bytes are the column indices (4 B per adjacency entry),
+    the neighbour labels it reads (4 B each), the offsets and own label of
+    every vertex (12 B), per pass.  The reference's rate is reported too:
+    num_iters * num_edges / time (components_main.cc:179)."""
+    t0 = time.perf_counter()
+    m = ctx.generate_rmat(args.scale, args.edge_factor, seed=args.seed)
+    gen_s = time.perf_counter() - t0
+    nv, nnz, _ = ctx.info()
+    ctx.components()  # warm-up; also builds the per-graph slice table
+    runs = [ctx.components() for _ in range(args.cc_runs)]
+    ok = ctx.components_check()
+    ms = [r["ms"] for r in runs]
+    passes = sum(r["num_iters"] for r in runs)
+    pass_ms = sum(r["pass_ms"] for r in runs)
+    alg = sum(4 * nnz * r["num_iters"] + 4 * r["labels_read"] + 12 * nv * r["num_iters"]
+              for r in runs)
+    achieved = alg / (pass_ms * 1e-3) / 1e9
+    traffic = None
+    if os.path.exists(CC_TRAFFIC):
+        with open(CC_TRAFFIC) as f:
+            traffic = json.load(f).get("bytes_per_launch")
+    copy_gbs = copy_calibration(ctx)
+    mean_ms = float(np.mean(ms))
+    cpu = None
+    if not args.no_cpu:
+        import oracle
+        off, adj = ctx.csr()
+        t0 = time.perf_counter()
+        lab, n, it = oracle.components(off, adj)
+        dt = time.perf_counter() - t0
+        assert n == runs[0]["num_components"], (n, runs[0]["num_components"])
+        cpu = {"value": round(dt * 1e3, 1), "unit": "ms", "cores": 1, "kind": "port",
+               "sample": f"one full serial oracle components::run on the same graph "
+                         f"({it} passes), labels equal to the device's count"}
+    return {
+        "leg": "components",
+        "metric": f"components::run time, RMAT scale {args.scale}",
+        "value": round(mean_ms, 3), "unit": "ms", "higher_is_better": False,
+        "runs": args.cc_runs, "min_ms": round(min(ms), 3),
+        "num_components": runs[0]["num_components"],
+        "num_iters": [r["num_iters"] for r in runs],
+        "reference_gteps": round(float(np.mean([r["num_iters"] * m / (r["ms"] * 1e-3) / 1e9
+                                                 for r in runs])), 2),
+        "check": ok, "dtype": "int32",
+        "config": {"workload": f"RMAT scale {args.scale}, edge factor {args.edge_factor}, "
+                               f"A/B/C/D .57/.19/.19/.05, seed {args.seed}",
+                   "num_vertices": nv, "num_edges": m, "nnz": nnz,
+                   "graph_build_s": round(gen_s, 2)},
+        "roofline": {"kernel": "k_cc_hook", "bound": "hbm", "achieved": round(achieved, 1),
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                     "launches": passes, "algorithmic_bytes_per_launch": round(alg / passes),
+                     "avg_launch_ms": round(pass_ms / passes, 4),
+                     "labels_read_per_launch": round(sum(r["labels_read"] for r in runs) / passes)},
+        "copy_gbs": round(copy_gbs, 1),
+        "cpu_baseline": cpu,
+    }
+
+
 def cpu_baseline(ctx, roots, budget_s):
     import oracle
     off, adj = ctx.csr()
@@ -291,16 +360,23 @@ def main():
     ap.add_argument("--cpu-budget", type=float, default=20.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--model", default=MODEL, help="direction network parameters (float32)")
-    ap.add_argument("--leg", choices=("all", "roofline", "batches", "stress"), default="all",
+    ap.add_argument("--cc-runs", type=int, default=5, help="timed runs of --leg components")
+    ap.add_argument("--leg", choices=("all", "roofline", "batches", "stress", "components"),
+                    default="all",
                     help="roofline: only the roofline searches and the calibration copy (the "
                          "command profiled with rocprofv3); batches: 64 roots at scale 23 and "
-                         "256 uneven graphs; stress: RMAT scale 26, Erdos-Renyi and lattice")
+                         "256 uneven graphs; stress: RMAT scale 26, Erdos-Renyi and lattice; "
+                         "components: connected components on the same graph (one JSON line)")
     ap.add_argument("--summarize-pmc", nargs=2, metavar=("FETCH_CSV", "WRITE_CSV"),
-                    help="write the bottom-up traffic file from two rocprofv3 --pmc runs of "
-                         "--leg roofline, and exit")
+                    help="write the traffic file of the step kernel (--leg roofline runs) or, "
+                         "with --leg components, of the components edge pass, from two "
+                         "rocprofv3 --pmc runs, and exit")
     args = ap.parse_args()
     if args.summarize_pmc:
-        summarize_pmc(args.summarize_pmc[0], args.summarize_pmc[1], TRAFFIC)
+        if args.leg == "components":
+            summarize_pmc(args.summarize_pmc[0], args.summarize_pmc[1], CC_TRAFFIC, CC_KERNEL_RE)
+        else:
+            summarize_pmc(args.summarize_pmc[0], args.summarize_pmc[1], TRAFFIC)
         return
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only; --gpus {args.gpus} is not supported "
@@ -321,6 +397,9 @@ def main():
                     ctx.set_tuning(learned_margin_milli=int(mm))
         res = leg_batches(ctx, args) if args.leg == "batches" else leg_stress(ctx, args, have_model)
         print(json.dumps({"leg": args.leg, "results": res}))
+        return
+    if args.leg == "components":
+        print(json.dumps(leg_components(ctx, args)))
         return
     t0 = time.perf_counter()
     m = ctx.generate_rmat(args.scale, args.edge_factor, seed=args.seed)
