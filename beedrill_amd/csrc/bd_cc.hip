@@ -14,14 +14,16 @@
 // the vertex's own component, so each component ends labelled by its
 // smallest vertex id, whatever order the passes read the labels in.
 //
-// Once few labels change, a pass reads a neighbour's label only if it
+// Once few labels change, a pass only needs the labels of neighbours that
 // changed in the previous pass or shortcut (a bitmap of changed vertices):
 // an unchanged label was read by the vertex before, when it last changed,
 // and the vertex's own label has only gone down since, so it cannot lower
-// it.  Skipping it changes no pass's outcome, only its HBM traffic.  The
-// bitmap lookup costs a random L2 read per edge, so passes that follow many
-// changes (the edges of changed vertices at least half of all edges) read
-// every label without it.
+// it.  Such a pass runs as a push: every changed vertex offers its label to
+// its neighbours (atomicMin).  The CSR is symmetric, so these are the same
+// (vertex, neighbour) pairs; the pass's outcome is the same, its cost is
+// the changed vertices' edges instead of all of them.  Passes that follow
+// many changes (the edges of changed vertices at least half of all edges)
+// pull, with no atomics.
 #include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 
@@ -93,25 +95,26 @@ __global__ void k_cc_tasks(const int64_t *__restrict__ off, int64_t nv, td_task 
 // Owners are non-decreasing across the lanes of a chunk, so a suffix min
 // restricted to equal owners leaves the chunk's min for owner i in the
 // lane where i's run starts, max(s_i - t, 0), which lane i reads.
-// With all == 0, labels of neighbours not in chg_in are not read.  A vertex
-// whose label goes down is set in chg_out and its degree (a slice's length
-// for a long list) added to chg_deg; changed[0] is set if any label went
-// down; labels_read counts the labels gathered.
+// Pull pass: every vertex reads every neighbour's label.  A vertex whose
+// label goes down is set in chg_out and its degree (a slice's length for a
+// long list) added to chg_deg; changed[0] is set if any label went down;
+// labels_read counts the labels gathered.
 __global__ __launch_bounds__(256) void k_cc_hook(const int64_t *__restrict__ off,
                                                  const int32_t *__restrict__ cols, int32_t *comp,
                                                  int64_t nv, const td_task *__restrict__ tasks,
-                                                 int64_t ntasks, int all,
-                                                 const uint64_t *__restrict__ chg_in,
-                                                 uint64_t *__restrict__ chg_out,
+                                                 int64_t ntasks, uint64_t *__restrict__ chg_out,
                                                  int32_t *__restrict__ changed,
                                                  unsigned long long *__restrict__ labels_read,
-                                                 unsigned long long *__restrict__ chg_deg) {
+                                                 unsigned long long *__restrict__ chg_deg,
+                                                 unsigned long long *__restrict__ edge_bytes) {
     const int lane = threadIdx.x & 63;
     const int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
     const int64_t nwaves = ((int64_t)gridDim.x * blockDim.x) >> 6;
     const int64_t nunits = ntasks + (nv + 63) / 64;
     bool ch = false;
-    unsigned long long nread = 0, ndeg = 0;
+    // nbytes: algorithmic bytes besides the 8 B (column id + label) of each
+    // label read
+    unsigned long long nread = 0, ndeg = 0, nbytes = 0;
     for (int64_t u = wave; u < nunits; u += nwaves) {
         if (u < ntasks) {
             const int64_t beg = tasks[u].beg, end = tasks[u].end;
@@ -126,9 +129,7 @@ __global__ __launch_bounds__(256) void k_cc_hook(const int64_t *__restrict__ off
                 }
 #pragma unroll
                 for (int r = 0; r < 4; ++r) {
-                    const bool live =
-                        k0 + r * 64 < end && (all || ((chg_in[c[r] >> 6] >> (c[r] & 63)) & 1));
-                    if (live) {
+                    if (k0 + r * 64 < end) {
                         const uint32_t l = (uint32_t)comp[c[r]];
                         m = l < m ? l : m;
                         ++nread;
@@ -136,7 +137,7 @@ __global__ __launch_bounds__(256) void k_cc_hook(const int64_t *__restrict__ off
                 }
             }
             m = (uint32_t)wave_min(m);
-            // m stays 0xffffffff when no neighbour in the slice changed
+            // a slice is never empty, so m is a label; checked all the same
             if (lane == 0 && m != 0xffffffffu && (int32_t)m < comp[src]) {
                 if ((int32_t)m < atomicMin(comp + src, (int32_t)m)) {
                     ch = true;
@@ -153,6 +154,7 @@ __global__ __launch_bounds__(256) void k_cc_hook(const int64_t *__restrict__ off
             b = off[v];
             e = off[v + 1];
             mine = comp[v];
+            nbytes += 12;
         }
         const int32_t d = e - b < kCcHeavy ? (int32_t)(e - b) : 0;
         const int32_t s_end = wave_incl_scan32(d);
@@ -171,7 +173,7 @@ __global__ __launch_bounds__(256) void k_cc_hook(const int64_t *__restrict__ off
             // one past the owner's last lane in this chunk
             const int32_t run_end = min(shfl(s_end, o) - t, 64);
             const int32_t c = __builtin_nontemporal_load(cols + k);
-            const bool live = q < total && (all || ((chg_in[c >> 6] >> (c & 63)) & 1));
+            const bool live = q < total;
             int32_t m = 0x7fffffff;
             if (live) {
                 m = comp[c];
@@ -198,10 +200,107 @@ __global__ __launch_bounds__(256) void k_cc_hook(const int64_t *__restrict__ off
         }
     }
     if (__ballot(ch) && lane == 0) changed[0] = 1;
+    nbytes = wave_sum(nbytes + 8 * nread);
     nread = wave_sum(nread);
     ndeg = wave_sum(ndeg);
-    if (lane == 0 && nread) atomicAdd(&labels_read[(blockIdx.x % kSlots) * kLine], nread);
-    if (lane == 0 && ndeg) atomicAdd(&chg_deg[(blockIdx.x % kSlots) * kLine], ndeg);
+    const int slot = (blockIdx.x % kSlots) * kLine;
+    if (lane == 0 && nread) atomicAdd(&labels_read[slot], nread);
+    if (lane == 0 && ndeg) atomicAdd(&chg_deg[slot], ndeg);
+    if (lane == 0 && nbytes) atomicAdd(&edge_bytes[slot], nbytes);
+}
+
+// Push pass: every vertex set in chg_in offers its label to its neighbours;
+// a neighbour whose label goes down (atomicMin) is set in chg_out and its
+// degree added to chg_deg.  Long lists go by slices as in the pull pass;
+// the short lists of the changed vertices of a 64-vertex group are taken 64
+// edges at a time, one per lane, the lane finding its source by a binary
+// search over the group's degree prefix.  labels_read counts the
+// neighbours' labels read.
+__global__ __launch_bounds__(256) void k_cc_push(const int64_t *__restrict__ off,
+                                                 const int32_t *__restrict__ cols, int32_t *comp,
+                                                 int64_t nv, const td_task *__restrict__ tasks,
+                                                 int64_t ntasks, const uint64_t *__restrict__ chg_in,
+                                                 uint64_t *__restrict__ chg_out,
+                                                 int32_t *__restrict__ changed,
+                                                 unsigned long long *__restrict__ labels_read,
+                                                 unsigned long long *__restrict__ chg_deg,
+                                                 unsigned long long *__restrict__ edge_bytes) {
+    const int lane = threadIdx.x & 63;
+    const int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const int64_t nwaves = ((int64_t)gridDim.x * blockDim.x) >> 6;
+    const int64_t nunits = ntasks + (nv + 63) / 64;
+    bool ch = false;
+    // nbytes: algorithmic bytes besides the 8 B (column id + label) of each
+    // label read
+    unsigned long long nread = 0, ndeg = 0, nbytes = 0;
+    // offer label l to vertex w
+    auto offer = [&](int32_t w, int32_t l) {
+        ++nread;
+        if (l < comp[w] && l < atomicMin(comp + w, l)) {
+            ch = true;
+            atomicOr((unsigned long long *)chg_out + (w >> 6), 1ull << (w & 63));
+            ndeg += (unsigned long long)(off[w + 1] - off[w]);
+        }
+    };
+    for (int64_t u = wave; u < nunits; u += nwaves) {
+        if (u < ntasks) {
+            const int32_t src = tasks[u].src;
+            if (lane == 0) nbytes += 8;
+            if (!((chg_in[src >> 6] >> (src & 63)) & 1)) continue;
+            if (lane == 0) nbytes += 4;
+            const int64_t beg = tasks[u].beg, end = tasks[u].end;
+            const int32_t l = comp[src];
+            for (int64_t k0 = beg + lane; k0 < end; k0 += 4 * 64) {
+                int32_t c[4];
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    const int64_t k = k0 + r * 64;
+                    c[r] = __builtin_nontemporal_load(cols + (k < end ? k : end - 1));
+                }
+#pragma unroll
+                for (int r = 0; r < 4; ++r)
+                    if (k0 + r * 64 < end) offer(c[r], l);
+            }
+            continue;
+        }
+        const int64_t g = u - ntasks;
+        const uint64_t bits = chg_in[g];
+        if (lane == 0) nbytes += 8;
+        if (!bits) continue;
+        const int64_t v = g * 64 + lane;
+        int64_t b = 0, e = 0;
+        if (v < nv && ((bits >> lane) & 1)) {
+            b = off[v];
+            e = off[v + 1];
+            nbytes += 12;
+        }
+        const int32_t d = e - b < kCcHeavy ? (int32_t)(e - b) : 0;
+        const int32_t lv = d > 0 ? comp[v] : 0;
+        const int32_t s_end = wave_incl_scan32(d);
+        const int32_t s = s_end - d;
+        const int32_t total = __builtin_amdgcn_readlane(s_end, 63);
+        const int64_t base_k = b - s;  // k = base_k of the source + q
+        for (int32_t t = 0; t < total; t += 64) {
+            const int32_t q = t + lane;
+            const int32_t qa = q < total ? q : total - 1;
+            int o = 0;
+#pragma unroll
+            for (int step = 32; step; step >>= 1)
+                if (shfl(s, o + step) <= qa) o += step;
+            const int64_t k = shfl(base_k, o) + qa;
+            const int32_t l = shfl(lv, o);
+            const int32_t w = __builtin_nontemporal_load(cols + k);
+            if (q < total) offer(w, l);
+        }
+    }
+    if (__ballot(ch) && lane == 0) changed[0] = 1;
+    nbytes = wave_sum(nbytes + 8 * nread);
+    nread = wave_sum(nread);
+    ndeg = wave_sum(ndeg);
+    const int slot = (blockIdx.x % kSlots) * kLine;
+    if (lane == 0 && nread) atomicAdd(&labels_read[slot], nread);
+    if (lane == 0 && ndeg) atomicAdd(&chg_deg[slot], ndeg);
+    if (lane == 0 && nbytes) atomicAdd(&edge_bytes[slot], nbytes);
 }
 
 // Shortcut (:74-78): each label becomes its root.  Vertices whose label
@@ -273,16 +372,17 @@ extern "C" int bd_cc_run(bd_ctx *c, bd_cc_stats *out) {
         (rc = dev_alloc(c, (void **)&c->d_cc_chg, sizeof(uint64_t) * 2 * nw, "changed-label bitmaps")))
         return rc;
     // one line of flags (int32: [0] a label went down, [1] a label out of
-    // range), then three kSlots x kLine counter blocks: components, labels
-    // read, degrees of changed vertices
-    const size_t flag_bytes = sizeof(unsigned long long) * (kLine + 3 * kSlots * kLine);
+    // range), then four kSlots x kLine counter blocks: components, labels
+    // read, algorithmic bytes of the edge passes, degrees of changed vertices
+    const size_t flag_bytes = sizeof(unsigned long long) * (kLine + 4 * kSlots * kLine);
     if (!c->d_cc_flag && (rc = dev_alloc(c, (void **)&c->d_cc_flag, flag_bytes, "label flags")))
         return rc;
     c->have_cc = false;
     int32_t *changed = c->d_cc_flag;
     unsigned long long *count = (unsigned long long *)c->d_cc_flag + kLine;
     unsigned long long *nread = count + kSlots * kLine;
-    unsigned long long *chg_deg = nread + kSlots * kLine;
+    unsigned long long *ebytes = nread + kSlots * kLine;
+    unsigned long long *chg_deg = ebytes + kSlots * kLine;
     std::vector<unsigned long long> h_deg(kSlots * kLine);
     uint64_t *chg[2] = {c->d_cc_chg, c->d_cc_chg + nw};
     const dim3 grid(grid_for(c, nv, 256)), block(256);
@@ -306,23 +406,29 @@ extern "C" int bd_cc_run(bd_ctx *c, bd_cc_stats *out) {
     const dim3 hook_grid(grid_for(c, nunits * 64, 256));
     BD_HIP(c, hipEventRecord(c->ev[0], c->stream));
     BD_HIP(c, hipMemsetAsync(changed, 0, sizeof(int32_t) * 2, c->stream));
-    BD_HIP(c, hipMemsetAsync(count, 0, sizeof(unsigned long long) * 2 * kSlots * kLine, c->stream));
+    BD_HIP(c, hipMemsetAsync(count, 0, sizeof(unsigned long long) * 3 * kSlots * kLine, c->stream));
     // pass i reads chg[i & 1] and sets chg[(i + 1) & 1]
     hipLaunchKernelGGL(k_cc_init, grid, block, 0, c->stream, c->d_comp, nv, chg[1], chg[0]);
     BD_HIP(c, hipGetLastError());
     int64_t iters = 0;
     double pass_ms = 0;
-    bool all = true;  // read every label (first pass, or after many changes)
+    bool all = true;  // pull every label (first pass, or after many changes), else push
     for (;;) {
         ++iters;
         if (iters > nv + 2) return set_err(c, BD_EHIP, "components did not converge");
         uint64_t *in = chg[iters & 1], *next = chg[(iters + 1) & 1];
         BD_HIP(c, hipMemsetAsync(changed, 0, sizeof(int32_t), c->stream));  // [1] stays
         BD_HIP(c, hipMemsetAsync(chg_deg, 0, sizeof(unsigned long long) * h_deg.size(), c->stream));
-        hipExtLaunchKernelGGL(k_cc_hook, hook_grid, block, 0, c->stream, c->ev[2], c->ev[3], 0,
-                              (const int64_t *)c->d_off, (const int32_t *)c->d_cols, c->d_comp, nv,
-                              (const td_task *)c->d_cc_tasks, c->cc_ntasks, (int)all,
-                              (const uint64_t *)in, next, changed, nread, chg_deg);
+        if (all)
+            hipExtLaunchKernelGGL(k_cc_hook, hook_grid, block, 0, c->stream, c->ev[2], c->ev[3], 0,
+                                  (const int64_t *)c->d_off, (const int32_t *)c->d_cols, c->d_comp,
+                                  nv, (const td_task *)c->d_cc_tasks, c->cc_ntasks, next, changed,
+                                  nread, chg_deg, ebytes);
+        else
+            hipExtLaunchKernelGGL(k_cc_push, hook_grid, block, 0, c->stream, c->ev[2], c->ev[3], 0,
+                                  (const int64_t *)c->d_off, (const int32_t *)c->d_cols, c->d_comp,
+                                  nv, (const td_task *)c->d_cc_tasks, c->cc_ntasks,
+                                  (const uint64_t *)in, next, changed, nread, chg_deg, ebytes);
         BD_HIP(c, hipGetLastError());
         int32_t h[2] = {0, 0};
         BD_HIP(c, hipMemcpyAsync(h, changed, sizeof(h), hipMemcpyDeviceToHost, c->stream));
@@ -345,16 +451,17 @@ extern "C" int bd_cc_run(bd_ctx *c, bd_cc_stats *out) {
     hipLaunchKernelGGL(k_cc_count, grid, block, 0, c->stream, c->d_comp, nv, count);
     BD_HIP(c, hipGetLastError());
     BD_HIP(c, hipEventRecord(c->ev[1], c->stream));
-    std::vector<unsigned long long> h(2 * kSlots * kLine);
+    std::vector<unsigned long long> h(3 * kSlots * kLine);
     BD_HIP(c, hipMemcpyAsync(h.data(), count, sizeof(unsigned long long) * h.size(),
                              hipMemcpyDeviceToHost, c->stream));
     BD_HIP(c, hipStreamSynchronize(c->stream));
     float ms = 0;
     BD_HIP(c, hipEventElapsedTime(&ms, c->ev[0], c->ev[1]));
-    unsigned long long ncomp = 0, nlab = 0;
+    unsigned long long ncomp = 0, nlab = 0, nbytes = 0;
     for (int s = 0; s < kSlots; ++s) {
         ncomp += h[s * kLine];
         nlab += h[(kSlots + s) * kLine];
+        nbytes += h[(2 * kSlots + s) * kLine];
     }
     c->have_cc = true;
     if (out) {
@@ -363,6 +470,7 @@ extern "C" int bd_cc_run(bd_ctx *c, bd_cc_stats *out) {
         out->ms = ms;
         out->pass_ms = pass_ms;
         out->labels_read = (int64_t)nlab;
+        out->edge_bytes = (int64_t)nbytes;
     }
     return BD_OK;
 }

@@ -31,8 +31,8 @@ TRAFFIC = os.path.join(ROOT, "profiles", "step_kernel_traffic_rmat_s24.json")
 # The step kernel (both directions) as rocprofv3 names it:
 # k_step<workgroup size, 2>.
 STEP_KERNEL_RE = r"k_step<\d+, 2>"
-# Connected components: the edge pass, and its traffic file.
-CC_KERNEL_RE = r"k_cc_hook"
+# Connected components: the edge pass (pull or push), and its traffic file.
+CC_KERNEL_RE = r"k_cc_(hook|push)"
 CC_TRAFFIC = os.path.join(ROOT, "profiles", "cc_edge_pass_traffic_rmat_s24.json")
 COPY_BYTES = 1 << 30  # calibration copy: 1 GiB read + 1 GiB written
 
@@ -269,10 +269,14 @@ def leg_components(ctx, args):
     time of each run (label initialisation, edge passes, shortcuts and the
     count; the reference times run() alone, components_main.cc:170-180),
     over args.cc_runs runs after one warm-up.  The roofline is the edge pass
-    (k_cc_hook), timed per launch with HIP events on the library's stream:
-    its algorithmic bytes are the column indices (4 B per adjacency entry),
-    the neighbour labels it reads (4 B each), the offsets and own label of
-    every vertex (12 B), per pass.  The reference's rate is reported too:
+    (k_cc_hook pulls every label; k_cc_push offers the labels of the
+    vertices that changed), timed per launch with HIP events on the
+    library's stream.  Algorithmic bytes of a pull: the column indices (4 B
+    per adjacency entry), the labels read (4 B each), the offsets and label
+    of every vertex (12 B); of a push: the changed-vertex bitmap (V/8), the
+    offsets and label of the changed vertices (12 B each, bounded by 12 B
+    per vertex), their column indices and the labels they read (4 + 4 B per
+    label read).  The reference's rate is reported too:
     num_iters * num_edges / time (components_main.cc:179)."""
     t0 = time.perf_counter()
     m = ctx.generate_rmat(args.scale, args.edge_factor, seed=args.seed)
@@ -284,8 +288,7 @@ def leg_components(ctx, args):
     ms = [r["ms"] for r in runs]
     passes = sum(r["num_iters"] for r in runs)
     pass_ms = sum(r["pass_ms"] for r in runs)
-    alg = sum(4 * nnz * r["num_iters"] + 4 * r["labels_read"] + 12 * nv * r["num_iters"]
-              for r in runs)
+    alg = sum(r["edge_bytes"] for r in runs)
     achieved = alg / (pass_ms * 1e-3) / 1e9
     traffic = None
     if os.path.exists(CC_TRAFFIC):
@@ -318,7 +321,8 @@ def leg_components(ctx, args):
                                f"A/B/C/D .57/.19/.19/.05, seed {args.seed}",
                    "num_vertices": nv, "num_edges": m, "nnz": nnz,
                    "graph_build_s": round(gen_s, 2)},
-        "roofline": {"kernel": "k_cc_hook", "bound": "hbm", "achieved": round(achieved, 1),
+        "roofline": {"kernel": "k_cc_hook + k_cc_push", "bound": "hbm",
+                     "achieved": round(achieved, 1),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                      "launches": passes, "algorithmic_bytes_per_launch": round(alg / passes),

@@ -23,6 +23,7 @@ def compare(ctx, orc, off, adj):
     assert 0 < st["pass_ms"] <= st["ms"]
     # the first pass reads every neighbour label, later ones only changed ones
     assert len(adj) <= st["labels_read"] <= st["num_iters"] * len(adj)
+    assert st["edge_bytes"] >= 8 * st["labels_read"] + 12 * (len(off) - 1)
     assert ctx.components_check()
     return st
 
