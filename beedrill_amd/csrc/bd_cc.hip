@@ -95,14 +95,19 @@ __global__ void k_cc_tasks(const int64_t *__restrict__ off, int64_t nv, td_task 
 // Owners are non-decreasing across the lanes of a chunk, so a suffix min
 // restricted to equal owners leaves the chunk's min for owner i in the
 // lane where i's run starts, max(s_i - t, 0), which lane i reads.
-// Pull pass: every vertex reads every neighbour's label.  A vertex whose
-// label goes down is set in chg_out and its degree (a slice's length for a
-// long list) added to chg_deg; changed[0] is set if any label went down;
-// labels_read counts the labels gathered.
+// Pull pass: every vertex reads every neighbour's label.  In the first pass
+// (first != 0) the neighbour's id stands in for its label: every label is
+// still its vertex's id when the pass starts, and a pass may read a
+// neighbour's label from before that neighbour's own update in the same
+// pass, so this is one of the pass's own outcomes, with no label gathered.
+// A vertex whose label goes down is set in chg_out and its degree (a
+// slice's length for a long list) added to chg_deg; changed[0] is set if
+// any label went down; labels_read counts the labels gathered.
 __global__ __launch_bounds__(256) void k_cc_hook(const int64_t *__restrict__ off,
                                                  const int32_t *__restrict__ cols, int32_t *comp,
                                                  int64_t nv, const td_task *__restrict__ tasks,
-                                                 int64_t ntasks, uint64_t *__restrict__ chg_out,
+                                                 int64_t ntasks, int first,
+                                                 uint64_t *__restrict__ chg_out,
                                                  int32_t *__restrict__ changed,
                                                  unsigned long long *__restrict__ labels_read,
                                                  unsigned long long *__restrict__ chg_deg,
@@ -130,9 +135,15 @@ __global__ __launch_bounds__(256) void k_cc_hook(const int64_t *__restrict__ off
 #pragma unroll
                 for (int r = 0; r < 4; ++r) {
                     if (k0 + r * 64 < end) {
-                        const uint32_t l = (uint32_t)comp[c[r]];
-                        m = l < m ? l : m;
-                        ++nread;
+                        if (first) {
+                            const uint32_t l = (uint32_t)c[r];
+                            m = l < m ? l : m;
+                            nbytes += 4;
+                        } else {
+                            const uint32_t l = (uint32_t)comp[c[r]];
+                            m = l < m ? l : m;
+                            ++nread;
+                        }
                     }
                 }
             }
@@ -176,8 +187,13 @@ __global__ __launch_bounds__(256) void k_cc_hook(const int64_t *__restrict__ off
             const bool live = q < total;
             int32_t m = 0x7fffffff;
             if (live) {
-                m = comp[c];
-                ++nread;
+                if (first) {
+                    m = c;
+                    nbytes += 4;
+                } else {
+                    m = comp[c];
+                    ++nread;
+                }
             }
 #pragma unroll
             for (int step = 1; step < 64; step <<= 1) {
@@ -422,8 +438,8 @@ extern "C" int bd_cc_run(bd_ctx *c, bd_cc_stats *out) {
         if (all)
             hipExtLaunchKernelGGL(k_cc_hook, hook_grid, block, 0, c->stream, c->ev[2], c->ev[3], 0,
                                   (const int64_t *)c->d_off, (const int32_t *)c->d_cols, c->d_comp,
-                                  nv, (const td_task *)c->d_cc_tasks, c->cc_ntasks, next, changed,
-                                  nread, chg_deg, ebytes);
+                                  nv, (const td_task *)c->d_cc_tasks, c->cc_ntasks,
+                                  (int)(iters == 1), next, changed, nread, chg_deg, ebytes);
         else
             hipExtLaunchKernelGGL(k_cc_push, hook_grid, block, 0, c->stream, c->ev[2], c->ev[3], 0,
                                   (const int64_t *)c->d_off, (const int32_t *)c->d_cols, c->d_comp,

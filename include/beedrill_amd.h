@@ -208,17 +208,19 @@ typedef struct {
     double pass_ms;           /* kernel time of the edge passes alone        */
     int64_t labels_read;      /* neighbour labels the edge passes read       */
     int64_t edge_bytes;       /* algorithmic bytes of the edge passes: 8 per
-                                 label read (column id + label), 12 per
-                                 vertex of a pull, per changed vertex of a
-                                 push, and the changed-vertex bitmap        */
+                                 label read (column id + label), 4 per
+                                 column id of the first pass, 12 per vertex
+                                 of a pull, per changed vertex of a push,
+                                 and the changed-vertex bitmap              */
 } bd_cc_stats;
 
 /* components::run (components.cc:38-104) on the resident graph: label
  * propagation (each vertex takes its neighbours' smallest label) and
  * shortcutting until an edge pass changes nothing.  Each component ends
- * labelled by its smallest vertex id.  Once few labels change, a pass
- * offers only the changed vertices' labels to their neighbours (the same
- * pairs as a pull over changed neighbours; the outcome is unchanged). */
+ * labelled by its smallest vertex id.  The first pass takes neighbour ids
+ * for labels (each label is still its vertex's id); once few labels change,
+ * a pass offers only the changed vertices' labels to their neighbours (the
+ * same pairs as a pull over changed neighbours; the outcome is unchanged). */
 int bd_cc_run(bd_ctx *ctx, bd_cc_stats *out);
 /* components::component_ (components.h:13-14) of the last run: labels[nv]. */
 int bd_cc_fetch(bd_ctx *ctx, int64_t *labels);

@@ -21,9 +21,11 @@ def compare(ctx, orc, off, adj):
     assert st["num_components"] == n
     assert st["num_iters"] >= 1 and st["ms"] > 0
     assert 0 < st["pass_ms"] <= st["ms"]
-    # the first pass reads every neighbour label, later ones only changed ones
-    assert len(adj) <= st["labels_read"] <= st["num_iters"] * len(adj)
-    assert st["edge_bytes"] >= 8 * st["labels_read"] + 12 * (len(off) - 1)
+    # the first pass takes neighbour ids, later pulls read every label and
+    # pushes the changed vertices' labels only
+    nnz, nv = int(off[-1]), len(off) - 1
+    assert 0 <= st["labels_read"] <= (st["num_iters"] - 1) * nnz
+    assert st["edge_bytes"] >= 4 * nnz + 12 * nv + 8 * st["labels_read"]
     assert ctx.components_check()
     return st
 
