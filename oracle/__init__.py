@@ -73,6 +73,12 @@ def lib():
                                     ctypes.c_int64, ctypes.c_int64, _i64p, _i32p,
                                     ctypes.c_void_p, ctypes.c_int64]
         L.oracle_beamer.restype = ctypes.c_int64
+        L.oracle_beamer_omp.argtypes = [ctypes.c_int64, ctypes.c_int64, _i64p, _i32p,
+                                        ctypes.c_int64, ctypes.c_int64, ctypes.c_int64, _i64p,
+                                        np.ctypeslib.ndpointer(dtype=np.int8, flags="C_CONTIGUOUS"),
+                                        ctypes.c_int64]
+        L.oracle_beamer_omp.restype = ctypes.c_int64
+        L.oracle_omp_threads.restype = ctypes.c_int
         L.oracle_td_only.argtypes = [ctypes.c_int64, ctypes.c_int64, _i64p, _i32p, ctypes.c_int64,
                                      _i64p, _i32p]
         L.oracle_td_only.restype = ctypes.c_int64
@@ -232,6 +238,23 @@ def beamer_timed(off, adj, m, source, alpha=15, beta=18):
     if n < 0:
         raise ValueError("bad beamer args")
     return parent
+
+
+def beamer_omp(off, adj, m, source, alpha=15, beta=18):
+    """run_beamer on every host core (oracle_bfs_omp.c): (parent, dirs).
+    For timing the all-core CPU baseline; directions equal beamer()'s."""
+    nv = len(off) - 1
+    parent = np.empty(nv, np.int64)
+    dirs = np.empty(nv + 2, np.int8)
+    n = lib().oracle_beamer_omp(nv, m, off, np.ascontiguousarray(adj, np.int32), source, alpha,
+                                beta, parent, dirs, len(dirs))
+    if n < 0:
+        raise ValueError("bad beamer args")
+    return parent, dirs[:n].copy()
+
+
+def omp_threads():
+    return int(lib().oracle_omp_threads())
 
 
 def td_only(off, adj, m, source):

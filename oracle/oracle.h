@@ -143,6 +143,18 @@ void oracle_head_index(int64_t nv, const int64_t *off, const int32_t *adj, int32
  * unchanged.  NULL restores the in-order definition. */
 void oracle_set_bu_head(const int32_t *head);
 
+/* run_beamer on every host core (OpenMP; oracle_bfs_omp.c), the CPU
+ * baseline's parallel leg.  Same rules as oracle_beamer: the direction of
+ * every step equals its (dirs[i] = ORACLE_TD or ORACLE_BU, may be NULL);
+ * parents are any valid BFS tree.  Returns the number of steps, or -1.
+ * Pinned by tests against oracle_beamer's directions, check() and the
+ * traversed-edge count. */
+int64_t oracle_beamer_omp(int64_t nv, int64_t m, const int64_t *off, const int32_t *adj,
+                          int64_t source, int64_t alpha, int64_t beta, int64_t *parent,
+                          int8_t *dirs, int64_t dirs_cap);
+/* Threads oracle_beamer_omp uses (omp_get_max_threads). */
+int oracle_omp_threads(void);
+
 /* run_with_migrating_threads (hybrid_bfs.cc:197-213): top-down only. */
 int64_t oracle_td_only(int64_t nv, int64_t m, const int64_t *off, const int32_t *adj,
                        int64_t source, int64_t *parent, int32_t *depth);

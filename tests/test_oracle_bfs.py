@@ -287,3 +287,30 @@ def test_head_first_scan_changes_only_edges_examined(orc):
                 seen = (d1[nb] >= 0) & (d1[nb] <= L)
                 want += int(np.argmax(seen)) + 1 if seen.any() else len(nb)
             assert rec["edges_examined"] == want
+
+
+# ------------------------------------------------- all-core CPU baseline
+
+@pytest.mark.parametrize("scale,ef,seed", [(11, 8, 0), (13, 16, 4), (12, 2, 7)])
+def test_parallel_beamer_matches_serial(orc, scale, ef, seed):
+    # the all-core restatement takes the serial one's direction at every
+    # step, builds a valid tree and reaches the same vertices
+    nv, s, d = orc.make_rmat(scale, ef, seed=seed)
+    off, adj, m = csr_of(orc, nv, s, d)
+    for root in orc.pick_roots(off, 4):
+        parent, dirs = orc.beamer_omp(off, adj, m, root)
+        ref_parent, _, levels = orc.beamer(off, adj, m, root)
+        assert dirs.tolist() == [lv["dir"] for lv in levels]
+        assert orc.check(off, adj, root, parent)
+        assert orc.count_traversed(off, parent) == orc.count_traversed(off, ref_parent)
+        np.testing.assert_array_equal(parent >= 0, ref_parent >= 0)
+
+
+def test_parallel_beamer_on_special_graphs(orc):
+    for nv, s, d in (path_graph(200), star_graph(50), complete_graph(12), two_components()):
+        off, adj = orc.build_csr(nv, s, d)
+        m = len(s)
+        parent, dirs = orc.beamer_omp(off, adj, m, 0)
+        _, _, levels = orc.beamer(off, adj, m, 0)
+        assert dirs.tolist() == [lv["dir"] for lv in levels]
+        assert orc.check(off, adj, 0, parent)
