@@ -6,8 +6,8 @@
 //   edge pass (:56-66): each vertex takes the smallest label among its
 //     neighbours when that is smaller than its own.  The CSR is symmetric,
 //     so every directed edge (src -> dst) lowers label[src] in src's own
-//     pass: a label is written only by its vertex's lane, with no atomics.
-//     Lists shorter than 64 are done by one lane, longer ones by the wave.
+//     pull: a short list's label is written only by its vertex's lane, with
+//     no atomics; long lists are cut into slices merged by atomicMin.
 //   shortcut pass (:74-78): label[v] = label[label[v]] until it is a root.
 // The run ends after the first edge pass that changes nothing (:70); the
 // iteration count includes that pass.  Labels only decrease and stay ids of
@@ -35,22 +35,22 @@
 namespace bd {
 namespace {
 
-// Every vertex its own component; every vertex "changed" for the first pass.
-__global__ void k_cc_init(int32_t *__restrict__ comp, int64_t nv, uint64_t *__restrict__ chg_all,
-                          uint64_t *__restrict__ chg_none) {
+// Every vertex its own component; both changed-vertex bitmaps empty.
+__global__ void k_cc_init(int32_t *__restrict__ comp, int64_t nv, uint64_t *__restrict__ chg0,
+                          uint64_t *__restrict__ chg1) {
     for (int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v < nv;
          v += (int64_t)gridDim.x * blockDim.x) {
         comp[v] = (int32_t)v;
         if ((v & 63) == 0) {
-            chg_all[v >> 6] = ~0ull;
-            chg_none[v >> 6] = 0;
+            chg0[v >> 6] = 0;
+            chg1[v >> 6] = 0;
         }
     }
 }
 
 // Lists of kCcHeavy edges or more are cut into slices of kCcSlice edges,
 // one wave each, merged into the label with atomicMin; shorter lists are
-// walked by the wave that owns their vertex.
+// walked 64 edges at a time by the wave that owns their vertex.
 constexpr int kCcHeavy = 256;
 constexpr int kCcSlice = 2048;
 
@@ -84,7 +84,7 @@ __global__ void k_cc_tasks(const int64_t *__restrict__ off, int64_t nv, td_task 
     }
 }
 
-// One edge pass.  Units [0, ntasks) are heavy slices; unit ntasks + g is
+// Pull edge pass.  Units [0, ntasks) are heavy slices; unit ntasks + g is
 // the vertex group [64g, 64g + 64), whose light lists are consecutive in
 // cols (heavy lists between them are skipped).  The group's light edges are
 // numbered q = 0..T-1 (light vertex i owns q in [s_i, s_i + d_i)) and taken
@@ -95,7 +95,7 @@ __global__ void k_cc_tasks(const int64_t *__restrict__ off, int64_t nv, td_task 
 // Owners are non-decreasing across the lanes of a chunk, so a suffix min
 // restricted to equal owners leaves the chunk's min for owner i in the
 // lane where i's run starts, max(s_i - t, 0), which lane i reads.
-// Pull pass: every vertex reads every neighbour's label.  In the first pass
+// Every vertex reads every neighbour's label.  In the first pass
 // (first != 0) the neighbour's id stands in for its label: every label is
 // still its vertex's id when the pass starts, and a pass may read a
 // neighbour's label from before that neighbour's own update in the same
@@ -424,7 +424,7 @@ extern "C" int bd_cc_run(bd_ctx *c, bd_cc_stats *out) {
     BD_HIP(c, hipMemsetAsync(changed, 0, sizeof(int32_t) * 2, c->stream));
     BD_HIP(c, hipMemsetAsync(count, 0, sizeof(unsigned long long) * 3 * kSlots * kLine, c->stream));
     // pass i reads chg[i & 1] and sets chg[(i + 1) & 1]
-    hipLaunchKernelGGL(k_cc_init, grid, block, 0, c->stream, c->d_comp, nv, chg[1], chg[0]);
+    hipLaunchKernelGGL(k_cc_init, grid, block, 0, c->stream, c->d_comp, nv, chg[0], chg[1]);
     BD_HIP(c, hipGetLastError());
     int64_t iters = 0;
     double pass_ms = 0;
