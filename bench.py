@@ -281,6 +281,18 @@ def leg_components(ctx, args):
     t0 = time.perf_counter()
     m = ctx.generate_rmat(args.scale, args.edge_factor, seed=args.seed)
     gen_s = time.perf_counter() - t0
+    out = components_measure(ctx, args, m)
+    out = {"leg": "components",
+           "metric": f"components::run time, RMAT scale {args.scale}", **out}
+    out["config"] = {"workload": f"RMAT scale {args.scale}, edge factor {args.edge_factor}, "
+                                 f"A/B/C/D .57/.19/.19/.05, seed {args.seed}",
+                     "num_vertices": ctx.info()[0], "num_edges": m, "nnz": ctx.info()[1],
+                     "graph_build_s": round(gen_s, 2)}
+    return out
+
+
+def components_measure(ctx, args, m):
+    """components::run on the resident graph (see leg_components)."""
     nv, nnz, _ = ctx.info()
     ctx.components()  # warm-up; also builds the per-graph slice table
     runs = [ctx.components() for _ in range(args.cc_runs)]
@@ -308,8 +320,6 @@ def leg_components(ctx, args):
                "sample": f"one full serial oracle components::run on the same graph "
                          f"({it} passes), labels equal to the device's count"}
     return {
-        "leg": "components",
-        "metric": f"components::run time, RMAT scale {args.scale}",
         "value": round(mean_ms, 3), "unit": "ms", "higher_is_better": False,
         "runs": args.cc_runs, "min_ms": round(min(ms), 3),
         "num_components": runs[0]["num_components"],
@@ -317,10 +327,6 @@ def leg_components(ctx, args):
         "reference_gteps": round(float(np.mean([r["num_iters"] * m / (r["ms"] * 1e-3) / 1e9
                                                  for r in runs])), 2),
         "check": ok, "dtype": "int32",
-        "config": {"workload": f"RMAT scale {args.scale}, edge factor {args.edge_factor}, "
-                               f"A/B/C/D .57/.19/.19/.05, seed {args.seed}",
-                   "num_vertices": nv, "num_edges": m, "nnz": nnz,
-                   "graph_build_s": round(gen_s, 2)},
         "roofline": {"kernel": "k_cc_hook + k_cc_push", "bound": "hbm",
                      "achieved": round(achieved, 1),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s",
@@ -464,6 +470,9 @@ def main():
     log(f"roofline: {roof['achieved']} GB/s over {roof['launches']} step launches; bottom-up "
         f"steps {roof['bu_levels']['achieved']} GB/s")
     cpu = None if args.no_cpu else cpu_baseline(ctx, roots, args.cpu_budget)
+    cc = components_measure(ctx, args, m)
+    log(f"components: {cc['value']} ms, {cc['num_components']} components, passes "
+        f"{cc['num_iters']}")
 
     out = {
         "metric": METRIC,
@@ -492,6 +501,7 @@ def main():
                      for k, v in results.items()},
         "learned_vs_beamer": (round(results["learned"]["hmean_gteps"] /
                                     results["beamer"]["hmean_gteps"], 4) if have_model else None),
+        "components": {"metric": "components::run time on the same graph", **cc},
     }
     print(json.dumps(out))
 
