@@ -406,6 +406,8 @@ extern "C" int bd_cc_run(bd_ctx *c, bd_cc_stats *out) {
         // slices of the long lists, once per graph (like the graph's own
         // indexes, outside the timed run)
         const int64_t cap = c->nnz / kCcSlice + c->nnz / kCcHeavy + 1;
+        dev_free(c->d_cc_tasks);  // left by a build that failed part-way
+        c->d_cc_tasks = nullptr;
         if ((rc = dev_alloc(c, (void **)&c->d_cc_tasks, sizeof(td_task) * cap, "component slices")))
             return rc;
         BD_HIP(c, hipMemsetAsync(count, 0, sizeof(unsigned long long), c->stream));
