@@ -48,7 +48,8 @@ class LevelRec(ctypes.Structure):
 class CCStats(ctypes.Structure):
     _fields_ = [("num_components", ctypes.c_int64), ("num_iters", ctypes.c_int64),
                 ("ms", ctypes.c_double), ("pass_ms", ctypes.c_double),
-                ("labels_read", ctypes.c_int64), ("edge_bytes", ctypes.c_int64)]
+                ("labels_read", ctypes.c_int64), ("edge_bytes", ctypes.c_int64),
+                ("pass_launches", ctypes.c_int64)]
 
     def as_dict(self):
         return {f: getattr(self, f) for f, _ in self._fields_}

@@ -225,6 +225,133 @@ __global__ __launch_bounds__(256) void k_cc_hook(const int64_t *__restrict__ off
     if (lane == 0 && nbytes) atomicAdd(&edge_bytes[slot], nbytes);
 }
 
+// Pull passes tiled by XCD.  Vertex ids are cut into 8 blocks of 2^bshift
+// (the last may be short).  Workgroups go to the 8 XCDs round-robin, so the
+// workgroups with blockIdx % 8 == x share XCD x's L2: in phase p they pull
+// row block x (their vertices) from column block (x + p) % 8 only, and the
+// labels they read sit in one block (8 MB at 2^24 vertices) instead of all
+// of the label array.  Eight phases make one pull pass; a vertex's label is
+// written in each phase by the one wave that owns its group.  Short lists
+// are regrouped once per graph so that each column block's neighbours are
+// contiguous: bcols holds the regrouped list in the vertex's own CSR range,
+// and byte c of bend[v] is where block c's run ends (0 for empty and long
+// lists, whose slices pull as before).
+__global__ void k_cc_blocks(const int64_t *__restrict__ off, const int32_t *__restrict__ cols,
+                            int64_t nv, int bshift, int32_t *__restrict__ bcols,
+                            uint64_t *__restrict__ bend) {
+    for (int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v < nv;
+         v += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t b = off[v], e = off[v + 1];
+        if (e == b || e - b >= kCcHeavy) {
+            bend[v] = 0;
+            continue;
+        }
+        // per-block counts, one byte each (a short list has < 256 entries)
+        uint64_t cnt = 0;
+        for (int64_t k = b; k < e; ++k) cnt += 1ull << (8 * (cols[k] >> bshift));
+        uint64_t ends = cnt;  // bytewise inclusive prefix
+        ends += ends << 8;
+        ends += ends << 16;
+        ends += ends << 32;
+        bend[v] = ends;
+        uint64_t cur = ends - cnt;  // bytewise starts
+        for (int64_t k = b; k < e; ++k) {
+            const int32_t u = cols[k];
+            const int sh = 8 * (u >> bshift);
+            bcols[b + (int64_t)((cur >> sh) & 255)] = u;
+            cur += 1ull << sh;
+        }
+    }
+}
+
+// One phase of a tiled pull pass (see above).  Same work per group as the
+// pull pass's short lists, restricted to one column block.
+__global__ __launch_bounds__(256) void k_cc_tile(const int64_t *__restrict__ off,
+                                                 const int32_t *__restrict__ bcols,
+                                                 const uint64_t *__restrict__ bend, int32_t *comp,
+                                                 int64_t nv, int bshift, int phase,
+                                                 uint64_t *__restrict__ chg_out,
+                                                 int32_t *__restrict__ changed,
+                                                 unsigned long long *__restrict__ labels_read,
+                                                 unsigned long long *__restrict__ chg_deg,
+                                                 unsigned long long *__restrict__ edge_bytes) {
+    const int lane = threadIdx.x & 63;
+    const int xcd = blockIdx.x & 7;
+    const int col = (xcd + phase) & 7;
+    const int64_t wave = ((int64_t)(blockIdx.x >> 3) * blockDim.x + threadIdx.x) >> 6;
+    const int64_t nwaves = ((int64_t)(gridDim.x >> 3) * blockDim.x) >> 6;
+    const int64_t r0 = (int64_t)xcd << bshift;
+    const int64_t r1 = r0 + (1ll << bshift) < nv ? r0 + (1ll << bshift) : nv;
+    bool ch = false;
+    unsigned long long nread = 0, ndeg = 0, nbytes = 0;
+    for (int64_t g0 = r0 + wave * 64; g0 < r1; g0 += nwaves * 64) {
+        const int64_t v = g0 + lane;
+        int64_t b = 0;
+        int32_t d = 0, deg = 0;
+        int32_t mine = 0x7fffffff;
+        if (v < r1) {
+            const uint64_t e8 = bend[v];
+            if (e8) {
+                const int32_t st = col ? (int32_t)((e8 >> (8 * (col - 1))) & 255) : 0;
+                d = (int32_t)((e8 >> (8 * col)) & 255) - st;
+                deg = (int32_t)(e8 >> 56);
+                b = off[v] + st;
+            }
+            mine = comp[v];
+            // a vertex's offset and own label count once per pass, as in an
+            // untiled pull; the phases' re-reads are the tiling's own cost
+            if (phase == 0) nbytes += 12;
+        }
+        const int32_t s_end = wave_incl_scan32(d);
+        const int32_t s = s_end - d;
+        const int32_t total = __builtin_amdgcn_readlane(s_end, 63);
+        const int64_t base_k = b - s;
+        int32_t acc = mine;
+        for (int32_t t = 0; t < total; t += 64) {
+            const int32_t q = t + lane;
+            const int32_t qa = q < total ? q : total - 1;
+            int o = 0;
+#pragma unroll
+            for (int step = 32; step; step >>= 1)
+                if (shfl(s, o + step) <= qa) o += step;
+            const int64_t k = shfl(base_k, o) + qa;
+            const int32_t run_end = min(shfl(s_end, o) - t, 64);
+            const int32_t c = __builtin_nontemporal_load(bcols + k);
+            int32_t m = 0x7fffffff;
+            if (q < total) {
+                m = comp[c];
+                ++nread;
+            }
+#pragma unroll
+            for (int step = 1; step < 64; step <<= 1) {
+                const int32_t other = shfl(m, (lane + step) & 63);
+                if (lane + step < run_end && other < m) m = other;
+            }
+            const bool mine_here = d > 0 && s < t + 64 && s_end > t;
+            const int32_t got = shfl(m, mine_here ? (s > t ? s - t : 0) : lane);
+            if (mine_here && got < acc) acc = got;
+        }
+        const bool down = acc < mine;
+        if (down) {
+            comp[v] = acc;
+            ndeg += (unsigned long long)deg;
+        }
+        const uint64_t downs = __ballot(down);
+        if (downs && lane == 0) {
+            atomicOr((unsigned long long *)chg_out + (g0 >> 6), (unsigned long long)downs);
+            ch = true;
+        }
+    }
+    if (__ballot(ch) && lane == 0) changed[0] = 1;
+    nbytes = wave_sum(nbytes + 8 * nread);
+    nread = wave_sum(nread);
+    ndeg = wave_sum(ndeg);
+    const int slot = (blockIdx.x % kSlots) * kLine;
+    if (lane == 0 && nread) atomicAdd(&labels_read[slot], nread);
+    if (lane == 0 && ndeg) atomicAdd(&chg_deg[slot], ndeg);
+    if (lane == 0 && nbytes) atomicAdd(&edge_bytes[slot], nbytes);
+}
+
 // Push pass: every vertex set in chg_in offers its label to its neighbours;
 // a neighbour whose label goes down (atomicMin) is set in chg_out and its
 // degree added to chg_deg.  Long lists go by slices as in the pull pass;
@@ -419,16 +546,34 @@ extern "C" int bd_cc_run(bd_ctx *c, bd_cc_stats *out) {
         BD_HIP(c, hipStreamSynchronize(c->stream));
         if ((int64_t)n > cap) return set_err(c, BD_EHIP, "component slices overflow");
         c->cc_ntasks = (int64_t)n;
+        // short lists regrouped by column block for the tiled pull passes
+        int bshift = 6;
+        while ((8ll << bshift) < nv) ++bshift;
+        dev_free(c->d_cc_bcols);
+        dev_free(c->d_cc_bend);
+        c->d_cc_bcols = nullptr;
+        c->d_cc_bend = nullptr;
+        c->cc_ntasks = -1;  // until the regrouping is built too
+        if ((rc = dev_alloc(c, (void **)&c->d_cc_bcols, sizeof(int32_t) * c->nnz, "regrouped lists")) ||
+            (rc = dev_alloc(c, (void **)&c->d_cc_bend, sizeof(uint64_t) * nv, "block ends")))
+            return rc;
+        hipLaunchKernelGGL(k_cc_blocks, grid, block, 0, c->stream, (const int64_t *)c->d_off,
+                           (const int32_t *)c->d_cols, nv, bshift, c->d_cc_bcols, c->d_cc_bend);
+        BD_HIP(c, hipGetLastError());
+        c->cc_bshift = bshift;
+        c->cc_ntasks = (int64_t)n;
     }
     const int64_t nunits = c->cc_ntasks + nw;
     const dim3 hook_grid(grid_for(c, nunits * 64, 256));
+    // a whole number of workgroups per XCD
+    const dim3 tile_grid((unsigned)std::max<int64_t>(8, grid_for(c, nv, 256) & ~7ll));
     BD_HIP(c, hipEventRecord(c->ev[0], c->stream));
     BD_HIP(c, hipMemsetAsync(changed, 0, sizeof(int32_t) * 2, c->stream));
     BD_HIP(c, hipMemsetAsync(count, 0, sizeof(unsigned long long) * 3 * kSlots * kLine, c->stream));
     // pass i reads chg[i & 1] and sets chg[(i + 1) & 1]
     hipLaunchKernelGGL(k_cc_init, grid, block, 0, c->stream, c->d_comp, nv, chg[0], chg[1]);
     BD_HIP(c, hipGetLastError());
-    int64_t iters = 0;
+    int64_t iters = 0, launches = 0;
     double pass_ms = 0;
     bool all = true;  // pull every label (first pass, or after many changes), else push
     for (;;) {
@@ -437,7 +582,22 @@ extern "C" int bd_cc_run(bd_ctx *c, bd_cc_stats *out) {
         uint64_t *in = chg[iters & 1], *next = chg[(iters + 1) & 1];
         BD_HIP(c, hipMemsetAsync(changed, 0, sizeof(int32_t), c->stream));  // [1] stays
         BD_HIP(c, hipMemsetAsync(chg_deg, 0, sizeof(unsigned long long) * h_deg.size(), c->stream));
-        if (all)
+        if (all && iters > 1) {
+            // tiled pull: the long lists' slices, then 8 phases of short lists
+            const dim3 slice_grid(grid_for(c, std::max<int64_t>(c->cc_ntasks, 1) * 64, 256));
+            hipExtLaunchKernelGGL(k_cc_hook, slice_grid, block, 0, c->stream, c->ev[2], nullptr, 0,
+                                  (const int64_t *)c->d_off, (const int32_t *)c->d_cols, c->d_comp,
+                                  (int64_t)0, (const td_task *)c->d_cc_tasks, c->cc_ntasks, 0,
+                                  next, changed, nread, chg_deg, ebytes);
+            BD_HIP(c, hipGetLastError());
+            for (int phase = 0; phase < 8; ++phase)
+                hipExtLaunchKernelGGL(k_cc_tile, tile_grid, block, 0, c->stream, nullptr,
+                                      phase == 7 ? c->ev[3] : nullptr, 0,
+                                      (const int64_t *)c->d_off, (const int32_t *)c->d_cc_bcols,
+                                      (const uint64_t *)c->d_cc_bend, c->d_comp, nv, c->cc_bshift,
+                                      phase, next, changed, nread, chg_deg, ebytes);
+            launches += 9;
+        } else if (all)
             hipExtLaunchKernelGGL(k_cc_hook, hook_grid, block, 0, c->stream, c->ev[2], c->ev[3], 0,
                                   (const int64_t *)c->d_off, (const int32_t *)c->d_cols, c->d_comp,
                                   nv, (const td_task *)c->d_cc_tasks, c->cc_ntasks,
@@ -447,6 +607,7 @@ extern "C" int bd_cc_run(bd_ctx *c, bd_cc_stats *out) {
                                   (const int64_t *)c->d_off, (const int32_t *)c->d_cols, c->d_comp,
                                   nv, (const td_task *)c->d_cc_tasks, c->cc_ntasks,
                                   (const uint64_t *)in, next, changed, nread, chg_deg, ebytes);
+        if (!(all && iters > 1)) ++launches;
         BD_HIP(c, hipGetLastError());
         int32_t h[2] = {0, 0};
         BD_HIP(c, hipMemcpyAsync(h, changed, sizeof(h), hipMemcpyDeviceToHost, c->stream));
@@ -489,6 +650,7 @@ extern "C" int bd_cc_run(bd_ctx *c, bd_cc_stats *out) {
         out->pass_ms = pass_ms;
         out->labels_read = (int64_t)nlab;
         out->edge_bytes = (int64_t)nbytes;
+        out->pass_launches = launches;
     }
     return BD_OK;
 }

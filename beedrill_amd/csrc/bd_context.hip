@@ -45,7 +45,12 @@ static void free_search(bd_ctx *c) {
     dev_free(c->d_cc_flag);
     dev_free(c->d_cc_tasks);
     dev_free(c->d_cc_chg);
+    dev_free(c->d_cc_bcols);
+    dev_free(c->d_cc_bend);
     c->d_cc_chg = nullptr;
+    c->d_cc_bcols = nullptr;
+    c->d_cc_bend = nullptr;
+    c->cc_bshift = 0;
     c->d_comp = nullptr;
     c->d_cc_flag = nullptr;
     c->d_cc_tasks = nullptr;

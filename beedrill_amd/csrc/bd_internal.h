@@ -121,6 +121,9 @@ struct bd_ctx {
     int32_t *d_cc_flag = nullptr; // [0] an edge pass changed a label, [1..] counters
     bool have_cc = false;         // d_comp holds a finished run
     uint64_t *d_cc_chg = nullptr;       // two bitmaps of vertices whose label changed
+    int32_t *d_cc_bcols = nullptr;      // short lists regrouped by column block
+    uint64_t *d_cc_bend = nullptr;      // per vertex: end of each block's run (one byte each)
+    int cc_bshift = 0;                  // column blocks of 2^cc_bshift vertices
     bd::td_task *d_cc_tasks = nullptr;  // slices of the long lists of the edge pass
     int64_t cc_ntasks = -1;             // -1: not built for the resident graph
     uint64_t *d_vis = nullptr;

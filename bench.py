@@ -32,7 +32,7 @@ TRAFFIC = os.path.join(ROOT, "profiles", "step_kernel_traffic_rmat_s24.json")
 # k_step<workgroup size, 2>.
 STEP_KERNEL_RE = r"k_step<\d+, 2>"
 # Connected components: the edge pass (pull or push), and its traffic file.
-CC_KERNEL_RE = r"k_cc_(hook|push)"
+CC_KERNEL_RE = r"k_cc_(hook|push|tile)"
 CC_TRAFFIC = os.path.join(ROOT, "profiles", "cc_edge_pass_traffic_rmat_s24.json")
 COPY_BYTES = 1 << 30  # calibration copy: 1 GiB read + 1 GiB written
 
@@ -269,9 +269,11 @@ def leg_components(ctx, args):
     time of each run (label initialisation, edge passes, shortcuts and the
     count; the reference times run() alone, components_main.cc:170-180),
     over args.cc_runs runs after one warm-up.  The roofline is the edge pass
-    (k_cc_hook pulls every label; k_cc_push offers the labels of the
-    vertices that changed), timed per launch with HIP events on the
-    library's stream.  Algorithmic bytes of a pull: the column indices (4 B
+    (k_cc_hook pulls every label -- after the first pass as the long lists'
+    slices plus eight XCD-tiled k_cc_tile phases -- and k_cc_push offers the
+    labels of the vertices that changed), timed per pass with HIP events on
+    the library's stream; avg_launch_ms is that time over the passes'
+    kernel launches.  Algorithmic bytes of a pull: the column indices (4 B
     per adjacency entry), the labels read (4 B each), the offsets and label
     of every vertex (12 B); of a push: the changed-vertex bitmap (V/8), the
     offsets and label of the changed vertices (12 B each, bounded by 12 B
@@ -299,6 +301,7 @@ def components_measure(ctx, args, m):
     ok = ctx.components_check()
     ms = [r["ms"] for r in runs]
     passes = sum(r["num_iters"] for r in runs)
+    launches = sum(r["pass_launches"] for r in runs)
     pass_ms = sum(r["pass_ms"] for r in runs)
     alg = sum(r["edge_bytes"] for r in runs)
     achieved = alg / (pass_ms * 1e-3) / 1e9
@@ -327,13 +330,14 @@ def components_measure(ctx, args, m):
         "reference_gteps": round(float(np.mean([r["num_iters"] * m / (r["ms"] * 1e-3) / 1e9
                                                  for r in runs])), 2),
         "check": ok, "dtype": "int32",
-        "roofline": {"kernel": "k_cc_hook + k_cc_push", "bound": "hbm",
+        "roofline": {"kernel": "k_cc_hook + k_cc_tile + k_cc_push", "bound": "hbm",
                      "achieved": round(achieved, 1),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-                     "launches": passes, "algorithmic_bytes_per_launch": round(alg / passes),
-                     "avg_launch_ms": round(pass_ms / passes, 4),
-                     "labels_read_per_launch": round(sum(r["labels_read"] for r in runs) / passes)},
+                     "passes": passes, "launches": launches,
+                     "algorithmic_bytes_per_launch": round(alg / launches),
+                     "avg_launch_ms": round(pass_ms / launches, 4),
+                     "labels_read_per_launch": round(sum(r["labels_read"] for r in runs) / launches)},
         "copy_gbs": round(copy_gbs, 1),
         "cpu_baseline": cpu,
     }

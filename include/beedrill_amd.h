@@ -212,6 +212,8 @@ typedef struct {
                                  column id of the first pass, 12 per vertex
                                  of a pull, per changed vertex of a push,
                                  and the changed-vertex bitmap              */
+    int64_t pass_launches;    /* kernel launches of the edge passes (a tiled
+                                 pull pass takes 9)                         */
 } bd_cc_stats;
 
 /* components::run (components.cc:38-104) on the resident graph: label

@@ -26,6 +26,7 @@ def compare(ctx, orc, off, adj):
     nnz, nv = int(off[-1]), len(off) - 1
     assert 0 <= st["labels_read"] <= (st["num_iters"] - 1) * nnz
     assert st["edge_bytes"] >= 4 * nnz + 12 * nv + 8 * st["labels_read"]
+    assert st["num_iters"] <= st["pass_launches"] <= 9 * st["num_iters"]
     assert ctx.components_check()
     return st
 
