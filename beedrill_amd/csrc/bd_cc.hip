@@ -264,12 +264,91 @@ __global__ void k_cc_blocks(const int64_t *__restrict__ off, const int32_t *__re
     }
 }
 
+// Long lists for the tiled pulls: each is regrouped by column block like
+// the short lists (a wave per list, found through its first slice), and
+// each block's run is cut into pieces of kCcSlice edges, filed under bucket
+// (row block of the vertex) * 8 + column block.  mode 0 counts the pieces
+// of each bucket into cursor[]; mode 1 (cursor = the buckets' starts)
+// regroups and files them.
+__global__ __launch_bounds__(256) void k_cc_heavy_tiles(
+    const int64_t *__restrict__ off, const int32_t *__restrict__ cols, int bshift,
+    const td_task *__restrict__ tasks, int64_t ntasks, int32_t *__restrict__ bcols,
+    unsigned long long *__restrict__ cursor, td_task *__restrict__ pieces, int64_t cap, int mode) {
+    const int lane = threadIdx.x & 63;
+    const uint64_t below = (1ull << lane) - 1;
+    const int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const int64_t nwaves = ((int64_t)gridDim.x * blockDim.x) >> 6;
+    for (int64_t u = wave; u < ntasks; u += nwaves) {
+        const int32_t src = tasks[u].src;
+        const int64_t b = off[src], e = off[(int64_t)src + 1];
+        if (tasks[u].beg != b) continue;  // not the list's first slice
+        int64_t cnt[8];
+#pragma unroll
+        for (int c = 0; c < 8; ++c) cnt[c] = 0;
+        for (int64_t k0 = b; k0 < e; k0 += 64) {
+            const int64_t k = k0 + lane;
+            const int blk = k < e ? cols[k] >> bshift : 8;
+#pragma unroll
+            for (int c = 0; c < 8; ++c) cnt[c] += __popcll(__ballot(blk == c));
+        }
+        const int row = src >> bshift;
+        if (mode == 0) {
+            if (lane == 0) {
+#pragma unroll
+                for (int c = 0; c < 8; ++c)
+                    if (cnt[c])
+                        atomicAdd(cursor + row * 8 + c,
+                                  (unsigned long long)((cnt[c] + kCcSlice - 1) / kCcSlice));
+            }
+            continue;
+        }
+        int64_t st[8], cur[8];
+        int64_t acc = 0;
+#pragma unroll
+        for (int c = 0; c < 8; ++c) {
+            st[c] = acc;
+            acc += cnt[c];
+            cur[c] = 0;
+        }
+        for (int64_t k0 = b; k0 < e; k0 += 64) {
+            const int64_t k = k0 + lane;
+            const int32_t w = k < e ? cols[k] : 0;
+            const int blk = k < e ? w >> bshift : 8;
+#pragma unroll
+            for (int c = 0; c < 8; ++c) {
+                const uint64_t mask = __ballot(blk == c);
+                if (blk == c) bcols[b + st[c] + cur[c] + __popcll(mask & below)] = w;
+                cur[c] += __popcll(mask);
+            }
+        }
+#pragma unroll
+        for (int c = 0; c < 8; ++c) {
+            if (!cnt[c]) continue;
+            const int64_t n = (cnt[c] + kCcSlice - 1) / kCcSlice;
+            unsigned long long at = 0;
+            if (lane == 0) at = atomicAdd(cursor + row * 8 + c, (unsigned long long)n);
+            at = (unsigned long long)lane_bcast((int64_t)at, 0);
+            for (int64_t i = lane; i < n; i += 64) {
+                if ((int64_t)at + i >= cap) break;
+                td_task t;
+                t.beg = b + st[c] + i * kCcSlice;
+                t.end = t.beg + kCcSlice < b + st[c] + cnt[c] ? t.beg + kCcSlice : b + st[c] + cnt[c];
+                t.src = src;
+                t.pad = 0;
+                pieces[at + i] = t;
+            }
+        }
+    }
+}
+
 // One phase of a tiled pull pass (see above).  Same work per group as the
 // pull pass's short lists, restricted to one column block.
 __global__ __launch_bounds__(256) void k_cc_tile(const int64_t *__restrict__ off,
                                                  const int32_t *__restrict__ bcols,
                                                  const uint64_t *__restrict__ bend, int32_t *comp,
                                                  int64_t nv, int bshift, int phase,
+                                                 const td_task *__restrict__ pieces,
+                                                 const int64_t *__restrict__ piece_base,
                                                  uint64_t *__restrict__ chg_out,
                                                  int32_t *__restrict__ changed,
                                                  unsigned long long *__restrict__ labels_read,
@@ -284,6 +363,37 @@ __global__ __launch_bounds__(256) void k_cc_tile(const int64_t *__restrict__ off
     const int64_t r1 = r0 + (1ll << bshift) < nv ? r0 + (1ll << bshift) : nv;
     bool ch = false;
     unsigned long long nread = 0, ndeg = 0, nbytes = 0;
+    // long lists: this bucket's pieces, a wave each, merged by atomicMin
+    const int bucket = xcd * 8 + col;
+    for (int64_t u = piece_base[bucket] + wave; u < piece_base[bucket + 1]; u += nwaves) {
+        const int64_t beg = pieces[u].beg, end = pieces[u].end;
+        const int32_t src = pieces[u].src;
+        uint32_t m = 0xffffffffu;
+        for (int64_t k0 = beg + lane; k0 < end; k0 += 4 * 64) {
+            int32_t cc[4];
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int64_t k = k0 + r * 64;
+                cc[r] = __builtin_nontemporal_load(bcols + (k < end ? k : end - 1));
+            }
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                if (k0 + r * 64 < end) {
+                    const uint32_t l = (uint32_t)comp[cc[r]];
+                    m = l < m ? l : m;
+                    ++nread;
+                }
+            }
+        }
+        m = (uint32_t)wave_min(m);
+        if (lane == 0 && m != 0xffffffffu && (int32_t)m < comp[src]) {
+            if ((int32_t)m < atomicMin(comp + src, (int32_t)m)) {
+                ch = true;
+                ndeg += (unsigned long long)(end - beg);
+                atomicOr((unsigned long long *)chg_out + (src >> 6), 1ull << (src & 63));
+            }
+        }
+    }
     for (int64_t g0 = r0 + wave * 64; g0 < r1; g0 += nwaves * 64) {
         const int64_t v = g0 + lane;
         int64_t b = 0;
@@ -560,6 +670,40 @@ extern "C" int bd_cc_run(bd_ctx *c, bd_cc_stats *out) {
         hipLaunchKernelGGL(k_cc_blocks, grid, block, 0, c->stream, (const int64_t *)c->d_off,
                            (const int32_t *)c->d_cols, nv, bshift, c->d_cc_bcols, c->d_cc_bend);
         BD_HIP(c, hipGetLastError());
+        // long lists: regrouped too, their pieces filed by (row, column) block
+        const dim3 slice_grid(grid_for(c, std::max<int64_t>((int64_t)n, 1) * 64, 256));
+        unsigned long long pcnt[64], pbase[65];
+        BD_HIP(c, hipMemsetAsync(count, 0, sizeof(pcnt), c->stream));
+        hipLaunchKernelGGL(k_cc_heavy_tiles, slice_grid, block, 0, c->stream,
+                           (const int64_t *)c->d_off, (const int32_t *)c->d_cols, bshift,
+                           (const td_task *)c->d_cc_tasks, (int64_t)n, c->d_cc_bcols, count,
+                           (td_task *)nullptr, (int64_t)0, 0);
+        BD_HIP(c, hipGetLastError());
+        BD_HIP(c, hipMemcpyAsync(pcnt, count, sizeof(pcnt), hipMemcpyDeviceToHost, c->stream));
+        BD_HIP(c, hipStreamSynchronize(c->stream));
+        pbase[0] = 0;
+        for (int i = 0; i < 64; ++i) pbase[i + 1] = pbase[i] + pcnt[i];
+        dev_free(c->d_cc_pieces);
+        dev_free(c->d_cc_pbase);
+        c->d_cc_pieces = nullptr;
+        c->d_cc_pbase = nullptr;
+        if ((rc = dev_alloc(c, (void **)&c->d_cc_pieces,
+                            sizeof(td_task) * std::max<unsigned long long>(pbase[64], 1),
+                            "long-list pieces")) ||
+            (rc = dev_alloc(c, (void **)&c->d_cc_pbase, sizeof(pbase), "piece buckets")))
+            return rc;
+        BD_HIP(c, hipMemcpyAsync(count, pbase, sizeof(pcnt), hipMemcpyHostToDevice, c->stream));
+        BD_HIP(c, hipMemcpyAsync(c->d_cc_pbase, pbase, sizeof(pbase), hipMemcpyHostToDevice,
+                                 c->stream));
+        hipLaunchKernelGGL(k_cc_heavy_tiles, slice_grid, block, 0, c->stream,
+                           (const int64_t *)c->d_off, (const int32_t *)c->d_cols, bshift,
+                           (const td_task *)c->d_cc_tasks, (int64_t)n, c->d_cc_bcols, count,
+                           c->d_cc_pieces, (int64_t)pbase[64], 1);
+        BD_HIP(c, hipGetLastError());
+        BD_HIP(c, hipMemcpyAsync(pcnt, count, sizeof(pcnt), hipMemcpyDeviceToHost, c->stream));
+        BD_HIP(c, hipStreamSynchronize(c->stream));
+        for (int i = 0; i < 64; ++i)
+            if (pcnt[i] != pbase[i + 1]) return set_err(c, BD_EHIP, "long-list pieces miscounted");
         c->cc_bshift = bshift;
         c->cc_ntasks = (int64_t)n;
     }
@@ -583,20 +727,17 @@ extern "C" int bd_cc_run(bd_ctx *c, bd_cc_stats *out) {
         BD_HIP(c, hipMemsetAsync(changed, 0, sizeof(int32_t), c->stream));  // [1] stays
         BD_HIP(c, hipMemsetAsync(chg_deg, 0, sizeof(unsigned long long) * h_deg.size(), c->stream));
         if (all && iters > 1) {
-            // tiled pull: the long lists' slices, then 8 phases of short lists
-            const dim3 slice_grid(grid_for(c, std::max<int64_t>(c->cc_ntasks, 1) * 64, 256));
-            hipExtLaunchKernelGGL(k_cc_hook, slice_grid, block, 0, c->stream, c->ev[2], nullptr, 0,
-                                  (const int64_t *)c->d_off, (const int32_t *)c->d_cols, c->d_comp,
-                                  (int64_t)0, (const td_task *)c->d_cc_tasks, c->cc_ntasks, 0,
-                                  next, changed, nread, chg_deg, ebytes);
-            BD_HIP(c, hipGetLastError());
+            // tiled pull: 8 phases, each XCD pulling from one column block
             for (int phase = 0; phase < 8; ++phase)
-                hipExtLaunchKernelGGL(k_cc_tile, tile_grid, block, 0, c->stream, nullptr,
+                hipExtLaunchKernelGGL(k_cc_tile, tile_grid, block, 0, c->stream,
+                                      phase == 0 ? c->ev[2] : nullptr,
                                       phase == 7 ? c->ev[3] : nullptr, 0,
                                       (const int64_t *)c->d_off, (const int32_t *)c->d_cc_bcols,
                                       (const uint64_t *)c->d_cc_bend, c->d_comp, nv, c->cc_bshift,
-                                      phase, next, changed, nread, chg_deg, ebytes);
-            launches += 9;
+                                      phase, (const td_task *)c->d_cc_pieces,
+                                      (const int64_t *)c->d_cc_pbase, next, changed, nread,
+                                      chg_deg, ebytes);
+            launches += 8;
         } else if (all)
             hipExtLaunchKernelGGL(k_cc_hook, hook_grid, block, 0, c->stream, c->ev[2], c->ev[3], 0,
                                   (const int64_t *)c->d_off, (const int32_t *)c->d_cols, c->d_comp,

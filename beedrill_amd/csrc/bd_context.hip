@@ -47,6 +47,10 @@ static void free_search(bd_ctx *c) {
     dev_free(c->d_cc_chg);
     dev_free(c->d_cc_bcols);
     dev_free(c->d_cc_bend);
+    dev_free(c->d_cc_pieces);
+    dev_free(c->d_cc_pbase);
+    c->d_cc_pieces = nullptr;
+    c->d_cc_pbase = nullptr;
     c->d_cc_chg = nullptr;
     c->d_cc_bcols = nullptr;
     c->d_cc_bend = nullptr;

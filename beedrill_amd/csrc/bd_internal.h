@@ -124,6 +124,8 @@ struct bd_ctx {
     int32_t *d_cc_bcols = nullptr;      // short lists regrouped by column block
     uint64_t *d_cc_bend = nullptr;      // per vertex: end of each block's run (one byte each)
     int cc_bshift = 0;                  // column blocks of 2^cc_bshift vertices
+    bd::td_task *d_cc_pieces = nullptr; // long-list pieces by (row block, column block)
+    int64_t *d_cc_pbase = nullptr;      // [65] start of each bucket's pieces
     bd::td_task *d_cc_tasks = nullptr;  // slices of the long lists of the edge pass
     int64_t cc_ntasks = -1;             // -1: not built for the resident graph
     uint64_t *d_vis = nullptr;
