@@ -10,7 +10,9 @@ hybrid_bfs_main.cc:267-282).  TEPS = count_num_traversed_edges() / time
 (hybrid_bfs_main.cc:136-144).  The graph is generated on the GPU with the
 build's seeded RMAT generator and is resident in HBM before timing starts.
 
-Prints one JSON line.
+Prints one JSON line.  Beside the metric it carries the step kernel's
+roofline, the CPU baselines (one core and all cores), the 64-root batch and
+connected components (components::run) on the same resident graph.
 """
 import argparse
 import json
