@@ -49,7 +49,8 @@ class CCStats(ctypes.Structure):
     _fields_ = [("num_components", ctypes.c_int64), ("num_iters", ctypes.c_int64),
                 ("ms", ctypes.c_double), ("pass_ms", ctypes.c_double),
                 ("labels_read", ctypes.c_int64), ("edge_bytes", ctypes.c_int64),
-                ("pass_launches", ctypes.c_int64)]
+                ("pass_launches", ctypes.c_int64), ("prep_ms", ctypes.c_double),
+                ("prep_bytes", ctypes.c_int64)]
 
     def as_dict(self):
         return {f: getattr(self, f) for f, _ in self._fields_}
@@ -119,6 +120,7 @@ def lib():
         "bd_bfs_levels": ([vp, vp, i64, ctypes.POINTER(i64)], ctypes.c_int),
         "bd_bfs_count_traversed": ([vp, ctypes.POINTER(i64)], ctypes.c_int),
         "bd_bfs_check": ([vp, i64, ctypes.POINTER(ctypes.c_int)], ctypes.c_int),
+        "bd_bfs_check_parents": ([vp, i64, _i64p, ctypes.POINTER(ctypes.c_int)], ctypes.c_int),
         "bd_bfs_profile": ([vp, i64, i64, i64, ctypes.c_int, ctypes.POINTER(RunStats)],
                            ctypes.c_int),
         "bd_set_tuning": ([vp, ctypes.POINTER(Tuning)], ctypes.c_int),
@@ -322,8 +324,19 @@ class Context:
         return v.value
 
     def check(self, source):
+        """hybrid_bfs::check of the last search (hybrid_bfs.cc:281-379)."""
         ok = ctypes.c_int(0)
         self._ok(self._L.bd_bfs_check(self._h, int(source), ctypes.byref(ok)))
+        return bool(ok.value)
+
+    def check_parents(self, source, parent):
+        """hybrid_bfs::check of a given parent array (negative = unreached)."""
+        p = np.ascontiguousarray(parent, np.int64)
+        nv, _, _ = self.info()
+        if p.size != nv:
+            raise ValueError(f"parent array has {p.size} entries, the graph {nv} vertices")
+        ok = ctypes.c_int(0)
+        self._ok(self._L.bd_bfs_check_parents(self._h, int(source), p, ctypes.byref(ok)))
         return bool(ok.value)
 
     # ------------------------------------------------- connected components

@@ -8,11 +8,11 @@ objective is measured on a graph and roots that the benchmark does not use
 (RMAT generator seed 1; roots 65-96 of the lcg(0) sequence), so the result is
 not fitted to the benchmark's own inputs.
 
-    python -m beedrill_amd.autotune --scale 24 --out beedrill_amd/models/autotune_rmat_s24.json
+    python -m beedrill_amd.autotune --scale 24 --out beedrill_amd/models/tuning_rmat_s24.json
 
-Prints one line per trial and writes the best tuning and the trial log as
-JSON.  The library's compiled-in defaults are the values this found on
-MI355X.
+Prints one line per trial (the trial log) and writes the best tuning, its
+objective and the defaults' as JSON; --log FILE also keeps every trial.
+The library's compiled-in defaults are the values this found on MI355X.
 """
 from __future__ import annotations
 
@@ -93,7 +93,8 @@ def main():
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--roots", type=int, default=32)
     ap.add_argument("--passes", type=int, default=3)
-    ap.add_argument("--out", default=None)
+    ap.add_argument("--out", default=None, help="best tuning (JSON)")
+    ap.add_argument("--log", default=None, help="every trial (JSON)")
     args = ap.parse_args()
     ctx = bd.Context(0)
     t0 = time.time()
@@ -111,9 +112,13 @@ def main():
            "defaults": {k: defaults[k] for k in SPACE}, "defaults_gteps": base,
            "best": best, "best_gteps": confirm, "trials": len(trials),
            "seconds": round(time.time() - t0, 1), "log": trials}
-    print(json.dumps({k: v for k, v in out.items() if k != "log"}))
+    summary = {k: v for k, v in out.items() if k != "log"}
+    print(json.dumps(summary))
     if args.out:
         with open(args.out, "w") as f:
+            json.dump(summary, f, indent=1)
+    if args.log:
+        with open(args.log, "w") as f:
             json.dump(out, f, indent=1)
 
 

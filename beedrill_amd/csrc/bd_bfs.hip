@@ -1274,6 +1274,10 @@ __global__ __launch_bounds__(64) void k_init(step_args a, int64_t source,
     seg_counters *g = a.seg;
     __shared__ step_smem<64> s;
     const int l = lane_id();
+    // the timed region starts here: pushing the sources and the first
+    // direction choice are inside run_beamer's timed call
+    // (hybrid_bfs.cc:157-167, hybrid_bfs_main.cc:267-282)
+    if (l == 0) g->t_start = wall_clock64();
     const uint64_t below = (1ULL << l) - 1;
     for (int w = 0; w < 3; ++w) {
         for (int i = l; i <= kSeg; i += 64) {
@@ -1374,10 +1378,6 @@ __global__ __launch_bounds__(64) void k_init(step_args a, int64_t source,
     mlp_wave_regs w;
     if (a.policy == BD_POLICY_LEARNED && a.has_mlp) mlp_load(a.mlp_ops, w);
     choose_dir(a, s, c, -1, 0, 0, w);
-    if (l == 0) {
-        __threadfence();
-        g->t_start = wall_clock64();
-    }
 }
 
 // clear() (hybrid_bfs.cc:433-443) for the device layout.
@@ -1487,6 +1487,81 @@ __global__ void k_tree_depth(const uint64_t *__restrict__ t, int32_t *__restrict
         }
         if (t[(uint32_t)x] != (uint64_t)(uint32_t)x) err[2] = 1;  // ends at a non-source
         depth[v] = (int32_t)(x >> 32);
+    }
+}
+
+// hybrid_bfs::check (hybrid_bfs.cc:281-379) on the device, given the depths
+// of the parent tree (k_tree_*, which already failed any chain that does not
+// end at a self-parent vertex).  The reference runs a textbook BFS and
+// compares; here every reached vertex v is tested instead:
+//   - only `source` is its own parent (the reference fails any other
+//     self-parent: its parent is not one level up), and the source is reached;
+//   - v's parent is in v's list (the tree's depths make it one level up);
+//   - every neighbour u of v is reached and |depth u - depth v| <= 1.
+// The tests pass exactly when the reference's do: the reached set then
+// contains the source and is closed under adjacency, so it is the source's
+// component, and along a shortest path depths rise by at most one per edge
+// while the tree path to the source is depth v long, so depth v is the
+// distance.  Conversely a tree the reference accepts has BFS depths, which
+// pass every test.  err[0]: a parent is not a neighbour, err[1]: an edge
+// spans two levels or leaves the reached set, err[2]: a second root, or the
+// source unreached or not its own parent.  One wave per 64 vertices; lists
+// of 64 or more are scanned by the whole wave.
+__global__ __launch_bounds__(256) void k_check_tree(const int64_t *__restrict__ off,
+                                                    const int32_t *__restrict__ cols,
+                                                    const int32_t *__restrict__ parent,
+                                                    const int32_t *__restrict__ depth, int64_t nv,
+                                                    int64_t source, int32_t *__restrict__ err) {
+    const int lane = threadIdx.x & 63;
+    const int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const int64_t nwaves = ((int64_t)gridDim.x * blockDim.x) >> 6;
+    for (int64_t base = wave * 64; base < nv; base += nwaves * 64) {
+        const int64_t v = base + lane;
+        const bool valid = v < nv;
+        const int32_t d = valid ? depth[v] : -1;
+        const int32_t p = valid ? parent[v] : -1;
+        bool bad_root = false, bad_par = false, bad_edge = false;
+        if (valid && v == source && (d != 0 || p != (int32_t)v)) bad_root = true;
+        if (d >= 0 && p == (int32_t)v && v != source) bad_root = true;
+        int64_t b = 0, e = 0;
+        const bool scan = d >= 0;
+        if (scan) {
+            b = off[v];
+            e = off[v + 1];
+        }
+        const bool alone = scan && e - b < 64;
+        bool found = !scan || v == source;
+        if (alone) {
+            for (int64_t k = b; k < e; ++k) {
+                const int32_t u = cols[k];
+                const int32_t du = depth[u];
+                found |= u == p;
+                if (du < 0 || du > d + 1 || du < d - 1) bad_edge = true;
+            }
+        }
+        uint64_t lm = __ballot(scan && !alone);
+        while (lm) {
+            const int src = __ffsll((unsigned long long)lm) - 1;
+            lm &= lm - 1;
+            const int64_t kb = lane_bcast(b, src), ke = lane_bcast(e, src);
+            const int32_t ds = lane_bcast(d, src), ps = lane_bcast(p, src);
+            bool f = false, be = false;
+            for (int64_t k = kb + lane; k < ke; k += 64) {
+                const int32_t u = cols[k];
+                const int32_t du = depth[u];
+                f |= u == ps;
+                if (du < 0 || du > ds + 1 || du < ds - 1) be = true;
+            }
+            const bool fw = __ballot(f) != 0, bw = __ballot(be) != 0;
+            if (lane == src) {
+                found |= fw;
+                bad_edge |= bw;
+            }
+        }
+        bad_par = scan && !found;
+        if (__ballot(bad_par) && lane == 0) err[0] = 1;
+        if (__ballot(bad_edge) && lane == 0) err[1] = 1;
+        if (__ballot(bad_root) && lane == 0) err[2] = 1;
     }
 }
 
@@ -1943,9 +2018,13 @@ extern "C" int bd_bfs_set_script(bd_ctx *c, const int32_t *dirs, int64_t n) {
     return BD_OK;
 }
 
-// Depths of the last search into d_depth, from its parent tree (k_tree_*).
-int derive_depth(bd_ctx *c) {
-    if (c->depth_valid) return BD_OK;
+// Depths of the parent tree `parent` (nv entries, -1 unreached) into
+// `depth` (k_tree_*).  *why is null when the tree is well formed, else the
+// reason it is not: a reached vertex with an unreached or out-of-range
+// parent, a chain that never ends (a cycle), or one that ends at a vertex
+// that is not its own parent.  Returns a status only for runtime errors.
+int tree_depths(bd_ctx *c, const int32_t *parent, int32_t *depth, const char **why) {
+    *why = nullptr;
     const int64_t nv = c->nv;
     uint64_t *t = nullptr;
     // nv tree words, then err[4]
@@ -1957,7 +2036,7 @@ int derive_depth(bd_ctx *c) {
     } g{t};
     int32_t *err = (int32_t *)(t + nv);
     const dim3 grid(grid_for(c, nv, 256)), block(256);
-    hipLaunchKernelGGL(k_tree_init, grid, block, 0, c->stream, c->d_parent, t, nv);
+    hipLaunchKernelGGL(k_tree_init, grid, block, 0, c->stream, parent, t, nv);
     BD_HIP(c, hipGetLastError());
     int32_t h[4] = {1, 0, 0, 0};
     // a chain of length D takes ceil(log2 D) + 1 rounds; ids are 31 bits
@@ -1967,16 +2046,62 @@ int derive_depth(bd_ctx *c) {
         BD_HIP(c, hipGetLastError());
         BD_HIP(c, hipMemcpyAsync(h, err, sizeof(h), hipMemcpyDeviceToHost, c->stream));
         BD_HIP(c, hipStreamSynchronize(c->stream));
-        if (h[1])
-            return set_err(c, BD_EHIP, "parent tree: a reached vertex has an unreached parent");
+        if (h[1]) {
+            *why = "parent tree: a reached vertex has an unreached parent";
+            return BD_OK;
+        }
     }
-    if (h[0]) return set_err(c, BD_EHIP, "parent tree: chains do not end");
-    hipLaunchKernelGGL(k_tree_depth, grid, block, 0, c->stream, t, c->d_depth, nv, err);
+    if (h[0]) {
+        *why = "parent tree: chains do not end";
+        return BD_OK;
+    }
+    hipLaunchKernelGGL(k_tree_depth, grid, block, 0, c->stream, t, depth, nv, err);
     BD_HIP(c, hipGetLastError());
     BD_HIP(c, hipMemcpyAsync(h, err, sizeof(h), hipMemcpyDeviceToHost, c->stream));
     BD_HIP(c, hipStreamSynchronize(c->stream));
-    if (h[2]) return set_err(c, BD_EHIP, "parent tree: a chain closes on a non-source");
+    if (h[2]) *why = "parent tree: a chain closes on a non-source";
+    return BD_OK;
+}
+
+// Depths of the last search into d_depth, from its parent tree.
+int derive_depth(bd_ctx *c) {
+    if (c->depth_valid) return BD_OK;
+    const char *why = nullptr;
+    int rc = tree_depths(c, c->d_parent, c->d_depth, &why);
+    if (rc) return rc;
+    if (why) return set_err(c, BD_EHIP, why);
     c->depth_valid = true;
+    return BD_OK;
+}
+
+// check() of the parent tree `parent` from `source` on the device; `depth`
+// holds its depths when have_depth, else receives them (scratch of nv
+// entries when the tree is not the last search's).  *ok = 1 PASS, 0 FAIL.
+int check_tree(bd_ctx *c, const int32_t *parent, int32_t *depth, bool have_depth,
+               int64_t source, int *ok) {
+    int rc;
+    if (!have_depth) {
+        const char *why = nullptr;
+        if ((rc = tree_depths(c, parent, depth, &why))) return rc;
+        if (why) {
+            *ok = 0;
+            return BD_OK;
+        }
+    }
+    int32_t *err = nullptr;
+    if ((rc = dev_alloc(c, (void **)&err, sizeof(int32_t) * 4, "check flags"))) return rc;
+    struct guard {
+        int32_t *p;
+        ~guard() { dev_free(p); }
+    } g{err};
+    int32_t h[4] = {0, 0, 0, 0};
+    BD_HIP(c, hipMemsetAsync(err, 0, sizeof(h), c->stream));
+    hipLaunchKernelGGL(k_check_tree, dim3(grid_for(c, c->nv, 256)), dim3(256), 0, c->stream,
+                       c->d_off, c->d_cols, parent, depth, c->nv, source, err);
+    BD_HIP(c, hipGetLastError());
+    BD_HIP(c, hipMemcpyAsync(h, err, sizeof(h), hipMemcpyDeviceToHost, c->stream));
+    BD_HIP(c, hipStreamSynchronize(c->stream));
+    *ok = (h[0] | h[1] | h[2]) ? 0 : 1;
     return BD_OK;
 }
 
@@ -2030,57 +2155,49 @@ extern "C" int bd_bfs_count_traversed(bd_ctx *c, int64_t *out) {
     return BD_OK;
 }
 
-// hybrid_bfs::check (hybrid_bfs.cc:281-379): a fresh serial BFS gives the
-// depths; every reached vertex's parent must be a neighbour one level up, the
-// source its own parent, and reachability must agree.
+// hybrid_bfs::check (hybrid_bfs.cc:281-379) of the last search, on the
+// device (k_check_tree: the same PASS/FAIL as the reference's textbook-BFS
+// comparison, without a host copy of the graph).
 extern "C" int bd_bfs_check(bd_ctx *c, int64_t source, int *ok) {
     if (!c || !ok) return BD_EINVAL;
     if (!c->have_result) return set_err(c, BD_ESTATE, "no search has run");
     if (source < 0 || source >= c->nv) return set_err(c, BD_ERANGE, "source out of range");
+    BD_HIP(c, hipSetDevice(c->device));
+    if (!c->depth_valid) {
+        // a malformed tree is a FAIL, not an error
+        const char *why = nullptr;
+        int rc = tree_depths(c, c->d_parent, c->d_depth, &why);
+        if (rc) return rc;
+        if (why) {
+            *ok = 0;
+            return BD_OK;
+        }
+        c->depth_valid = true;
+    }
+    return check_tree(c, c->d_parent, c->d_depth, true, source, ok);
+}
+
+// hybrid_bfs::check of a caller's parent array (nv entries, negative =
+// unreached) against the resident graph, on the device.
+extern "C" int bd_bfs_check_parents(bd_ctx *c, int64_t source, const int64_t *parent, int *ok) {
+    if (!c || !ok || !parent) return BD_EINVAL;
+    if (!c->d_off) return set_err(c, BD_ESTATE, "no graph");
+    if (source < 0 || source >= c->nv) return set_err(c, BD_ERANGE, "source out of range");
+    BD_HIP(c, hipSetDevice(c->device));
     const int64_t nv = c->nv;
-    std::vector<int64_t> off(nv + 1);
-    std::vector<int32_t> cols(c->nnz);
-    std::vector<int64_t> parent(nv);
-    int rc = bd_graph_download(c, off.data(), cols.data());
-    if (!rc) rc = bd_bfs_fetch(c, nullptr, parent.data());
+    // out-of-range parents become nv, which the tree kernels reject
+    std::vector<int32_t> p(nv);
+    for (int64_t v = 0; v < nv; ++v)
+        p[v] = parent[v] < 0 ? -1 : (parent[v] >= nv ? (int32_t)nv : (int32_t)parent[v]);
+    int32_t *d = nullptr;
+    int rc = dev_alloc(c, (void **)&d, sizeof(int32_t) * 2 * nv, "checked tree");
     if (rc) return rc;
-    std::vector<int64_t> depth(nv, -1);
-    std::vector<int64_t> q;
-    q.reserve(nv);
-    q.push_back(source);
-    depth[source] = 0;
-    for (size_t h = 0; h < q.size(); ++h) {
-        const int64_t u = q[h];
-        for (int64_t e = off[u]; e < off[u + 1]; ++e) {
-            const int64_t v = cols[e];
-            if (depth[v] == -1) {
-                depth[v] = depth[u] + 1;
-                q.push_back(v);
-            }
-        }
-    }
-    bool correct = true;
-    for (int64_t u = 0; u < nv && correct; ++u) {
-        if (depth[u] >= 0 && parent[u] >= 0) {
-            if (u == source) {
-                if (!(parent[u] == u && depth[u] == 0)) correct = false;
-                continue;
-            }
-            bool found = false;
-            for (int64_t e = off[u]; e < off[u + 1]; ++e) {
-                if (cols[e] == parent[u]) {
-                    if (depth[cols[e]] != depth[u] - 1) break;
-                    found = true;
-                    break;
-                }
-            }
-            if (!found) correct = false;
-        } else if ((depth[u] < 0) != (parent[u] < 0)) {
-            correct = false;
-        }
-    }
-    *ok = correct ? 1 : 0;
-    return BD_OK;
+    struct guard {
+        int32_t *p;
+        ~guard() { dev_free(p); }
+    } g{d};
+    BD_HIP(c, hipMemcpy(d, p.data(), sizeof(int32_t) * nv, hipMemcpyHostToDevice));
+    return check_tree(c, d, d + nv, false, source, ok);
 }
 
 // Time both directions at every step from the same state; the search follows

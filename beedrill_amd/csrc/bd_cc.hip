@@ -641,7 +641,8 @@ extern "C" int bd_cc_run(bd_ctx *c, bd_cc_stats *out) {
     const dim3 grid(grid_for(c, nv, 256)), block(256);
     if (c->cc_ntasks < 0) {
         // slices of the long lists, once per graph (like the graph's own
-        // indexes, outside the timed run)
+        // indexes, outside the timed run; its time and memory are reported)
+        BD_HIP(c, hipEventRecord(c->ev[0], c->stream));
         const int64_t cap = c->nnz / kCcSlice + c->nnz / kCcHeavy + 1;
         dev_free(c->d_cc_tasks);  // left by a build that failed part-way
         c->d_cc_tasks = nullptr;
@@ -704,6 +705,15 @@ extern "C" int bd_cc_run(bd_ctx *c, bd_cc_stats *out) {
         BD_HIP(c, hipStreamSynchronize(c->stream));
         for (int i = 0; i < 64; ++i)
             if (pcnt[i] != pbase[i + 1]) return set_err(c, BD_EHIP, "long-list pieces miscounted");
+        BD_HIP(c, hipEventRecord(c->ev[1], c->stream));
+        BD_HIP(c, hipEventSynchronize(c->ev[1]));
+        float pms = 0;
+        BD_HIP(c, hipEventElapsedTime(&pms, c->ev[0], c->ev[1]));
+        c->cc_prep_ms = pms;
+        c->cc_prep_bytes = (int64_t)(sizeof(td_task) * cap + sizeof(int32_t) * c->nnz +
+                                     sizeof(uint64_t) * nv +
+                                     sizeof(td_task) * std::max<unsigned long long>(pbase[64], 1) +
+                                     sizeof(pbase));
         c->cc_bshift = bshift;
         c->cc_ntasks = (int64_t)n;
     }
@@ -792,6 +802,8 @@ extern "C" int bd_cc_run(bd_ctx *c, bd_cc_stats *out) {
         out->labels_read = (int64_t)nlab;
         out->edge_bytes = (int64_t)nbytes;
         out->pass_launches = launches;
+        out->prep_ms = c->cc_prep_ms;
+        out->prep_bytes = c->cc_prep_bytes;
     }
     return BD_OK;
 }

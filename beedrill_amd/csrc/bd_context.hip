@@ -59,6 +59,8 @@ static void free_search(bd_ctx *c) {
     c->d_cc_flag = nullptr;
     c->d_cc_tasks = nullptr;
     c->cc_ntasks = -1;
+    c->cc_prep_ms = 0;
+    c->cc_prep_bytes = 0;
     c->have_cc = false;
     dev_free(c->d_vis);
     dev_free(c->d_vis_save);
@@ -162,7 +164,7 @@ extern "C" int bd_create(bd_ctx **out, int device) {
     bd_ctx *c = new bd_ctx();
     c->device = device;
     // launch shape and thresholds: beedrill_amd/autotune.py's best on MI355X
-    // (RMAT scale 24, generator seed 1, models/autotune_rmat_s24.json)
+    // (RMAT scale 24, generator seed 1, models/tuning_rmat_s24.json)
     c->tune.block = 128;
     c->tune.grid_per_cu = 8;
     c->tune.bu_words = 64;

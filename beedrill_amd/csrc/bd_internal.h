@@ -128,6 +128,8 @@ struct bd_ctx {
     int64_t *d_cc_pbase = nullptr;      // [65] start of each bucket's pieces
     bd::td_task *d_cc_tasks = nullptr;  // slices of the long lists of the edge pass
     int64_t cc_ntasks = -1;             // -1: not built for the resident graph
+    double cc_prep_ms = 0;              // device time of that one-time build
+    int64_t cc_prep_bytes = 0;          // device memory it holds
     uint64_t *d_vis = nullptr;
     uint64_t *d_vis_save = nullptr;
     uint64_t *d_bm[3] = {nullptr, nullptr, nullptr};

@@ -41,17 +41,17 @@ static const struct option long_options[] = {{"graph_filename", required_argumen
 
 static void print_help(const char *argv0) {
     LOG("Usage: %s [OPTIONS]\n", argv0);
-    LOG("\t--graph_filename     Path to graph file to load\n");
-    LOG("\t--distributed_load   Accepted for compatibility; the file is read once\n");
-    LOG("\t--num_trials         Run the algorithm this many times.\n");
-    LOG("\t--dump_edge_list     Print the edge list to stdout after loading (slow)\n");
-    LOG("\t--check_graph        Validate the constructed graph against the edge list (slow)\n");
-    LOG("\t--dump_graph         Print the graph to stdout after construction (slow)\n");
-    LOG("\t--check_results      Validate the results (slow)\n");
-    LOG("\t--dump_results       Print the results to stdout (slow)\n");
-    LOG("\t--version            Print version info\n");
-    LOG("\t--help               Print command line help\n");
-    LOG("\t--generate SPEC      Generate graph500-scaleN or A-B-C-D-E-V.rmat on the GPU\n");
+    LOG("\t--graph_filename     el64 edge-list file to read (see --generate for a synthetic graph)\n");
+    LOG("\t--distributed_load   Kept for scripts that pass it; the file is always read once\n");
+    LOG("\t--num_trials         Number of labelling runs\n");
+    LOG("\t--dump_edge_list     Write the loaded edges to stdout (large output)\n");
+    LOG("\t--check_graph        Compare the built CSR with the edge list (takes a while)\n");
+    LOG("\t--dump_graph         Write the built CSR to stdout (large output)\n");
+    LOG("\t--check_results      Verify the component labels after the runs\n");
+    LOG("\t--dump_results       Write each vertex's component label to stdout\n");
+    LOG("\t--version            Show the build version and exit\n");
+    LOG("\t--help               Show this list and exit\n");
+    LOG("\t--generate SPEC      Build graph500-scaleN or A-B-C-D-E-V.rmat on the GPU\n");
     LOG("\t--seed N             Generator seed (default 0)\n");
     LOG("\t--device N           GPU ordinal (default 0)\n");
 }

@@ -52,22 +52,23 @@ static const struct option long_options[] = {{"graph_filename", required_argumen
 
 static void print_help(const char *argv0) {
     LOG("Usage: %s [OPTIONS]\n", argv0);
-    LOG("\t--graph_filename     Path to graph file to load\n");
-    LOG("\t--distributed_load   Accepted for compatibility; the file is read once\n");
-    LOG("\t--heavy_threshold    Accepted for compatibility (unused, as in the reference)\n");
-    LOG("\t--num_trials         Run BFS this many times.\n");
-    LOG("\t--source_vertex      Use this as the source vertex. If unspecified, pick random vertices.\n");
-    LOG("\t--algorithm          Select BFS implementation to run\n");
-    LOG("\t--alpha              Alpha parameter for direction-optimizing BFS\n");
-    LOG("\t--beta               Beta parameter for direction-optimizing BFS\n");
-    LOG("\t--sort_edge_blocks   Sort each adjacency list by vertex id\n");
-    LOG("\t--dump_edge_list     Print the edge list to stdout after loading (slow)\n");
-    LOG("\t--check_graph        Validate the constructed graph against the edge list (slow)\n");
-    LOG("\t--dump_graph         Print the graph to stdout after construction (slow)\n");
-    LOG("\t--check_results      Validate the BFS results (slow)\n");
-    LOG("\t--version            Print version info\n");
-    LOG("\t--help               Print command line help\n");
-    LOG("\t--generate SPEC      Generate graph500-scaleN or A-B-C-D-E-V.rmat on the GPU\n");
+    LOG("\t--graph_filename     el64 edge-list file to read (see --generate for a synthetic graph)\n");
+    LOG("\t--distributed_load   Kept for scripts that pass it; the file is always read once\n");
+    LOG("\t--heavy_threshold    Kept for scripts that pass it; has no effect\n");
+    LOG("\t--num_trials         Number of searches (one root each)\n");
+    LOG("\t--source_vertex      Root of every search; by default each search draws a root\n");
+    LOG("\t--algorithm          beamer_hybrid | migrating_threads | remote_writes |\n");
+    LOG("\t                     remote_writes_hybrid\n");
+    LOG("\t--alpha              Top-down to bottom-up switch factor (default 15)\n");
+    LOG("\t--beta               Bottom-up to top-down switch factor (default 18)\n");
+    LOG("\t--sort_edge_blocks   Order every adjacency list by neighbour id before searching\n");
+    LOG("\t--dump_edge_list     Write the loaded edges to stdout (large output)\n");
+    LOG("\t--check_graph        Compare the built CSR with the edge list (takes a while)\n");
+    LOG("\t--dump_graph         Write the built CSR to stdout (large output)\n");
+    LOG("\t--check_results      Verify every search's parent tree after it runs\n");
+    LOG("\t--version            Show the build version and exit\n");
+    LOG("\t--help               Show this list and exit\n");
+    LOG("\t--generate SPEC      Build graph500-scaleN or A-B-C-D-E-V.rmat on the GPU\n");
     LOG("\t--seed N             Generator seed (default 0)\n");
     LOG("\t--policy NAME        beamer (default) | learned | td_only | bu_only\n");
     LOG("\t--mlp_weights FILE   float32 parameters of the 8-32-32-2 direction network\n");

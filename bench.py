@@ -1,14 +1,21 @@
 """Benchmark: harmonic-mean GTEPS over 64 roots on RMAT scale 24, learned
 direction policy vs Beamer (alpha 15, beta 18), one MI355X.
 
-A step is one breadth-first search from one root (hybrid_bfs::run_beamer on
-the device, clear() untimed), its time the device time from the first step
-kernel to the end of the last (the reference times run_beamer alone,
-hybrid_bfs_main.cc:267-282).  TEPS = count_num_traversed_edges() / time
-(hybrid_bfs.cc:401-414); the aggregate is the harmonic mean over roots
-(combined.cc:108-146).  Roots are the reference's lcg(0) picks
-(hybrid_bfs_main.cc:136-144).  The graph is generated on the GPU with the
-build's seeded RMAT generator and is resident in HBM before timing starts.
+A search is one breadth-first search from one root (hybrid_bfs::run_beamer on
+the device, clear() untimed), its time the device time from the start of the
+search (the source pushed, the first direction chosen) to the end of its last
+level (the reference times run_beamer alone, hybrid_bfs_main.cc:267-282).
+TEPS = count_num_traversed_edges() / time (hybrid_bfs.cc:401-414).  The roots
+are the reference's first 64 lcg(0) picks (hybrid_bfs_main.cc:136-144), every
+pass.  A step of this benchmark is one pass of the reference's trial loop
+(hybrid_bfs_main.cc:252-306) over those 64 roots; --steps passes are timed
+after --warmup untimed ones, the two policies alternating pass by pass.  The
+metric aggregates every timed search as aggregate_stats does
+(combined.cc:108-146): harmonic mean, mean, sample stddev, min, max.  After
+timing, every root is searched again with each policy and the result checked
+(hybrid_bfs::check, as --check_results does, hybrid_bfs_main.cc:283-292).
+The graph is generated on the GPU with the build's seeded RMAT generator and
+is resident in HBM before timing starts.
 
 Prints one JSON line.  Beside the metric it carries the step kernel's
 roofline, the CPU baselines (one core and all cores), the 64-root batch and
@@ -43,27 +50,79 @@ def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
+NROOTS = 64  # the metric's roots: the first 64 lcg(0) picks
+
+
 def hmean(x):
     x = np.asarray(x, np.float64)
     return float(len(x) / np.sum(1.0 / x))
 
 
-def run_policy(ctx, roots, policy, warmup):
-    for r in roots[:warmup]:
-        ctx.bfs(int(r), 15, 18, policy)
-    gteps, ms, steps = [], [], []
-    t0 = time.perf_counter()
+def aggregate(x):
+    """aggregate_stats::compute (combined.cc:108-146): harmonic mean, mean,
+    sample standard deviation (N - 1), min and max."""
+    x = np.asarray(x, np.float64)
+    return {"n": int(len(x)), "hmean": hmean(x), "mean": float(np.mean(x)),
+            "stddev": float(np.std(x, ddof=1)) if len(x) > 1 else 0.0,
+            "min": float(np.min(x)), "max": float(np.max(x))}
+
+
+def search_pass(ctx, roots, policy):
+    """One pass of the trial loop: (GTEPS per root, device ms per root)."""
+    g, ms = [], []
     for r in roots:
         st = ctx.bfs(int(r), 15, 18, policy)
-        gteps.append(st["traversed_edges"] / (st["ms"] * 1e-3) / 1e9)
+        g.append(st["traversed_edges"] / (st["ms"] * 1e-3) / 1e9)
         ms.append(st["ms"])
-        steps.append((st["td_steps"], st["bu_steps"]))
+    return g, ms
+
+
+def run_policies(ctx, roots, policies, passes, warmup):
+    """warmup untimed passes, then `passes` timed passes over the same roots,
+    the policies taking turns pass by pass (so drift over the run touches
+    both alike)."""
+    for _ in range(warmup):
+        for pol in policies:
+            search_pass(ctx, roots, pol)
+    res = {pol: {"gteps": [], "ms": [], "pass_ms": [], "pass_hmean": []} for pol in policies}
+    t0 = time.perf_counter()
+    for _ in range(passes):
+        for pol in policies:
+            g, ms = search_pass(ctx, roots, pol)
+            r = res[pol]
+            r["gteps"] += g
+            r["ms"] += ms
+            r["pass_ms"].append(float(np.sum(ms)))
+            r["pass_hmean"].append(hmean(g))
     wall = time.perf_counter() - t0
-    return {"hmean_gteps": hmean(gteps), "mean_ms": float(np.mean(ms)),
-            "sum_edges_over_sum_time_gteps": float(
-                sum(g * m for g, m in zip(gteps, ms)) / sum(ms)),
-            "min_gteps": float(np.min(gteps)), "max_gteps": float(np.max(gteps)),
-            "wall_s": wall, "per_root_gteps": [round(g, 3) for g in gteps]}
+    out = {}
+    for pol, r in res.items():
+        agg = aggregate(r["gteps"])
+        ms = np.asarray(r["ms"])
+        out[pol] = {"hmean_gteps": agg["hmean"], "mean_gteps": agg["mean"],
+                    "stddev_gteps": agg["stddev"], "min_gteps": agg["min"],
+                    "max_gteps": agg["max"], "searches": agg["n"],
+                    "mean_ms": float(ms.mean()),
+                    "ms_per_pass": float(np.mean(r["pass_ms"])),
+                    "sum_edges_over_sum_time_gteps": float(
+                        np.sum(np.asarray(r["gteps"]) * ms) / ms.sum()),
+                    "pass_hmean_min": float(np.min(r["pass_hmean"])),
+                    "pass_hmean_max": float(np.max(r["pass_hmean"]))}
+    return out, wall
+
+
+def check_roots(ctx, roots, policies):
+    """hybrid_bfs::check on every root, once per policy (untimed), as the
+    reference's --check_results does after each trial
+    (hybrid_bfs_main.cc:283-292)."""
+    failed = []
+    for pol in policies:
+        for r in roots:
+            ctx.bfs(int(r), 15, 18, pol)
+            if not ctx.check(int(r)):
+                failed.append([pol, int(r)])
+    return {"roots": len(roots), "policies": list(policies), "failed": failed,
+            "pass": not failed}
 
 
 def bu_bytes(rec, nv, n_isolated):
@@ -179,7 +238,8 @@ def summarize_pmc(fetch_csv, write_csv, out_path, kernel_re=STEP_KERNEL_RE):
 
 def batched_roots(ctx, roots, sequential_ms):
     """The same 64 roots in one bit-parallel search (depths of every root
-    written), against the sum of the 64 sequential searches' times."""
+    written), against one pass of the 64 sequential searches (the mean over
+    the timed passes of the sum of their device times)."""
     ctx.multi_root(roots[:8], want_depths=False)  # warm up
     dp, ms, levels = ctx.multi_root(roots, want_depths=True)
     del dp
@@ -383,8 +443,9 @@ def cpu_baseline(ctx, roots, budget_s):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=64, help="timed roots per policy")
-    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--steps", type=int, default=10,
+                    help="timed passes of the trial loop over the 64 roots, per policy")
+    ap.add_argument("--warmup", type=int, default=2, help="untimed passes before them")
     ap.add_argument("--scale", type=int, default=24)
     ap.add_argument("--edge-factor", type=int, default=16)
     ap.add_argument("--seed", type=int, default=0)
@@ -438,7 +499,7 @@ def main():
     gen_s = time.perf_counter() - t0
     nv, nnz, _ = ctx.info()
     log(f"graph: scale {args.scale}, {nv} vertices, {m} edges, built in {gen_s:.2f} s")
-    roots = ctx.pick_roots(args.steps)
+    roots = ctx.pick_roots(NROOTS)
     # degree-0 vertices are never scanned by a bottom-up step
     off_h = np.empty(nv + 1, np.int64)
     ctx._ok(ctx._L.bd_graph_download(ctx._h, off_h.ctypes.data, None))
@@ -464,13 +525,18 @@ def main():
         print(json.dumps({"leg": "roofline", "roofline": roof, "copy_gbs": round(copy_gbs, 1)}))
         return
 
-    results = {"beamer": run_policy(ctx, roots, "beamer", args.warmup)}
-    log(f"beamer: {results['beamer']['hmean_gteps']:.2f} GTEPS (harmonic mean)")
-    if have_model:
-        results["learned"] = run_policy(ctx, roots, "learned", args.warmup)
-        log(f"learned: {results['learned']['hmean_gteps']:.2f} GTEPS (harmonic mean)")
-    head = results["learned"] if have_model else results["beamer"]
-    batch = batched_roots(ctx, roots, results["beamer"]["mean_ms"] * len(roots))
+    policies = ("beamer", "learned") if have_model else ("beamer",)
+    results, timed_s = run_policies(ctx, roots, policies, args.steps, args.warmup)
+    for pol in policies:
+        r = results[pol]
+        log(f"{pol}: {r['hmean_gteps']:.2f} GTEPS harmonic mean over {r['searches']} searches "
+            f"(stddev {r['stddev_gteps']:.1f}, {r['ms_per_pass']:.3f} ms per pass of "
+            f"{len(roots)} roots)")
+    head = results[policy]
+    checked = check_roots(ctx, roots, policies)
+    log(f"check: {'PASS' if checked['pass'] else 'FAIL'} on {len(roots)} roots x "
+        f"{len(policies)} policies")
+    batch = batched_roots(ctx, roots, results["beamer"]["ms_per_pass"])
     log(f"64 roots batched: {batch['ms']:.3f} ms vs {batch['sequential_ms']:.3f} ms sequential")
     roof = roofline(ctx, roots[:args.roofline_roots], policy, nv, n_isolated)
     log(f"roofline: {roof['achieved']} GB/s over {roof['launches']} step launches; bottom-up "
@@ -487,7 +553,7 @@ def main():
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(head["mean_ms"], 4),
+        "ms_per_step": round(head["ms_per_pass"], 4),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -495,16 +561,25 @@ def main():
         "data": "synthetic: seeded RMAT generated on the GPU (reference generator stream, "
                 "build-defined relabel/shuffle)",
         "config": {"workload": f"RMAT scale {args.scale}, edge factor {args.edge_factor}, "
-                               f"A/B/C/D .57/.19/.19/.05, seed {args.seed}, {args.steps} lcg(0) "
-                               f"roots, alpha 15 beta 18, policy {policy}",
+                               f"A/B/C/D .57/.19/.19/.05, seed {args.seed}, {len(roots)} lcg(0) "
+                               f"roots per step (one pass of the trial loop), alpha 15 beta 18, "
+                               f"policy {policy}",
+                   "roots": len(roots), "searches_per_policy": head["searches"],
                    "learned_margin_milli": ctx.get_tuning()["learned_margin_milli"],
                    "num_vertices": nv, "num_edges": m, "nnz": nnz,
                    "graph_build_s": round(gen_s, 2)},
+        "aggregate": {"hmean": round(head["hmean_gteps"], 3),
+                      "mean": round(head["mean_gteps"], 3),
+                      "stddev": round(head["stddev_gteps"], 3),
+                      "min": round(head["min_gteps"], 3), "max": round(head["max_gteps"], 3),
+                      "n": head["searches"]},
+        "check": checked["pass"],
+        "checked": checked,
+        "timed_s": round(timed_s, 3),
         "roofline": roof,
         "cpu_baseline": cpu,
         "batched_64_roots": batch,
-        "policies": {k: {kk: vv for kk, vv in v.items() if kk != "per_root_gteps"}
-                     for k, v in results.items()},
+        "policies": results,
         "learned_vs_beamer": (round(results["learned"]["hmean_gteps"] /
                                     results["beamer"]["hmean_gteps"], 4) if have_model else None),
         "components": {"metric": "components::run time on the same graph", **cc},

@@ -178,9 +178,20 @@ int bd_bfs_levels(bd_ctx *ctx, bd_level_rec *recs, int64_t cap, int64_t *n);
 /* hybrid_bfs::count_num_traversed_edges (hybrid_bfs.cc:401-414). */
 int bd_bfs_count_traversed(bd_ctx *ctx, int64_t *out);
 
-/* hybrid_bfs::check (hybrid_bfs.cc:281-379) against the last run, computed on
- * the host from a downloaded copy.  *ok = 1 PASS, 0 FAIL. */
+/* hybrid_bfs::check (hybrid_bfs.cc:281-379) against the last run, on the
+ * device: the same PASS/FAIL as the reference's textbook-BFS comparison
+ * (every parent a neighbour one level up, the source its own parent,
+ * reachability equal), tested as a tree whose every edge spans at most one
+ * level and whose reached set is closed under adjacency.
+ * *ok = 1 PASS, 0 FAIL. */
 int bd_bfs_check(bd_ctx *ctx, int64_t source, int *ok);
+
+/* hybrid_bfs::check (hybrid_bfs.cc:281-379) of a caller's parent array
+ * (nv entries; negative = unreached, as the reference's -deg(v) encoding
+ * after clear(), hybrid_bfs.cc:433-443) against the resident graph, on the
+ * device.  Lets a caller validate a tree from any source, e.g. the
+ * reference's own parent_ array, without a host copy of the graph. */
+int bd_bfs_check_parents(bd_ctx *ctx, int64_t source, const int64_t *parent, int *ok);
 
 /* Time both directions at every step from the same state (no reference
  * counterpart: the per-step hook sites are hybrid_bfs.cc:173-177, 184-188).
@@ -213,7 +224,12 @@ typedef struct {
                                  of a pull, per changed vertex of a push,
                                  and the changed-vertex bitmap              */
     int64_t pass_launches;    /* kernel launches of the edge passes (a tiled
-                                 pull pass takes 9)                         */
+                                 pull pass takes 8)                         */
+    double prep_ms;           /* device time of the per-graph regrouping the
+                                 passes use (slices of the long lists, lists
+                                 regrouped by column block), built by the
+                                 first run on a graph and not part of `ms` */
+    int64_t prep_bytes;       /* device memory that regrouping holds       */
 } bd_cc_stats;
 
 /* components::run (components.cc:38-104) on the resident graph: label

@@ -384,29 +384,6 @@ def test_argument_errors(ctx, orc):
     assert e.value.code == -6
 
 
-# ------------------------------------------------------------ RMAT scale 22
-
-def test_s22_depth_bit_exact_and_parents_valid(ctx, orc):
-    """Config: RMAT scale 22 (4,194,304 vertices, 67,108,864 generated edges).
-    Depth against the oracle's textbook BFS for 3 roots; parents through the
-    oracle's check(); per-level counters sum to the reached set."""
-    m = ctx.generate_rmat(22, 16)
-    off, adj = ctx.csr()
-    nv = len(off) - 1
-    assert nv == 1 << 22 and 0.85 * (16 << 22) < m < 16 << 22
-    deg = np.diff(off)
-    for root in ctx.pick_roots(3):
-        root = int(root)
-        st = ctx.bfs(root)
-        depth, parent = ctx.fetch()
-        np.testing.assert_array_equal(depth, orc.bfs_depth(off, adj, root))
-        assert orc.check(off, adj, root, parent)
-        lv = ctx.levels()
-        assert 1 + sum(r["next"] for r in lv) == st["reached"] == int((depth >= 0).sum())
-        assert st["traversed_edges"] == int(deg[depth >= 0].sum()) // 2
-        assert any(r["dir"] == bd.DIR_BU for r in lv)
-
-
 # ------------------------------------------------------------ batched searches
 
 def test_multi_root_matches_oracle(ctx, orc):
