@@ -68,11 +68,11 @@ class RunStats(ctypes.Structure):
 TUNING_FIELDS = ("block", "grid_per_cu", "wave_threshold", "task_threshold", "task_edges",
                  "bu_words", "bu_cap", "level_driver", "time_steps", "bu_head", "step_kernels",
                  "bu_grid_per_cu", "learned_margin_milli", "ms_alpha", "learned_min_skew",
-                 "td_split_edges")
+                 "td_split_edges", "vertex_order", "bu_batch")
 
 
 class Tuning(ctypes.Structure):
-    _fields_ = [(n, ctypes.c_int32) for n in TUNING_FIELDS] + [("reserved", ctypes.c_int32 * 1)]
+    _fields_ = [(n, ctypes.c_int32) for n in TUNING_FIELDS]
 
     def as_dict(self):
         return {f: getattr(self, f) for f in TUNING_FIELDS}

@@ -32,17 +32,19 @@ import beedrill_amd as bd  # noqa: E402
 
 # Values tried per field (all within the ranges bd_set_tuning accepts).
 SPACE = {
-    "bu_grid_per_cu": [4, 6, 8, 12, 16],
-    "bu_words": [16, 32, 48, 64],
+    "vertex_order": [1, 2, 3, 4],
+    "bu_batch": [1, 4],
+    "bu_grid_per_cu": [4, 6, 8, 10],
+    "bu_words": [4, 8, 16, 24, 32],
     "bu_cap": [8, 16, 32, 64, 128],
-    "grid_per_cu": [2, 3, 4, 5, 6, 8, 12],
+    "grid_per_cu": [4, 6, 8, 9, 10, 12],
     "wave_threshold": [16, 32, 64, 128, 256],
     "task_threshold": [16, 32, 48, 64, 96, 128, 256],
     "task_edges": [128, 256, 384, 768, 1536, 2048, 3072, 4096],
     "block": [128, 256, 512],
     "step_kernels": [1, 2],
     "bu_head": [1, 2],
-    "td_split_edges": [1 << 20, 1 << 21, 1 << 22, 1 << 23, 2147483647],
+    "td_split_edges": [1 << 14, 1 << 15, 1 << 16, 1 << 17, 1 << 18, 1 << 21, 2147483647],
 }
 
 

@@ -27,10 +27,11 @@
 //   concatenated by a block-wide scan and every thread takes one edge.
 //   Then waves take task slices.  A vertex is claimed by atomicOr on vis.
 //   Bottom-up (hybrid_bfs.cc:111-143): each wave owns chunks of `bu_words`
-//   visited words; lanes load the words, the unvisited vertices are ranked by
-//   a wave prefix sum, and then each lane takes one unvisited vertex and scans
-//   its list in CSR order, stopping at the first neighbour whose bit is set
-//   in bm[L%3].  Lists longer than bu_cap are finished by the whole wave.
+//   visited words; lanes load the words and list the unvisited vertices in
+//   LDS, and then each lane takes one unvisited vertex: its head index (the
+//   neighbour of largest degree) first, then its list in CSR order, stopping
+//   at the first neighbour whose bit is set in bm[L%3].  Lists longer than
+//   bu_cap are finished by the whole wave.
 //   Reading the depth-L bitmap, not vis, keeps the reference's two-phase rule
 //   (:117-141): a vertex found in this step is never a parent in this step.
 // A top-down step emits the next frontier as queue entries (staged in LDS and
@@ -46,6 +47,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <type_traits>
 #include <vector>
 
 #include "bd_device.h"
@@ -69,6 +71,12 @@ constexpr int64_t kPrecheckEdges = 1 << 16;  // top-down steps this heavy pre-ch
 constexpr int64_t kListEdges = 1 << 14;
 // Top-down rounds of 64 edges a wave visits together (see td_visit_n).
 constexpr int kTdRounds = 2;
+// Largest bottom-up chunk (visited words per wave): a chunk's unvisited
+// vertices are listed in LDS, 2 bytes each.
+constexpr int kBuChunkWords = 32;
+// Rounds of 64 bottom-up vertices whose head loads are issued together
+// (bu_batch > 1).
+constexpr int kBuBatch = 4;
 
 // A top-down step run with remote writes (hybrid_bfs.cc:41-67) takes two
 // launches: kDirTDWrite writes each frontier edge's source as the candidate
@@ -89,6 +97,7 @@ struct step_args {
     const int64_t *off;
     const int32_t *cols;
     const uint64_t *head;  // bottom-up head index: (degree << 32) | neighbour of largest degree
+    const int32_t *inv;    // graph id -> search id (null: the search runs on graph ids)
     uint64_t *vis;
     const uint64_t *front;
     uint64_t *next_bm;
@@ -106,6 +115,7 @@ struct step_args {
     const float *mlp;
     const uint4 *mlp_ops;  // the network's operands in MFMA lane order
     int64_t nwords;
+    int64_t live_nwords;   // words holding every vertex with edges (the sweeps cover these)
     int64_t task_cap;
     int64_t qseg_cap, tseg_cap;
     int64_t nv, m, nnz;
@@ -117,13 +127,14 @@ struct step_args {
     int32_t close_only;    // only publish the state before this step
     int32_t force_dir;     // -1: ctl->dir; 0/1: run this direction (profiling)
     int32_t wave_t, task_t, task_edges;
-    int32_t bu_words, bu_cap, bu_head;
+    int32_t bu_words, bu_cap, bu_head, bu_batch;
     int32_t policy, script_len, log_cap;
     int32_t mlp_h1, mlp_h2, has_mlp;
     float mlp_margin;      // log2-cost margin for the network to override Beamer
 };
 
 // ----------------------------------------------------------------- helpers
+
 
 // Index of the segment holding item i: the largest s with pref[s] <= i.
 __device__ __forceinline__ int seg_of(const int64_t *pref, int64_t i) {
@@ -165,9 +176,10 @@ struct step_smem {
             int32_t w_pref[BLOCK + 1];
             int32_t w_wsum[BLOCK / 64];
         };
-        struct {  // bottom-up: found bits of each wave's chunk, head misses,
-                  // first hit per pending list
+        struct {  // bottom-up: found bits of each wave's chunk, the chunk's
+                  // unvisited vertices, head misses, first hit per pending list
             uint64_t b_found[BLOCK / 64][64];
+            uint16_t b_list[BLOCK / 64][kBuChunkWords * 64];
             int32_t b_pend[BLOCK / 64][128];
             unsigned long long b_hit[BLOCK / 64][64];
         };
@@ -421,7 +433,8 @@ __device__ __forceinline__ void td_visit_n(const step_args &a, step_smem<BLOCK> 
         const uint64_t bit = 1ULL << (u[r] & 63);
         old[r] = ~0ULL;  // no claim
         if (valid[r] && !(seen[r] & bit))
-            old[r] = atomicOr((unsigned long long *)(a.vis + (u[r] >> 6)), (unsigned long long)bit);
+            old[r] = atomicOr((unsigned long long *)(a.vis + (u[r] >> 6)),
+                              (unsigned long long)bit);
     }
     bool won[kTdRounds];
     int64_t b[kTdRounds], e[kTdRounds];
@@ -596,10 +609,10 @@ template <int BLOCK>
 __device__ void td_bitmap_body(const step_args &a, step_smem<BLOCK> &s, bool tdw, unsigned long long &cnt,
                                unsigned long long &dsum, unsigned long long &dmax) {
     const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
-    const int64_t nranges = (a.nwords + BLOCK - 1) / BLOCK;
+    const int64_t nranges = (a.live_nwords + BLOCK - 1) / BLOCK;
     for (int64_t r = blockIdx.x; r < nranges; r += gridDim.x) {
         const int64_t wi = r * BLOCK + tid;
-        const uint64_t w = wi < a.nwords ? a.front[wi] : 0ULL;
+        const uint64_t w = wi < a.live_nwords ? a.front[wi] : 0ULL;
         const int32_t c = __popcll(w);
         s.w_bits[tid] = w;
         const int32_t incl = wave_incl_scan32(c);
@@ -722,27 +735,33 @@ __device__ void bu_pending(const step_args &a, step_smem<BLOCK> &s, const int32_
     wave_lds_sync();  // hit[] is rewritten by the next batch
 }
 
-// Rank g of a chunk's unvisited vertices -> word j, bit bpos.  Lane i holds
-// word i's unvisited bits `unv` and their exclusive rank prefix `excl`;
-// `total` is the chunk's count (ranks past it give the last vertex).
-__device__ __forceinline__ void rank_locate(int g, int total, int excl, uint64_t unv, int &j,
-                                            int &bpos) {
-    const int gg = g < total ? g : total - 1;
-    j = 0;
-#pragma unroll
-    for (int st = 32; st > 0; st >>= 1) {
-        const int mid = j + st;
-        const int em = __shfl(excl, mid < 64 ? mid : 63);
-        if (mid < 64 && em <= gg) j = mid;
-    }
-    const uint64_t bits = (uint64_t)__shfl((long long)unv, j);
-    bpos = select_bit(bits, gg - __shfl(excl, j));
+// List the unvisited vertices of a wave's chunk in LDS: lane i holds visited
+// word w0 + i (i < CW), and writes the positions (chunk-relative, word * 64 +
+// bit) of its word's clear bits, in order, after those of the lanes before
+// it.  Returns the count (wave-uniform); `unv` receives the lane's clear
+// bits.  One pass over the bits instead of a rank search per vertex.
+__device__ __forceinline__ int bu_list_chunk(const step_args &a, uint16_t *list, int64_t w0,
+                                             int CW, uint64_t &vw, uint64_t &unv) {
+    const int lane = lane_id();
+    const int64_t wi = w0 + lane;
+    const bool own = lane < CW && wi < a.live_nwords;
+    vw = own ? a.vis[wi] : ~0ULL;
+    if (own) a.clr[wi] = 0;
+    unv = ~vw;
+    const int cntw = __popcll(unv);
+    int pos = wave_incl_scan32(cntw);
+    const int total = lane_bcast(pos, 63);
+    pos -= cntw;
+    for (uint64_t u = unv; u; u &= u - 1)
+        list[pos++] = (uint16_t)(lane * 64 + (__ffsll((unsigned long long)u) - 1));
+    wave_lds_sync();
+    return total;
 }
 
 // Claim half of a top-down step run with remote writes (hybrid_bfs.cc:56-66):
 // every unvisited vertex with a candidate takes it as its parent and joins
 // the next frontier, which is left in the bitmap only, as after a bottom-up
-// step.  Waves own chunks of bu_words visited words and rank their unvisited
+// step.  Waves own chunks of bu_words visited words and list their unvisited
 // vertices as the bottom-up step does; each consumed candidate is reset, so
 // the array is all -1 again when the step ends.
 template <int BLOCK>
@@ -750,40 +769,50 @@ __device__ void td_claim_body(const step_args &a, step_smem<BLOCK> &s, unsigned 
                               unsigned long long &dsum, unsigned long long &dmax) {
     const int lane = lane_id(), wid = threadIdx.x >> 6;
     const int CW = a.bu_words;
-    const int64_t nchunks = (a.nwords + CW - 1) / CW;
+    const int64_t nchunks = (a.live_nwords + CW - 1) / CW;
+    uint16_t *list = s.b_list[wid];
     for (int64_t chunk = (int64_t)blockIdx.x * (BLOCK / 64) + wid; chunk < nchunks;
          chunk += (int64_t)gridDim.x * (BLOCK / 64)) {
         const int64_t w0 = chunk * CW;
-        const int64_t wi = w0 + lane;
-        const bool own = lane < CW && wi < a.nwords;
-        const uint64_t vw = own ? a.vis[wi] : ~0ULL;
-        if (own) a.clr[wi] = 0;
-        const uint64_t unv = ~vw;
-        const int cntw = __popcll(unv);
-        int excl = wave_incl_scan32(cntw);
-        const int total = lane_bcast(excl, 63);
-        excl -= cntw;
         s.b_found[wid][lane] = 0;
-        wave_lds_sync();
-        for (int r = 0; r < total; r += 64) {
-            int j, bpos;
-            rank_locate(r + lane, total, excl, unv, j, bpos);
-            const int64_t v = (w0 + j) * 64 + bpos;
-            const int32_t p = r + lane < total ? __builtin_nontemporal_load(a.cand + v) : -1;
-            if (p >= 0) {
+        uint64_t vw, unv;
+        const int total = bu_list_chunk(a, list, w0, CW, vw, unv);
+        // kBuBatch rounds of 64 vertices at a time: their candidates are
+        // loaded together, then the degrees of the claimed ones
+        for (int r0 = 0; r0 < total; r0 += 64 * kBuBatch) {
+            int idx[kBuBatch];
+            int32_t p[kBuBatch];
+#pragma unroll
+            for (int k = 0; k < kBuBatch; ++k) {
+                const int g = r0 + 64 * k + lane;
+                idx[k] = g < total ? list[g] : 0;
+                p[k] = g < total ? __builtin_nontemporal_load(a.cand + w0 * 64 + idx[k]) : -1;
+            }
+            unsigned long long d[kBuBatch];
+#pragma unroll
+            for (int k = 0; k < kBuBatch; ++k) {
+                const int64_t v = w0 * 64 + idx[k];
+                d[k] = 0;
+                if (p[k] >= 0)
+                    d[k] = a.bu_head ? (unsigned long long)(a.head[v] >> 32)
+                                     : (unsigned long long)(a.off[v + 1] - a.off[v]);
+            }
+#pragma unroll
+            for (int k = 0; k < kBuBatch; ++k) {
+                if (p[k] < 0) continue;
+                const int64_t v = w0 * 64 + idx[k];
                 a.cand[v] = -1;
-                a.parent[v] = p;
-                const unsigned long long d =
-                    a.bu_head ? (unsigned long long)(a.head[v] >> 32)
-                              : (unsigned long long)(a.off[v + 1] - a.off[v]);
+                a.parent[v] = p[k];
                 ++cnt;
-                dsum += d;
-                dmax = d > dmax ? d : dmax;
-                atomicOr((unsigned long long *)&s.b_found[wid][j], (unsigned long long)(1ULL << bpos));
+                dsum += d[k];
+                dmax = d[k] > dmax ? d[k] : dmax;
+                atomicOr((unsigned long long *)&s.b_found[wid][idx[k] >> 6],
+                         (unsigned long long)(1ULL << (idx[k] & 63)));
             }
         }
         wave_lds_sync();
-        if (own) {
+        const int64_t wi = w0 + lane;
+        if (lane < CW && wi < a.live_nwords) {
             const uint64_t f = s.b_found[wid][lane];
             a.next_bm[wi] = f;
             if (f) a.vis[wi] = vw | f;
@@ -792,58 +821,37 @@ __device__ void td_claim_body(const step_args &a, step_smem<BLOCK> &s, unsigned 
     }
 }
 
-// Bottom-up step.  Each wave owns chunks of bu_words visited words and ranks
-// their unvisited vertices with a wave prefix sum; per round of 64 vertices
-// each lane reads its vertex's head index (the neighbour most likely to be
-// in the frontier, read from a dense array in vertex order) and tests it.
-// Vertices whose head misses are queued in LDS and their lists scanned 64 at
-// a time, so the rare long scans do not hold up every round.  Found bits are
-// gathered in LDS and written by the word owner: no global atomics.
+// Bottom-up step.  Each wave owns chunks of bu_words visited words and lists
+// their unvisited vertices in LDS; per round of 64 vertices each lane reads
+// its vertex's head index (the neighbour most likely to be in the frontier,
+// read from a dense array in vertex order) and tests it.  Vertices whose
+// head misses are queued in LDS and their lists scanned 64 at a time, so the
+// rare long scans do not hold up every round.  Found bits are gathered in
+// LDS and written by the word owner: no global atomics.
 template <int BLOCK>
 __device__ void bu_body(const step_args &a, step_smem<BLOCK> &s, unsigned long long &cnt,
                         unsigned long long &dsum, unsigned long long &dmax,
                         unsigned long long &exam) {
     const int lane = lane_id(), wid = threadIdx.x >> 6;
     const int CW = a.bu_words;
-    const int64_t nchunks = (a.nwords + CW - 1) / CW;
+    const int64_t nchunks = (a.live_nwords + CW - 1) / CW;
     const uint64_t below = (1ULL << lane) - 1;
     int32_t *pend = s.b_pend[wid];
+    uint16_t *list = s.b_list[wid];
     for (int64_t chunk = (int64_t)blockIdx.x * (BLOCK / 64) + wid; chunk < nchunks;
          chunk += (int64_t)gridDim.x * (BLOCK / 64)) {
         const int64_t w0 = chunk * CW;
-        const int64_t wi = w0 + lane;
-        const bool own = lane < CW && wi < a.nwords;
-        const uint64_t vw = own ? a.vis[wi] : ~0ULL;
-        if (own) a.clr[wi] = 0;
-        const uint64_t unv = ~vw;
-        const int cntw = __popcll(unv);
-        int excl = wave_incl_scan32(cntw);
-        const int total = lane_bcast(excl, 63);
-        excl -= cntw;
         s.b_found[wid][lane] = 0;
+        uint64_t vw, unv;
+        const int total = bu_list_chunk(a, list, w0, CW, vw, unv);
         int np = 0;  // pending vertices (wave-uniform)
-        wave_lds_sync();
-        auto locate = [&](int g, int &j, int &bpos) { rank_locate(g, total, excl, unv, j, bpos); };
-        // Rounds of 64 vertices, software-pipelined: the next round's head
-        // entry is loaded while this round's frontier word is in flight.
-        int j, bpos;
-        locate(lane, j, bpos);
-        bool valid = lane < total;
-        uint64_t hd =
-            (valid && a.bu_head) ? __builtin_nontemporal_load(a.head + (w0 + j) * 64 + bpos) : 0ULL;
-        for (int r = 0; r < total; r += 64) {
-            const int64_t v = (w0 + j) * 64 + bpos;
+        // One vertex per lane: test its head entry hd against frontier word
+        // fw; on a miss queue it for the list scan.  Wave-collective.
+        auto visit = [&](bool valid, int idx, uint64_t hd, uint64_t fw) {
+            const int64_t v = w0 * 64 + idx;
             const int32_t h = (int32_t)(uint32_t)hd;
-            const bool probe = valid && a.bu_head && h >= 0;
-            const uint64_t fw = probe ? a.front[h >> 6] : 0ULL;
-            int jn, bposn;
-            locate(r + 64 + lane, jn, bposn);
-            const bool validn = r + 64 + lane < total;
-            const uint64_t hdn = (validn && a.bu_head)
-                                     ? __builtin_nontemporal_load(a.head + (w0 + jn) * 64 + bposn)
-                                     : 0ULL;
             bool miss = valid;
-            if (probe) {
+            if (valid && a.bu_head && h >= 0) {
                 ++exam;
                 if ((hd >> 32) == 1 && !((fw >> (h & 63)) & 1)) {
                     // the only neighbour is the head, which missed: the list
@@ -857,14 +865,10 @@ __device__ void bu_body(const step_args &a, step_smem<BLOCK> &s, unsigned long l
                     ++cnt;
                     dsum += d;
                     dmax = d > dmax ? d : dmax;
-                    atomicOr((unsigned long long *)&s.b_found[wid][j],
-                             (unsigned long long)(1ULL << bpos));
+                    atomicOr((unsigned long long *)&s.b_found[wid][idx >> 6],
+                             (unsigned long long)(1ULL << (idx & 63)));
                 }
             }
-            j = jn;
-            bpos = bposn;
-            valid = validn;
-            hd = hdn;
             const uint64_t mm = __ballot(miss);
             if (miss) pend[np + __popcll(mm & below)] = (int32_t)v;
             np += __popcll(mm);
@@ -878,13 +882,65 @@ __device__ void bu_body(const step_args &a, step_smem<BLOCK> &s, unsigned long l
                 np = rest;
                 wave_lds_sync();
             }
+        };
+        auto head_of = [&](bool valid, int idx) -> uint64_t {
+            return (valid && a.bu_head) ? __builtin_nontemporal_load(a.head + w0 * 64 + idx)
+                                        : 0ULL;
+        };
+        auto front_of = [&](bool valid, uint64_t hd) -> uint64_t {
+            const int32_t h = (int32_t)(uint32_t)hd;
+            return (valid && a.bu_head && h >= 0) ? a.front[h >> 6] : 0ULL;
+        };
+        // R rounds of 64 vertices at a time: every round's head entry, then
+        // every round's frontier word, are loaded before any is used, so the
+        // wave waits one chain of two loads per 64 * R vertices
+        auto batched = [&](auto rounds) {
+            constexpr int R = decltype(rounds)::value;
+            for (int r0 = 0; r0 < total; r0 += 64 * R) {
+                bool valid[R];
+                int idx[R];
+                uint64_t hd[R], fw[R];
+#pragma unroll
+                for (int k = 0; k < R; ++k) {
+                    const int g = r0 + 64 * k + lane;
+                    valid[k] = g < total;
+                    idx[k] = valid[k] ? list[g] : 0;
+                    hd[k] = head_of(valid[k], idx[k]);
+                }
+#pragma unroll
+                for (int k = 0; k < R; ++k) fw[k] = front_of(valid[k], hd[k]);
+#pragma unroll
+                for (int k = 0; k < R; ++k)
+                    if (r0 + 64 * k < total) visit(valid[k], idx[k], hd[k], fw[k]);
+            }
+        };
+        if (a.bu_batch > 1) {
+            batched(std::integral_constant<int, kBuBatch>{});
+        } else {
+            // rounds of 64 vertices, software-pipelined: the next round's
+            // head entry is loaded while this round's frontier word is in
+            // flight
+            bool valid = lane < total;
+            int idx = valid ? list[lane] : 0;
+            uint64_t hd = head_of(valid, idx);
+            for (int r = 0; r < total; r += 64) {
+                const uint64_t fw = front_of(valid, hd);
+                const bool validn = r + 64 + lane < total;
+                const int idxn = validn ? list[r + 64 + lane] : 0;
+                const uint64_t hdn = head_of(validn, idxn);
+                visit(valid, idx, hd, fw);
+                valid = validn;
+                idx = idxn;
+                hd = hdn;
+            }
         }
         if (np > 0) {
             wave_lds_sync();
             bu_pending(a, s, pend, np, w0, cnt, dsum, dmax, exam);
         }
         wave_lds_sync();
-        if (own) {
+        const int64_t wi = w0 + lane;
+        if (lane < CW && wi < a.live_nwords) {
             const uint64_t f = s.b_found[wid][lane];
             a.next_bm[wi] = f;
             if (f) a.vis[wi] = vw | f;
@@ -1245,7 +1301,7 @@ __global__ __launch_bounds__(BLOCK) void k_step(step_args a) {
     unsigned long long cnt = 0, dsum = 0, exam = 0, dmax = 0;
     if (DIRS != kStepBU && kdir == BD_DIR_TD) {
         const bool tdw = dir == kDirTDWrite;
-        for (int64_t i = (int64_t)blockIdx.x * BLOCK + threadIdx.x; i < a.nwords;
+        for (int64_t i = (int64_t)blockIdx.x * BLOCK + threadIdx.x; i < a.live_nwords;
              i += (int64_t)gridDim.x * BLOCK)
             a.clr[i] = 0;
         if (s.st.td_bitmap)
@@ -1297,7 +1353,8 @@ __global__ __launch_bounds__(64) void k_init(step_args a, int64_t source,
     for (int64_t i0 = 0; i0 < nsrc; i0 += 64) {
         const int64_t i = i0 + l;
         const bool valid = i < nsrc;
-        const int64_t v = valid ? (srcs ? srcs[i] : source) : 0;
+        int64_t v = valid ? (srcs ? srcs[i] : source) : 0;
+        if (a.inv) v = a.inv[v];  // sources are given in graph ids
         bool won = false;
         int64_t b = 0, e = 0;
         if (valid) {
@@ -1380,15 +1437,16 @@ __global__ __launch_bounds__(64) void k_init(step_args a, int64_t source,
     choose_dir(a, s, c, -1, 0, 0, w);
 }
 
-// clear() (hybrid_bfs.cc:433-443) for the device layout.
-__global__ void k_clear(int32_t *__restrict__ parent, int64_t nv, uint64_t *__restrict__ vis,
+// clear() (hybrid_bfs.cc:433-443) for the device layout: visited = the
+// degree-0 vertices, frontier bitmaps zero.
+__global__ void k_clear(step_args a, int32_t *__restrict__ parent, int64_t nv,
                         const uint64_t *__restrict__ dead, uint64_t *__restrict__ b0,
                         uint64_t *__restrict__ b1, uint64_t *__restrict__ b2, int64_t nwords) {
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nv; i += stride)
         parent[i] = -1;
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nwords; i += stride) {
-        vis[i] = dead[i];
+        a.vis[i] = dead[i];
         b0[i] = 0;
         b1[i] = 0;
         b2[i] = 0;
@@ -1429,6 +1487,23 @@ __global__ void k_count_traversed(const int32_t *__restrict__ parent,
         if (parent[v] >= 0) s += (unsigned long long)(off[v + 1] - off[v]);
     s = wave_sum(s);
     if ((threadIdx.x & 63) == 0 && s) atomicAdd(&out[(blockIdx.x % kSlots) * kLine], s);
+}
+
+// A search's result in graph ids: vertex v is search id inv[v], and a parent
+// p (a search id) is graph vertex perm[p].  Either output may be null.
+__global__ void k_to_graph_ids(const int32_t *__restrict__ inv, const int32_t *__restrict__ perm,
+                               const int32_t *__restrict__ sparent,
+                               const int32_t *__restrict__ sdepth, int64_t nv,
+                               int32_t *__restrict__ parent, int32_t *__restrict__ depth) {
+    for (int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v < nv;
+         v += (int64_t)gridDim.x * blockDim.x) {
+        const int32_t i = inv[v];
+        if (parent) {
+            const int32_t p = sparent[i];
+            parent[v] = p < 0 ? -1 : perm[p];
+        }
+        if (depth) depth[v] = sdepth[i];
+    }
 }
 
 // Depths from the parent tree.  The steps write parents only, as
@@ -1647,9 +1722,10 @@ int norm_block(int b) { return (b == 128 || b == 512) ? b : 256; }
 
 step_args make_args(bd_ctx *c, int32_t level, int64_t alpha, int64_t beta, int policy) {
     step_args a;
-    a.off = c->d_off;
-    a.cols = c->d_cols;
+    a.off = c->d_soff;
+    a.cols = c->d_scols;
     a.head = c->d_head;
+    a.inv = c->d_inv;
     a.vis = c->d_vis;
     a.front = c->d_bm[level % 3];
     a.next_bm = c->d_bm[(level + 1) % 3];
@@ -1667,6 +1743,11 @@ step_args make_args(bd_ctx *c, int32_t level, int64_t alpha, int64_t beta, int p
     a.mlp = c->d_mlp;
     a.mlp_ops = c->d_mlp_ops;
     a.nwords = c->nwords;
+    // vertices past nv_active have no edges when the layout is renumbered:
+    // clear() sets their visited bits, no step finds them, and only an
+    // isolated source (whose search ends after one step) is ever in a
+    // frontier, so no sweep reads or clears their words
+    a.live_nwords = c->d_perm ? std::max<int64_t>(1, (c->nv_active + 63) / 64) : c->nwords;
     a.task_cap = c->task_cap;
     a.qseg_cap = c->qseg_cap;
     a.tseg_cap = c->tseg_cap;
@@ -1684,9 +1765,10 @@ step_args make_args(bd_ctx *c, int32_t level, int64_t alpha, int64_t beta, int p
     a.wave_t = std::max(1, c->tune.wave_threshold);
     a.task_t = std::max(16, c->tune.task_threshold);
     a.task_edges = std::max(64, c->tune.task_edges);
-    a.bu_words = c->tune.bu_words > 0 ? std::min(64, c->tune.bu_words) : 32;
+    a.bu_words = c->tune.bu_words > 0 ? std::min(kBuChunkWords, c->tune.bu_words) : kBuChunkWords;
     a.bu_cap = c->tune.bu_cap > 0 ? std::min(c->tune.bu_cap, 1 << 16) : 32;
     a.bu_head = c->tune.bu_head == 2 ? 0 : 1;
+    a.bu_batch = c->tune.bu_batch;
     a.policy = policy;
     a.script_len = (int32_t)c->script_len;
     a.log_cap = (int32_t)c->log_cap;
@@ -1807,8 +1889,10 @@ int check_run_args(bd_ctx *c, int64_t source, int64_t alpha, int64_t beta, int p
 int start_search(bd_ctx *c, int64_t source, int64_t alpha, int64_t beta, int policy,
                  const int64_t *srcs = nullptr, int64_t nsrc = 1, hipEvent_t init_end = nullptr) {
     const int64_t n = std::max(c->nv, c->nwords);
-    hipLaunchKernelGGL(k_clear, dim3(grid_for(c, n, 256)), dim3(256), 0, c->stream, c->d_parent,
-                       c->nv, c->d_vis, c->d_dead, c->d_bm[0], c->d_bm[1], c->d_bm[2], c->nwords);
+    const step_args ca = make_args(c, 0, alpha, beta, policy);
+    hipLaunchKernelGGL(k_clear, dim3(grid_for(c, n, 256)), dim3(256), 0, c->stream, ca,
+                       c->d_parent, c->nv, c->d_sdead, c->d_bm[0], c->d_bm[1], c->d_bm[2],
+                       c->nwords);
     if (!c->cand_clean)
         hipLaunchKernelGGL(k_fill_cand, dim3(grid_for(c, c->nv, 256)), dim3(256), 0, c->stream,
                            c->d_cand, c->nv);
@@ -1882,7 +1966,7 @@ int build_head_index(bd_ctx *c) {
     } g{d_max};
     BD_HIP(c, hipMemsetAsync(d_max, 0, sizeof(unsigned long long), c->stream));
     hipLaunchKernelGGL(k_head_index, dim3(grid_for(c, c->nv, 256)), dim3(256), 0, c->stream,
-                       c->d_off, c->d_cols, c->nv, c->d_head, d_max);
+                       c->d_soff, c->d_scols, c->nv, c->d_head, d_max);
     BD_HIP(c, hipGetLastError());
     unsigned long long h = 0;
     BD_HIP(c, hipMemcpyAsync(&h, d_max, sizeof(h), hipMemcpyDeviceToHost, c->stream));
@@ -2105,20 +2189,54 @@ int check_tree(bd_ctx *c, const int32_t *parent, int32_t *depth, bool have_depth
     return BD_OK;
 }
 
+// The last search's parents (and, with want_depth, depths) in graph ids on
+// the device: the search's own arrays when it ran on graph ids, else mapped
+// into `buf` (allocated here, 2 nv entries; the caller frees it).
+int graph_id_result(bd_ctx *c, bool want_depth, int32_t **buf, const int32_t **parent,
+                    const int32_t **depth) {
+    int rc;
+    *buf = nullptr;
+    if (want_depth && (rc = derive_depth(c))) return rc;
+    if (!c->d_perm) {
+        *parent = c->d_parent;
+        *depth = c->d_depth;
+        return BD_OK;
+    }
+    if ((rc = dev_alloc(c, (void **)buf, sizeof(int32_t) * 2 * c->nv, "results in graph ids")))
+        return rc;
+    hipLaunchKernelGGL(k_to_graph_ids, dim3(grid_for(c, c->nv, 256)), dim3(256), 0, c->stream,
+                       c->d_inv, c->d_perm, c->d_parent, c->d_depth, c->nv, *buf,
+                       want_depth ? *buf + c->nv : nullptr);
+    BD_HIP(c, hipGetLastError());
+    *parent = *buf;
+    *depth = *buf + c->nv;
+    return BD_OK;
+}
+
 extern "C" int bd_bfs_fetch(bd_ctx *c, int32_t *depth, int64_t *parent) {
     if (!c) return BD_EINVAL;
     if (!c->have_result) return set_err(c, BD_ESTATE, "no search has run");
+    if (!depth && !parent) return BD_OK;
     BD_HIP(c, hipSetDevice(c->device));
-    if (depth) {
-        int rc = derive_depth(c);
-        if (rc) return rc;
-        BD_HIP(c, hipMemcpy(depth, c->d_depth, sizeof(int32_t) * c->nv, hipMemcpyDeviceToHost));
-    }
+    int32_t *buf = nullptr;
+    const int32_t *dp = nullptr, *dd = nullptr;
+    int rc = graph_id_result(c, depth != nullptr, &buf, &dp, &dd);
+    struct guard {
+        int32_t *p;
+        ~guard() { dev_free(p); }
+    } g{buf};
+    if (rc) return rc;
+    if (depth)
+        BD_HIP(c, hipMemcpyAsync(depth, dd, sizeof(int32_t) * c->nv, hipMemcpyDeviceToHost,
+                                 c->stream));
+    std::vector<int32_t> p;
     if (parent) {
-        std::vector<int32_t> p(c->nv);
-        BD_HIP(c, hipMemcpy(p.data(), c->d_parent, sizeof(int32_t) * c->nv, hipMemcpyDeviceToHost));
-        for (int64_t v = 0; v < c->nv; ++v) parent[v] = p[v];
+        p.resize(c->nv);
+        BD_HIP(c, hipMemcpyAsync(p.data(), dp, sizeof(int32_t) * c->nv, hipMemcpyDeviceToHost,
+                                 c->stream));
     }
+    BD_HIP(c, hipStreamSynchronize(c->stream));
+    for (int64_t v = 0; parent && v < c->nv; ++v) parent[v] = p[v];
     return BD_OK;
 }
 
@@ -2142,7 +2260,7 @@ extern "C" int bd_bfs_count_traversed(bd_ctx *c, int64_t *out) {
     hipError_t e = hipMemsetAsync(d, 0, bytes, c->stream);
     if (e == hipSuccess) {
         hipLaunchKernelGGL(k_count_traversed, dim3(grid_for(c, c->nv, 256)), dim3(256), 0,
-                           c->stream, c->d_parent, c->d_off, c->nv, d);
+                           c->stream, c->d_parent, c->d_soff, c->nv, d);
         e = hipGetLastError();
     }
     if (e == hipSuccess) e = hipMemcpyAsync(h.data(), d, bytes, hipMemcpyDeviceToHost, c->stream);
@@ -2163,6 +2281,18 @@ extern "C" int bd_bfs_check(bd_ctx *c, int64_t source, int *ok) {
     if (!c->have_result) return set_err(c, BD_ESTATE, "no search has run");
     if (source < 0 || source >= c->nv) return set_err(c, BD_ERANGE, "source out of range");
     BD_HIP(c, hipSetDevice(c->device));
+    if (c->d_perm) {
+        // the tree as the caller reads it (graph ids), checked on the graph
+        int32_t *buf = nullptr;
+        const int32_t *dp = nullptr, *dd = nullptr;
+        int rc = graph_id_result(c, false, &buf, &dp, &dd);
+        struct guard {
+            int32_t *p;
+            ~guard() { dev_free(p); }
+        } g{buf};
+        if (rc) return rc;
+        return check_tree(c, dp, buf + c->nv, false, source, ok);
+    }
     if (!c->depth_valid) {
         // a malformed tree is a FAIL, not an error
         const char *why = nullptr;

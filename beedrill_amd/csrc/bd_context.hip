@@ -86,6 +86,7 @@ static void free_search(bd_ctx *c) {
 
 void free_graph(bd_ctx *c) {
     free_search(c);
+    free_search_layout(c);
     dev_free(c->d_off);
     dev_free(c->d_cols);
     dev_free(c->d_dead);
@@ -139,6 +140,7 @@ int alloc_search(bd_ctx *c) {
         if ((rc = dev_alloc(c, (void **)&t, sizeof(td_task) * (kSeg * c->tseg_cap + c->task_cap),
                             "tasks")))
             return rc;
+    if ((rc = build_search_layout(c))) return rc;
     return build_head_index(c);
 }
 
@@ -166,8 +168,8 @@ extern "C" int bd_create(bd_ctx **out, int device) {
     // launch shape and thresholds: beedrill_amd/autotune.py's best on MI355X
     // (RMAT scale 24, generator seed 1, models/tuning_rmat_s24.json)
     c->tune.block = 128;
-    c->tune.grid_per_cu = 8;
-    c->tune.bu_words = 64;
+    c->tune.grid_per_cu = 10;
+    c->tune.bu_words = 16;
     c->tune.bu_cap = 32;
     c->tune.wave_threshold = 256;
     c->tune.task_threshold = 16;
@@ -180,7 +182,9 @@ extern "C" int bd_create(bd_ctx **out, int device) {
     c->tune.learned_margin_milli = 250;
     c->tune.ms_alpha = 4;
     c->tune.learned_min_skew = 8;
-    c->tune.td_split_edges = 1 << 21;
+    c->tune.td_split_edges = 1 << 16;
+    c->tune.vertex_order = 1;
+    c->tune.bu_batch = 4;
     hipError_t e = hipSetDevice(device);
     if (e == hipSuccess) {
         hipDeviceProp_t prop;
@@ -286,7 +290,7 @@ extern "C" int bd_set_tuning(bd_ctx *c, const bd_tuning *t) {
             return set_err(c, BD_EINVAL, "block must be 128, 256 or 512");
         n.block = t->block;
     }
-    if (t->bu_words > 0) n.bu_words = std::min(t->bu_words, 64);
+    if (t->bu_words > 0) n.bu_words = std::min(t->bu_words, 32);
     if (t->bu_cap > 0) n.bu_cap = t->bu_cap;
     if (t->grid_per_cu > 0) n.grid_per_cu = std::min(t->grid_per_cu, 64);
     if (t->wave_threshold > 0) n.wave_threshold = t->wave_threshold;
@@ -301,9 +305,16 @@ extern "C" int bd_set_tuning(bd_ctx *c, const bd_tuning *t) {
     if (t->ms_alpha > 0) n.ms_alpha = t->ms_alpha;
     if (t->learned_min_skew > 0) n.learned_min_skew = t->learned_min_skew;
     if (t->td_split_edges > 0) n.td_split_edges = t->td_split_edges;
-    bool realloc = (n.task_edges != c->tune.task_edges || n.task_threshold != c->tune.task_threshold);
+    if (t->vertex_order > 4) return set_err(c, BD_EINVAL, "vertex_order must be 1 to 4");
+    if (t->vertex_order > 0) n.vertex_order = t->vertex_order;
+    if (t->bu_batch > 0) n.bu_batch = t->bu_batch >= 4 ? 4 : 1;
+    bool realloc = (n.task_edges != c->tune.task_edges || n.task_threshold != c->tune.task_threshold ||
+                    n.vertex_order != c->tune.vertex_order);
     c->tune = n;
-    if (realloc && c->d_off) return alloc_search(c);
+    if (realloc && c->d_off) {
+        BD_HIP(c, hipSetDevice(c->device));
+        return alloc_search(c);  // rebuilds the search layout when its order changed
+    }
     return BD_OK;
 }
 

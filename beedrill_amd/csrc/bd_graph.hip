@@ -19,6 +19,7 @@
 
 #include <rocprim/rocprim.hpp>
 
+#include "bd_device.h"
 #include "bd_gen_common.h"
 #include "bd_internal.h"
 
@@ -299,6 +300,7 @@ int build_csr_from_el(bd_ctx *c, int64_t nv, int64_t m) {
     const int64_t n2 = 2 * m;
     dev_buf keys, vals, skeys;
     int rc;
+    free_search_layout(c);
     dev_free(c->d_off);
     dev_free(c->d_cols);
     dev_free(c->d_dead);
@@ -334,6 +336,190 @@ int build_csr_from_el(bd_ctx *c, int64_t nv, int64_t m) {
     c->nwords = nwords;
     c->el_m = m;
     return alloc_search(c);
+}
+
+// ------------------------------------------------------------ search layout
+//
+// The BFS kernels read a renumbered copy of the CSR.  In degree order
+// (vertex_order 1) vertices are ranked by degree, largest first, ties broken
+// by graph id, and the degree-0 vertices come last; the 4096 largest get a
+// visited word each (spread_id), the rest follow in rank order.  Order 4 is
+// the plain degree order (no spreading); with vertex_order 3 the vertices
+// with edges come first, otherwise in graph order.  Either way the
+// arrays indexed by vertex (offsets, head index, parents, visited and
+// frontier bits) hold no isolated vertex before search id nv_active, so a
+// bottom-up step reads them densely; in degree order the high-degree
+// vertices, which most vertices find their parent through, also share a few
+// cache lines of every frontier bitmap.  Each adjacency list keeps its CSR
+// order (only the ids are mapped), so every count a step records
+// (frontier sizes and degrees, edges examined before the first hit) is what
+// it is on the graph's own numbering.
+
+namespace {
+
+constexpr int64_t kSpreadHubs = 4096;
+
+// keys[v] = (rank class << s) | v, v < 2^s: sorting their low 32 + s bits
+// gives the search order.
+__global__ void k_order_keys(const int64_t *__restrict__ off, int64_t nv, int order, int s,
+                             uint64_t *__restrict__ keys, unsigned long long *__restrict__ active) {
+    unsigned long long n = 0;
+    for (int64_t v = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; v < nv;
+         v += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t d = off[v + 1] - off[v];
+        uint64_t cls;
+        if (order == 1 || order == 4)
+            cls = 0xffffffffULL - (uint64_t)(d < 0xffffffffLL ? d : 0xffffffffLL);
+        else
+            cls = d == 0 ? 1 : 0;
+        keys[v] = (cls << s) | (uint64_t)v;
+        n += d > 0;
+    }
+    n = wave_sum(n);
+    if ((threadIdx.x & 63) == 0 && n) atomicAdd(active, n);
+}
+
+// Search id of the vertex of rank r (position in the sorted keys).  The
+// `spread` highest ranks get a 64-bit word each (search id 64 r), the next
+// 63 * spread ranks fill the other bits of those words in rank order, and
+// every later rank keeps its position.  Top-down claims are atomic ORs on
+// the visited word: with the hubs packed 64 to a word, every edge into any
+// of them would hit the same few words.
+__device__ __forceinline__ int64_t spread_id(int64_t r, int64_t spread) {
+    if (r < spread) return 64 * r;
+    const int64_t q = r - spread;
+    if (q < 63 * spread) return 64 * (q / 63) + 1 + q % 63;
+    return r;
+}
+
+// perm[i] = graph vertex of search id i, inv its inverse, sdeg[i] its degree.
+__global__ void k_order_perm(const uint64_t *__restrict__ skeys, const int64_t *__restrict__ off,
+                             int64_t nv, int s, int64_t spread, int32_t *__restrict__ perm,
+                             int32_t *__restrict__ inv, int64_t *__restrict__ sdeg) {
+    const uint64_t vmask = (1ULL << s) - 1;
+    for (int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; r < nv;
+         r += (int64_t)gridDim.x * blockDim.x) {
+        const int32_t v = (int32_t)(skeys[r] & vmask);
+        const int64_t i = spread_id(r, spread);
+        perm[i] = v;
+        inv[v] = (int32_t)i;
+        sdeg[i] = off[v + 1] - off[v];
+    }
+}
+
+// The renumbered lists: one wave per 64 search ids, a lane copying its own
+// list when it is shorter than 64, the whole wave otherwise.
+__global__ __launch_bounds__(256) void k_order_cols(
+    const int64_t *__restrict__ off, const int32_t *__restrict__ cols,
+    const int32_t *__restrict__ perm, const int32_t *__restrict__ inv,
+    const int64_t *__restrict__ soff, int64_t nv, int32_t *__restrict__ scols) {
+    const int lane = threadIdx.x & 63;
+    const int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const int64_t nwaves = ((int64_t)gridDim.x * blockDim.x) >> 6;
+    for (int64_t base = wave * 64; base < nv; base += nwaves * 64) {
+        const int64_t i = base + lane;
+        const bool valid = i < nv;
+        int64_t b = 0, e = 0, sb = 0;
+        if (valid) {
+            const int32_t v = perm[i];
+            b = off[v];
+            e = off[v + 1];
+            sb = soff[i];
+        }
+        if (valid && e - b < 64)
+            for (int64_t k = b; k < e; ++k) scols[sb + (k - b)] = inv[cols[k]];
+        uint64_t lm = __ballot(valid && e - b >= 64);
+        while (lm) {
+            const int src = __ffsll((unsigned long long)lm) - 1;
+            lm &= lm - 1;
+            const int64_t kb = lane_bcast(b, src), ke = lane_bcast(e, src),
+                          so = lane_bcast(sb, src);
+            for (int64_t k = kb + lane; k < ke; k += 64) scols[so + (k - kb)] = inv[cols[k]];
+        }
+    }
+}
+
+}  // namespace
+
+void free_search_layout(bd_ctx *c) {
+    if (c->d_perm) {  // renumbered: the layout owns its arrays
+        dev_free(c->d_soff);
+        dev_free(c->d_scols);
+        dev_free(c->d_sdead);
+    }
+    dev_free(c->d_perm);
+    dev_free(c->d_inv);
+    dev_free(c->d_head);
+    c->d_soff = nullptr;
+    c->d_scols = nullptr;
+    c->d_sdead = nullptr;
+    c->d_perm = nullptr;
+    c->d_inv = nullptr;
+    c->d_head = nullptr;
+    c->nv_active = 0;
+    c->s_order = 0;
+}
+
+int build_search_layout(bd_ctx *c) {
+    const int order = c->tune.vertex_order;
+    if (c->s_order == order && c->d_soff) return BD_OK;
+    free_search_layout(c);
+    const int64_t nv = c->nv, nnz = c->nnz;
+    int rc;
+    dev_buf keys, skeys, sdeg, cnt, tmp;
+    if ((rc = dev_alloc(c, &cnt.p, sizeof(unsigned long long), "active count"))) return rc;
+    if ((rc = dev_alloc(c, &keys.p, sizeof(uint64_t) * nv, "order keys"))) return rc;
+    BD_HIP(c, hipMemsetAsync(cnt.p, 0, sizeof(unsigned long long), c->stream));
+    const int sh = bits_for(nv);
+    hipLaunchKernelGGL(k_order_keys, dim3(grid1d(c, nv)), dim3(kBlock), 0, c->stream, c->d_off,
+                       nv, order == 2 ? 3 : order, sh, keys.as<uint64_t>(),
+                       cnt.as<unsigned long long>());
+    BD_HIP(c, hipGetLastError());
+    unsigned long long active = 0;
+    BD_HIP(c, hipMemcpyAsync(&active, cnt.p, sizeof(active), hipMemcpyDeviceToHost, c->stream));
+    BD_HIP(c, hipStreamSynchronize(c->stream));
+    c->nv_active = (int64_t)active;
+    if (order == 2) {  // the graph's own order
+        c->d_soff = c->d_off;
+        c->d_scols = c->d_cols;
+        c->d_sdead = c->d_dead;
+        c->s_order = order;
+        return BD_OK;
+    }
+    if ((rc = dev_alloc(c, &skeys.p, sizeof(uint64_t) * nv, "sorted order keys"))) return rc;
+    if ((rc = sort_keys_u64(c, keys.as<uint64_t>(), skeys.as<uint64_t>(), nv, 32 + sh))) return rc;
+    if ((rc = dev_alloc(c, &sdeg.p, sizeof(int64_t) * nv, "search degrees"))) return rc;
+    // from here the layout owns what it allocates (freed by free_search_layout)
+    if ((rc = dev_alloc(c, (void **)&c->d_perm, sizeof(int32_t) * nv, "search order"))) return rc;
+    if ((rc = dev_alloc(c, (void **)&c->d_inv, sizeof(int32_t) * nv, "search ids"))) return rc;
+    if ((rc = dev_alloc(c, (void **)&c->d_soff, sizeof(int64_t) * (nv + 1), "search offsets")))
+        return rc;
+    if ((rc = dev_alloc(c, (void **)&c->d_scols, sizeof(int32_t) * nnz, "search cols")))
+        return rc;
+    if ((rc = dev_alloc(c, (void **)&c->d_sdead, sizeof(uint64_t) * c->nwords,
+                        "search dead bitmap")))
+        return rc;
+    // degree order spreads the 4096 largest degrees one to a visited word
+    const int64_t spread = order == 1 ? std::min<int64_t>(kSpreadHubs, c->nv_active / 64) : 0;
+    hipLaunchKernelGGL(k_order_perm, dim3(grid1d(c, nv)), dim3(kBlock), 0, c->stream,
+                       skeys.as<uint64_t>(), c->d_off, nv, sh, spread, c->d_perm, c->d_inv,
+                       sdeg.as<int64_t>());
+    BD_HIP(c, hipGetLastError());
+    BD_HIP(c, hipMemsetAsync(c->d_soff, 0, sizeof(int64_t), c->stream));
+    size_t tb = 0;
+    BD_HIP(c, rocprim::inclusive_scan(nullptr, tb, sdeg.as<int64_t>(), c->d_soff + 1, (size_t)nv,
+                                      rocprim::plus<int64_t>(), c->stream));
+    if ((rc = dev_alloc(c, &tmp.p, tb, "scan temp"))) return rc;
+    BD_HIP(c, rocprim::inclusive_scan(tmp.p, tb, sdeg.as<int64_t>(), c->d_soff + 1, (size_t)nv,
+                                      rocprim::plus<int64_t>(), c->stream));
+    hipLaunchKernelGGL(k_order_cols, dim3(grid1d(c, nv)), dim3(kBlock), 0, c->stream, c->d_off,
+                       c->d_cols, c->d_perm, c->d_inv, c->d_soff, nv, c->d_scols);
+    hipLaunchKernelGGL(k_dead_bitmap, dim3(grid1d(c, c->nwords * 64)), dim3(kBlock), 0, c->stream,
+                       c->d_soff, nv, c->nwords, c->d_sdead);
+    BD_HIP(c, hipGetLastError());
+    BD_HIP(c, hipStreamSynchronize(c->stream));
+    c->s_order = order;
+    return BD_OK;
 }
 
 }  // namespace bd

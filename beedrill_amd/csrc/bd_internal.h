@@ -108,10 +108,24 @@ struct bd_ctx {
     int64_t *d_off = nullptr;
     int32_t *d_cols = nullptr;
     uint64_t *d_dead = nullptr;   // 1 = vertex of degree 0, or padding bit
-    uint64_t *d_head = nullptr;   // bottom-up head index (built with the graph)
+    uint64_t *d_head = nullptr;   // bottom-up head index, in search ids (built with the layout)
     int64_t max_deg = 0;          // largest degree (found with the head index)
     uint64_t *d_el = nullptr;     // el64 order, packed (src << 32) | dst
     int64_t el_m = 0;
+
+    // search layout: the CSR the BFS kernels read.  Vertices are renumbered
+    // (tune.vertex_order): search id i is graph vertex perm[i], graph vertex
+    // v is search id inv[v]; every adjacency list keeps its CSR order, ids
+    // mapped.  Searches run on search ids; sources are mapped on the device
+    // and results mapped back on readout.  With the graph's own order the
+    // pointers alias d_off / d_cols / d_dead and perm / inv are null.
+    int64_t *d_soff = nullptr;
+    int32_t *d_scols = nullptr;
+    uint64_t *d_sdead = nullptr;
+    int32_t *d_perm = nullptr;
+    int32_t *d_inv = nullptr;
+    int64_t nv_active = 0;        // vertices with edges
+    int s_order = 0;              // vertex_order the layout was built with (0: none yet)
 
     // search state
     int32_t *d_depth = nullptr;
@@ -190,6 +204,10 @@ void free_graph(bd_ctx *c);
 int alloc_search(bd_ctx *c);
 // Head index of the bottom-up step for the resident graph.
 int build_head_index(bd_ctx *c);
+// Search layout of the resident graph in the tuned vertex order (no-op when
+// it is built in that order); frees it first when the order changed.
+int build_search_layout(bd_ctx *c);
+void free_search_layout(bd_ctx *c);
 // Rebuild the step kernels' operand table of the direction network.
 int refresh_mlp_ops(bd_ctx *c);
 // Build device CSR from a device edge list (packed src<<32|dst, el order).

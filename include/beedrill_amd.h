@@ -259,7 +259,7 @@ typedef struct {
                                 into task slices                          */
     int32_t task_edges;      /* edges per task slice                       */
     int32_t bu_words;        /* bottom-up: visited words per wave chunk
-                                (1..64)                                   */
+                                (1..32)                                   */
     int32_t bu_cap;          /* bottom-up: neighbours a lane scans before
                                 its wave finishes the list together       */
     int32_t level_driver;    /* 1 = direction chosen on device and steps
@@ -301,7 +301,18 @@ typedef struct {
                                 each edge's source as the candidate parent
                                 of its unvisited end, the next claims one
                                 candidate per vertex; 2^31-1 = never       */
-    int32_t reserved[1];
+    int32_t vertex_order;    /* numbering of the CSR copy the searches read
+                                (results are always in graph ids): 1 =
+                                degree order, largest first, degree-0
+                                vertices last, the 4096 largest one per
+                                64-bit visited word; 2 = the graph's own
+                                ids (no copy); 3 = vertices with edges
+                                first, in graph order; 4 = degree order
+                                without the spreading                     */
+    int32_t bu_batch;        /* bottom-up: 1 = one round of 64 vertices per
+                                wave at a time (the next round's head
+                                entries prefetched); 4 = four rounds' head
+                                entries loaded together                   */
 } bd_tuning;
 int bd_set_tuning(bd_ctx *ctx, const bd_tuning *t);
 int bd_get_tuning(bd_ctx *ctx, bd_tuning *t);
