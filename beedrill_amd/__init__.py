@@ -132,6 +132,8 @@ def lib():
                               ctypes.c_int),
         "bd_bfs_multi_graph": ([vp, ctypes.c_int, _i64p, _i64p, vp,
                                 ctypes.POINTER(ctypes.c_double)], ctypes.c_int),
+        "bd_bfs_multi_graph_static": ([vp, ctypes.c_int, vp, vp,
+                                       ctypes.POINTER(ctypes.c_double)], ctypes.c_int),
         "bd_cc_run": ([vp, ctypes.POINTER(CCStats)], ctypes.c_int),
         "bd_cc_fetch": ([vp, _i64p], ctypes.c_int),
         "bd_cc_check": ([vp, ctypes.POINTER(ctypes.c_int)], ctypes.c_int),
@@ -407,6 +409,17 @@ class Context:
                                             dp.ctypes.data if dp is not None else None,
                                             ctypes.byref(ms)))
         return dp, ms.value
+
+    def multi_graph_static(self, vbase, roots):
+        """multi_graph through the allocation-free entry point (at most 4096
+        graphs; no depths output: read them with fetch()).  Returns the
+        search's device ms."""
+        vbase = np.ascontiguousarray(vbase, np.int64)
+        roots = np.ascontiguousarray(roots, np.int64)
+        ms = ctypes.c_double(0)
+        self._ok(self._L.bd_bfs_multi_graph_static(self._h, len(roots), vbase.ctypes.data,
+                                                   roots.ctypes.data, ctypes.byref(ms)))
+        return ms.value
 
     def upload_union(self, graphs):
         """Upload the disjoint union of CSR graphs [(offsets, cols, m), ...];

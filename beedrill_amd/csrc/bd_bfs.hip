@@ -2490,6 +2490,86 @@ extern "C" int bd_bfs_multi_graph(bd_ctx *c, int ngraphs, const int64_t *vbase,
     return BD_OK;
 }
 
+// ---------------------------------------- many graphs at once, no allocation
+
+// Staging of bd_bfs_multi_graph_static, in a file-scope device array instead
+// of memory allocated per call: [0, n) the search's sources, [n, 2n] the
+// graph bounds vbase, [2n + 1] the flag k_check_parts raises.  Every GPU the
+// library runs on holds its own copy (kStaticGraphs * 16 + 16 bytes, 64 KB)
+// until the process ends; host code reaches it only through the address
+// hipGetSymbolAddress gives for that device (bd_ctx::d_static).
+constexpr int kStaticGraphs = 4096;
+static __device__ __align__(256) int64_t scratch[2 * kStaticGraphs + 2];
+
+namespace bd {
+namespace {
+
+// sources = vbase[g] + roots[g] (roots were copied to [0, n)); clears the flag.
+__global__ void k_static_sources(int64_t *st, int n) {
+    for (int g = blockIdx.x * blockDim.x + threadIdx.x; g < n; g += gridDim.x * blockDim.x)
+        st[g] += st[n + g];
+    if (blockIdx.x == 0 && threadIdx.x == 0) st[2 * n + 1] = 0;
+}
+
+}  // namespace
+
+// The device's address of `scratch`, with the context's device current.
+int static_scratch_init(bd_ctx *c) {
+    void *p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        return hip_err(c, e, "hipGetSymbolAddress(scratch)");
+    }
+    c->d_static = (int64_t *)p;
+    return BD_OK;
+}
+
+}  // namespace bd
+
+// bd_bfs_multi_graph without its per-call device allocation (the staging of
+// sources, bounds and the check flag lives in `scratch`, at most
+// kStaticGraphs graphs) and without the depths output.  No counterpart in the
+// reference; for callers that cannot allocate.  Calls overlapping on one GPU
+// share the array and race.
+extern "C" int bd_bfs_multi_graph_static(bd_ctx *c, int ngraphs, const int64_t *vbase,
+                                         const int64_t *roots, double *ms) {
+    if (!c || ngraphs < 1 || !vbase || !roots) return BD_EINVAL;
+    if (ngraphs > kStaticGraphs)
+        return set_err(c, BD_ERANGE,
+                       "bd_bfs_multi_graph_static: at most 4096 graphs (its static staging)");
+    if (!c->d_off) return set_err(c, BD_ESTATE, "no graph");
+    if (!c->d_static) return set_err(c, BD_ESTATE, "no static staging on this device");
+    if (vbase[0] != 0 || vbase[ngraphs] != c->nv)
+        return set_err(c, BD_EINVAL, "vbase must run from 0 to the number of vertices");
+    for (int g = 0; g < ngraphs; ++g) {
+        if (vbase[g + 1] <= vbase[g]) return set_err(c, BD_EINVAL, "empty or unordered graph");
+        if (roots[g] < 0 || roots[g] >= vbase[g + 1] - vbase[g])
+            return set_err(c, BD_ERANGE, "root out of range of its graph");
+    }
+    BD_HIP(c, hipSetDevice(c->device));
+    int64_t *st = c->d_static;
+    BD_HIP(c, hipMemcpyAsync(st, roots, sizeof(int64_t) * ngraphs, hipMemcpyHostToDevice,
+                             c->stream));
+    BD_HIP(c, hipMemcpyAsync(st + ngraphs, vbase, sizeof(int64_t) * (ngraphs + 1),
+                             hipMemcpyHostToDevice, c->stream));
+    hipLaunchKernelGGL(k_static_sources, dim3((ngraphs + 255) / 256), dim3(256), 0, c->stream, st,
+                       ngraphs);
+    int *d_bad = (int *)(st + 2 * ngraphs + 1);
+    hipLaunchKernelGGL(k_check_parts, dim3(grid_for(c, c->nv, 256)), dim3(256), 0, c->stream,
+                       c->d_off, c->d_cols, c->nv, st + ngraphs, ngraphs, d_bad);
+    BD_HIP(c, hipGetLastError());
+    int bad = 0;
+    BD_HIP(c, hipMemcpyAsync(&bad, d_bad, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    BD_HIP(c, hipStreamSynchronize(c->stream));
+    if (bad) return set_err(c, BD_EINVAL, "an edge joins two graphs of the batch");
+    bd_run_stats rs;
+    const int rc = run_search(c, roots[0], 15, 18, BD_POLICY_BEAMER, &rs, st, ngraphs);
+    if (rc) return rc;
+    if (ms) *ms = rs.ms;
+    return BD_OK;
+}
+
 // Kernel time of every step launch of the last search run with time_steps 3
 // (device-driven, one event pair per launch) or 4 (one pair per chunk of
 // launches, each launch given the chunk's mean), including the launches

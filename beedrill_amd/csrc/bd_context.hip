@@ -214,6 +214,10 @@ extern "C" int bd_create(bd_ctx **out, int device) {
         bd_destroy(c);
         return BD_EHIP;
     }
+    if (static_scratch_init(c) != BD_OK) {
+        bd_destroy(c);
+        return BD_EHIP;
+    }
     *out = c;
     return BD_OK;
 }

@@ -184,6 +184,8 @@ struct bd_ctx {
     int32_t *d_ms_depth = nullptr;
     int64_t ms_depth_rows = 0;
 
+    int64_t *d_static = nullptr;  // this device's address of bd_bfs.hip's `scratch`
+
     bd_tuning tune{};
     hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
     std::vector<hipEvent_t> ev_pool;     // per-launch timing events (device-scope release)
@@ -208,6 +210,9 @@ int build_head_index(bd_ctx *c);
 // it is built in that order); frees it first when the order changed.
 int build_search_layout(bd_ctx *c);
 void free_search_layout(bd_ctx *c);
+// Look up this device's address of the allocation-free entry points'
+// staging array (with the context's device current).
+int static_scratch_init(bd_ctx *c);
 // Rebuild the step kernels' operand table of the direction network.
 int refresh_mlp_ops(bd_ctx *c);
 // Build device CSR from a device edge list (packed src<<32|dst, el order).

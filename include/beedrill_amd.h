@@ -351,6 +351,18 @@ int bd_bfs_multi_root(bd_ctx *ctx, const int64_t *roots, int n, int32_t *depths,
 int bd_bfs_multi_graph(bd_ctx *ctx, int ngraphs, const int64_t *vbase, const int64_t *roots,
                        int32_t *depths, double *ms);
 
+/* bd_bfs_multi_graph without allocating: no counterpart in the reference;
+ * for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  Its
+ * staging (the sources, the graph bounds, the cross-edge flag) lives in a
+ * static device array of the library, one per GPU, sized for at most 4096
+ * graphs; more are rejected with BD_ERANGE.  Same search and results as
+ * bd_bfs_multi_graph, read afterwards with bd_bfs_fetch / bd_bfs_levels
+ * (there is no depths argument).  Calls that overlap on one GPU share the
+ * array and race. */
+int bd_bfs_multi_graph_static(bd_ctx *ctx, int ngraphs, const int64_t *vbase,
+                              const int64_t *roots, double *ms);
+
 /* --------------------------------------------------- learned direction MLP */
 
 /* 8 -> h1 -> h2 -> 2 ReLU MLP predicting log step cost of (TD, BU).
