@@ -180,7 +180,7 @@ struct step_smem {
                   // unvisited vertices, head misses, first hit per pending list
             uint64_t b_found[BLOCK / 64][64];
             uint16_t b_list[BLOCK / 64][kBuChunkWords * 64];
-            int32_t b_pend[BLOCK / 64][128];
+            int32_t b_pend[BLOCK / 64][64 * (kBuBatch + 1)];
             unsigned long long b_hit[BLOCK / 64][64];
         };
     };
@@ -208,7 +208,7 @@ __device__ __forceinline__ int64_t seg_base(int64_t seg_cap) {
 // one reservation for the whole wave, then the slices are written 64 at a
 // time, lane j taking slice j of the concatenation (its owner found by a
 // binary search over the lanes' exclusive prefix).
-__device__ void emit_tasks(const step_args &a, bool huge, int32_t u, int64_t b, int64_t e) {
+__device__ __forceinline__ void emit_tasks(const step_args &a, bool huge, int32_t u, int64_t b, int64_t e) {
     const int lane = lane_id();
     const long long nt = huge ? (e - b + a.task_edges - 1) / a.task_edges : 0;
     const long long incl = wave_incl_scan(nt);
@@ -306,7 +306,7 @@ __device__ __forceinline__ void emit_next(const step_args &a, step_smem<BLOCK> &
 // Write the staged queue entries to this workgroup's segment (whole
 // workgroup).
 template <int BLOCK>
-__device__ void flush_queue(const step_args &a, step_smem<BLOCK> &s) {
+__device__ __forceinline__ void flush_queue(const step_args &a, step_smem<BLOCK> &s) {
     __syncthreads();
     const int n = s.qn < kQCap ? s.qn : kQCap;
     if (n > 0) {
@@ -466,7 +466,7 @@ __device__ __forceinline__ void td_visit_n(const step_args &a, step_smem<BLOCK> 
 // degree >= wave_t get a wave each; the edges of the others are concatenated
 // by a block-wide scan and every thread takes one.  Whole workgroup.
 template <int BLOCK>
-__device__ void td_tile(const step_args &a, step_smem<BLOCK> &s, bool tdw, int32_t v,
+__device__ __forceinline__ void td_tile(const step_args &a, step_smem<BLOCK> &s, bool tdw, int32_t v,
                         unsigned long long &cnt, unsigned long long &dsum,
                         unsigned long long &dmax) {
     const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
@@ -546,7 +546,7 @@ __device__ void td_tile(const step_args &a, step_smem<BLOCK> &s, bool tdw, int32
 // Task slices of the frontier's high-degree vertices, one wave each, in
 // workgroup-synchronous rounds.
 template <int BLOCK>
-__device__ void td_tasks(const step_args &a, step_smem<BLOCK> &s, bool tdw, unsigned long long &cnt,
+__device__ __forceinline__ void td_tasks(const step_args &a, step_smem<BLOCK> &s, bool tdw, unsigned long long &cnt,
                          unsigned long long &dsum, unsigned long long &dmax) {
     const int lane = lane_id(), wid = threadIdx.x >> 6;
     const int64_t ntasks = s.st.tpref[kSeg + 1];
@@ -586,7 +586,7 @@ __device__ void td_tasks(const step_args &a, step_smem<BLOCK> &s, bool tdw, unsi
 }
 
 template <int BLOCK>
-__device__ void td_body(const step_args &a, step_smem<BLOCK> &s, bool tdw, unsigned long long &cnt,
+__device__ __forceinline__ void td_body(const step_args &a, step_smem<BLOCK> &s, bool tdw, unsigned long long &cnt,
                         unsigned long long &dsum, unsigned long long &dmax) {
     const int64_t nitems = s.st.qpref[kSeg + 1];
     for (int64_t tile = blockIdx.x; tile * BLOCK < nitems; tile += gridDim.x) {
@@ -606,7 +606,7 @@ __device__ void td_body(const step_args &a, step_smem<BLOCK> &s, bool tdw, unsig
 // ranges of BLOCK frontier words, ranks their set bits with a block-wide
 // scan, and feeds them to td_tile BLOCK vertices at a time.
 template <int BLOCK>
-__device__ void td_bitmap_body(const step_args &a, step_smem<BLOCK> &s, bool tdw, unsigned long long &cnt,
+__device__ __forceinline__ void td_bitmap_body(const step_args &a, step_smem<BLOCK> &s, bool tdw, unsigned long long &cnt,
                                unsigned long long &dsum, unsigned long long &dmax) {
     const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
     const int64_t nranges = (a.live_nwords + BLOCK - 1) / BLOCK;
@@ -656,7 +656,7 @@ __device__ void td_bitmap_body(const step_args &a, step_smem<BLOCK> &s, bool tdw
 // the whole wave.  Found vertices get their parent and their bit in the
 // wave's found words (chunk word (v >> 6) - w0).
 template <int BLOCK>
-__device__ void bu_pending(const step_args &a, step_smem<BLOCK> &s, const int32_t *pend, int n,
+__device__ __forceinline__ void bu_pending(const step_args &a, step_smem<BLOCK> &s, const int32_t *pend, int n,
                            int64_t w0, unsigned long long &cnt, unsigned long long &dsum,
                            unsigned long long &dmax, unsigned long long &exam) {
     const int lane = lane_id(), wid = threadIdx.x >> 6;
@@ -732,6 +732,7 @@ __device__ void bu_pending(const step_args &a, step_smem<BLOCK> &s, const int32_
         atomicOr((unsigned long long *)&s.b_found[wid][(v >> 6) - w0],
                  (unsigned long long)(1ULL << (v & 63)));
     }
+
     wave_lds_sync();  // hit[] is rewritten by the next batch
 }
 
@@ -765,7 +766,7 @@ __device__ __forceinline__ int bu_list_chunk(const step_args &a, uint16_t *list,
 // vertices as the bottom-up step does; each consumed candidate is reset, so
 // the array is all -1 again when the step ends.
 template <int BLOCK>
-__device__ void td_claim_body(const step_args &a, step_smem<BLOCK> &s, unsigned long long &cnt,
+__device__ __forceinline__ void td_claim_body(const step_args &a, step_smem<BLOCK> &s, unsigned long long &cnt,
                               unsigned long long &dsum, unsigned long long &dmax) {
     const int lane = lane_id(), wid = threadIdx.x >> 6;
     const int CW = a.bu_words;
@@ -809,6 +810,7 @@ __device__ void td_claim_body(const step_args &a, step_smem<BLOCK> &s, unsigned 
                 atomicOr((unsigned long long *)&s.b_found[wid][idx[k] >> 6],
                          (unsigned long long)(1ULL << (idx[k] & 63)));
             }
+
         }
         wave_lds_sync();
         const int64_t wi = w0 + lane;
@@ -829,7 +831,7 @@ __device__ void td_claim_body(const step_args &a, step_smem<BLOCK> &s, unsigned 
 // rare long scans do not hold up every round.  Found bits are gathered in
 // LDS and written by the word owner: no global atomics.
 template <int BLOCK>
-__device__ void bu_body(const step_args &a, step_smem<BLOCK> &s, unsigned long long &cnt,
+__device__ __forceinline__ void bu_body(const step_args &a, step_smem<BLOCK> &s, unsigned long long &cnt,
                         unsigned long long &dsum, unsigned long long &dmax,
                         unsigned long long &exam) {
     const int lane = lane_id(), wid = threadIdx.x >> 6;
@@ -867,20 +869,30 @@ __device__ void bu_body(const step_args &a, step_smem<BLOCK> &s, unsigned long l
                     dmax = d > dmax ? d : dmax;
                     atomicOr((unsigned long long *)&s.b_found[wid][idx >> 6],
                              (unsigned long long)(1ULL << (idx & 63)));
+
                 }
             }
             const uint64_t mm = __ballot(miss);
             if (miss) pend[np + __popcll(mm & below)] = (int32_t)v;
             np += __popcll(mm);
-            if (np >= 64) {
+        };
+        // Scan the queued lists 64 at a time while 64 or more are queued
+        // (the queue holds up to 64 * (kBuBatch + 1): fewer than 64 left
+        // over, plus one batch of rounds).
+        auto drain = [&]() {
+            while (np >= 64) {
                 wave_lds_sync();
                 bu_pending(a, s, pend, 64, w0, cnt, dsum, dmax, exam);
                 const int rest = np - 64;
-                const int32_t t = lane < rest ? pend[64 + lane] : 0;
-                wave_lds_sync();
-                if (lane < rest) pend[lane] = t;
+                // move the rest down, 64 at a time in increasing order: each
+                // block is read before the block written over it
+                for (int i0 = 0; i0 < rest; i0 += 64) {
+                    const int32_t t = i0 + lane < rest ? pend[64 + i0 + lane] : 0;
+                    wave_lds_sync();
+                    if (i0 + lane < rest) pend[i0 + lane] = t;
+                    wave_lds_sync();
+                }
                 np = rest;
-                wave_lds_sync();
             }
         };
         auto head_of = [&](bool valid, int idx) -> uint64_t {
@@ -910,8 +922,8 @@ __device__ void bu_body(const step_args &a, step_smem<BLOCK> &s, unsigned long l
 #pragma unroll
                 for (int k = 0; k < R; ++k) fw[k] = front_of(valid[k], hd[k]);
 #pragma unroll
-                for (int k = 0; k < R; ++k)
-                    if (r0 + 64 * k < total) visit(valid[k], idx[k], hd[k], fw[k]);
+                for (int k = 0; k < R; ++k) visit(valid[k], idx[k], hd[k], fw[k]);
+                drain();
             }
         };
         if (a.bu_batch > 1) {
@@ -929,6 +941,7 @@ __device__ void bu_body(const step_args &a, step_smem<BLOCK> &s, unsigned long l
                 const int idxn = validn ? list[r + 64 + lane] : 0;
                 const uint64_t hdn = head_of(validn, idxn);
                 visit(valid, idx, hd, fw);
+                drain();
                 valid = validn;
                 idx = idxn;
                 hd = hdn;
@@ -965,7 +978,7 @@ __device__ __forceinline__ bool gt_quotient(int64_t s, int64_t e, int64_t alpha)
     return p > e;
 }
 
-__device__ void beamer_next(const step_args &a, ctl_block *c, int32_t last_dir, int64_t nnew,
+__device__ __forceinline__ void beamer_next(const step_args &a, ctl_block *c, int32_t last_dir, int64_t nnew,
                             int64_t nnew_deg) {
     if (last_dir == BD_DIR_TD) {
         c->scout = nnew_deg;
@@ -992,7 +1005,7 @@ __device__ void beamer_next(const step_args &a, ctl_block *c, int32_t last_dir, 
 // workgroup barrier): lane 0 updates the state; the learned policy evaluates
 // its network on the wave.
 template <int BLOCK>
-__device__ void choose_dir(const step_args &a, step_smem<BLOCK> &s, ctl_block *c, int32_t last_dir,
+__device__ __forceinline__ void choose_dir(const step_args &a, step_smem<BLOCK> &s, ctl_block *c, int32_t last_dir,
                            int64_t nnew, int64_t nnew_deg, const mlp_wave_regs &w) {
     if (lane_id() == 0) {
         switch (a.policy) {
@@ -1018,18 +1031,24 @@ __device__ void choose_dir(const step_args &a, step_smem<BLOCK> &s, ctl_block *c
         case BD_POLICY_LEARNED:
             // Beamer's choice and state, which the network may override
             beamer_next(a, c, last_dir, nnew, nnew_deg);
-            if (a.has_mlp) {
-                mlp_features_dev(c->frontier, c->frontier_deg, c->max_frontier_deg, c->unvisited,
-                                 c->unvisited_deg, c->level, c->nv, a.m, last_dir, s.x);
-            }
             break;
         default:
             beamer_next(a, c, last_dir, nnew, nnew_deg);
         }
     }
     // the network runs only for LEARNED with a network loaded (kernel
-    // arguments, the same in every lane: no LDS flag to wait for)
-    if (a.policy == BD_POLICY_LEARNED && a.has_mlp) {
+    // arguments, the same in every lane), and only where Beamer's rules
+    // switch direction (hybrid_bfs.cc:167 turning bottom-up, :178-179
+    // leaving it): it decides whether to switch now or to stay.  Elsewhere
+    // Beamer's choice stands without a network evaluation, whose latency
+    // every step would otherwise pay.
+    if (a.policy == BD_POLICY_LEARNED && a.has_mlp) wave_lds_sync();
+    if (a.policy == BD_POLICY_LEARNED && a.has_mlp && last_dir >= 0 && c->dir != last_dir) {
+        const int l = lane_id();
+        if (l < kMlpIn)
+            s.x[l] = mlp_feature_k(l, c->frontier, c->frontier_deg, c->max_frontier_deg,
+                                   c->unvisited, c->unvisited_deg, c->level, c->nv, a.m,
+                                   last_dir);
         wave_lds_sync();
         float y[2];
         mlp_eval(w, s.x, y);
@@ -1070,7 +1089,7 @@ __device__ void choose_dir(const step_args &a, step_smem<BLOCK> &s, ctl_block *c
 // Prefix sums of the segment lengths of a list into pref[0..kSeg+1] on one
 // wave: lane s holds the length of segment s (capped at seg_cap), `ovf` the
 // overflow segment's (capped at ovf_cap; read by every lane).
-__device__ void seg_prefix(int64_t n, int64_t ovf, int64_t seg_cap, int64_t ovf_cap,
+__device__ __forceinline__ void seg_prefix(int64_t n, int64_t ovf, int64_t seg_cap, int64_t ovf_cap,
                            int64_t *pref) {
     const int l = lane_id();
     n = n < seg_cap ? n : seg_cap;
@@ -1093,7 +1112,7 @@ static_assert(kCtlWords <= 3 * 64, "ctl_block is copied as three words per lane"
 // before any is used, so the rebuild costs one memory round trip.  Returns
 // true, having changed nothing, when the search ended before this level.
 template <int BLOCK>
-__device__ bool rebuild_state(const step_args &a, step_smem<BLOCK> &s, bool publish) {
+__device__ __forceinline__ bool rebuild_state(const step_args &a, step_smem<BLOCK> &s, bool publish) {
     const int L = a.level;
     const int l = lane_id();
     seg_counters *g = a.seg;
@@ -1205,7 +1224,7 @@ __device__ bool rebuild_state(const step_args &a, step_smem<BLOCK> &s, bool publ
 // counters the next step fills: acc[(L+1)%3] and the segment lengths
 // [(L+2)%3], neither of which anyone reads or writes during step L.
 template <int BLOCK>
-__device__ void publish_state(const step_args &a, step_smem<BLOCK> &s) {
+__device__ __forceinline__ void publish_state(const step_args &a, step_smem<BLOCK> &s) {
     const int L = a.level;
     const int l = lane_id();
     seg_counters *g = a.seg;
@@ -1236,7 +1255,7 @@ __device__ void publish_state(const step_args &a, step_smem<BLOCK> &s) {
 // Sum the block's counters and add them to its slot (spread over kSlots
 // cache lines so no line sees more than a few dozen atomics per step).
 template <int BLOCK>
-__device__ void block_flush_counters(const step_args &a, step_smem<BLOCK> &s, unsigned long long cnt,
+__device__ __forceinline__ void block_flush_counters(const step_args &a, step_smem<BLOCK> &s, unsigned long long cnt,
                                      unsigned long long dsum, unsigned long long exam,
                                      unsigned long long dmax) {
     const int lane = lane_id(), wid = threadIdx.x >> 6;

@@ -42,6 +42,27 @@ __host__ __device__ inline void mlp_features_dev(int64_t frontier, int64_t front
     x[7] = prev_dir == 1 ? 1.0f : 0.0f;
 }
 
+// Feature `k` (0..7) of mlp_features_dev alone, so that lanes 0-7 of a wave
+// compute the eight features side by side; the same arithmetic, in the same
+// order, as mlp_features_dev.
+__device__ inline float mlp_feature_k(int k, int64_t frontier, int64_t frontier_deg,
+                                      int64_t max_frontier_deg, int64_t unvisited,
+                                      int64_t unvisited_deg, int64_t level, int64_t nv, int64_t m,
+                                      int64_t prev_dir) {
+    const float lv = lg1p((float)nv);
+    const float le = lg1p(2.0f * (float)m);
+    switch (k) {
+    case 0: return lg1p((float)frontier) / lv;
+    case 1: return lg1p((float)frontier_deg) / le;
+    case 2: return lg1p((float)unvisited) / lv;
+    case 3: return lg1p((float)unvisited_deg) / le;
+    case 4: return (float)(level < 64 ? level : 64) / 64.0f;
+    case 5: return lg1p((float)max_frontier_deg) / lv;
+    case 6: return lg1p(2.0f * (float)m / (float)(nv > 0 ? nv : 1)) / 8.0f;
+    default: return prev_dir == 1 ? 1.0f : 0.0f;
+    }
+}
+
 // Orders LDS accesses among the lanes of one wave.  The network code below
 // runs on a single wave, so it never needs a workgroup barrier and may be
 // called by one wave of a larger workgroup.

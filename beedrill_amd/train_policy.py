@@ -106,14 +106,15 @@ MARGINS_MILLI = (1, 125, 250, 500, 1000, 2000)
 
 def choose_margin(ctx, params, roots_per_graph=8, log=print):
     """Pick the override margin on held-out graphs (generator seed 303,
-    scales 20-22): for each margin, the harmonic-mean GTEPS of the learned
-    policy over Beamer's on the same roots, combined as a geometric mean over
-    the graphs; the best margin wins (Beamer itself if no margin gains)."""
+    scales 21-23, larger than any training graph): for each margin, the
+    harmonic-mean GTEPS of the learned policy over Beamer's on the same
+    roots, combined as a geometric mean over the graphs; the best margin
+    wins (Beamer itself if no margin gains)."""
     ctx.mlp_set(params, 32, 32)
     tune = ctx.get_tuning()
     ratios = {m: [] for m in MARGINS_MILLI}
     try:
-        for scale in (20, 21, 22):
+        for scale in (21, 22, 23):
             for a in (0.50, 0.57):
                 bc = (0.95 - a) / 2
                 ctx.generate_rmat(scale, 16, a, bc, bc, 0.05, seed=303)

@@ -18,8 +18,10 @@ The graph is generated on the GPU with the build's seeded RMAT generator and
 is resident in HBM before timing starts.
 
 Prints one JSON line.  Beside the metric it carries the step kernel's
-roofline, the CPU baselines (one core and all cores), the 64-root batch and
-connected components (components::run) on the same resident graph.
+roofline, the CPU baselines (one core and all cores), what a perfect choice
+of one step's direction could gain over Beamer (direction_bound), the 64-root
+batch and connected components (components::run) on the same resident
+graph.
 """
 import argparse
 import json
@@ -123,6 +125,41 @@ def check_roots(ctx, roots, policies):
                 failed.append([pol, int(r)])
     return {"roots": len(roots), "policies": list(policies), "failed": failed,
             "pass": not failed}
+
+
+def direction_bound(ctx, roots, policy, budget_s):
+    """How much any direction policy could gain on these roots: per root the
+    device time of the search with Beamer's choices and with `policy` (best
+    of 2 each), and of every single-step flip of Beamer's choices (one run
+    each), over as many roots as fit in budget_s.  best = the fastest of
+    them per root.  Flipping an early bottom-up step to top-down, or a top-
+    down step to bottom-up, can cost tens of milliseconds: those are the
+    slow searches of this leg."""
+    t0 = time.perf_counter()
+    rows = []
+    for r in roots:
+        r = int(r)
+        st = ctx.bfs(r, 15, 18, "beamer")
+        b = min(st["ms"], ctx.bfs(r, 15, 18, "beamer")["ms"])
+        dirs = [x["dir"] for x in ctx.levels()]
+        lt = min(ctx.bfs(r, 15, 18, policy)["ms"] for _ in range(2))
+        flips = []
+        for L in range(len(dirs)):
+            sc = list(dirs)
+            sc[L] = 1 - sc[L]
+            ctx.set_script(np.array(sc, np.int32))
+            flips.append(ctx.bfs(r, 15, 18, "script")["ms"])
+        rows.append((st["traversed_edges"], b, lt, min([b, lt] + flips)))
+        if time.perf_counter() - t0 > budget_s:
+            break
+    g = {k: hmean([e / (t * 1e-3) / 1e9 for e, *ts in rows for t in [ts[i]]])
+         for i, k in enumerate(("beamer", "policy", "best"))}
+    return {"roots": len(rows), "policy": policy,
+            "beamer_gteps": round(g["beamer"], 3), "policy_gteps": round(g["policy"], 3),
+            "best_single_flip_gteps": round(g["best"], 3),
+            "policy_over_beamer": round(g["policy"] / g["beamer"], 4),
+            "best_over_beamer": round(g["best"] / g["beamer"], 4),
+            "seconds": round(time.perf_counter() - t0, 1)}
 
 
 def bu_bytes(rec, nv, n_isolated):
@@ -451,6 +488,8 @@ def main():
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--roofline-roots", type=int, default=8)
     ap.add_argument("--cpu-budget", type=float, default=20.0)
+    ap.add_argument("--bound-budget", type=float, default=40.0,
+                    help="seconds for the direction-choice bound (roots in order until spent)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--model", default=MODEL, help="direction network parameters (float32)")
     ap.add_argument("--cc-runs", type=int, default=5, help="timed runs of --leg components")
@@ -536,6 +575,10 @@ def main():
     checked = check_roots(ctx, roots, policies)
     log(f"check: {'PASS' if checked['pass'] else 'FAIL'} on {len(roots)} roots x "
         f"{len(policies)} policies")
+    bound = direction_bound(ctx, roots, policy, args.bound_budget) if have_model else None
+    if bound:
+        log(f"direction bound over {bound['roots']} roots: {policy} {bound['policy_over_beamer']}x, "
+            f"best single flip {bound['best_over_beamer']}x Beamer ({bound['seconds']} s)")
     batch = batched_roots(ctx, roots, results["beamer"]["ms_per_pass"])
     log(f"64 roots batched: {batch['ms']:.3f} ms vs {batch['sequential_ms']:.3f} ms sequential")
     roof = roofline(ctx, roots[:args.roofline_roots], policy, nv, n_isolated)
@@ -582,6 +625,7 @@ def main():
         "policies": results,
         "learned_vs_beamer": (round(results["learned"]["hmean_gteps"] /
                                     results["beamer"]["hmean_gteps"], 4) if have_model else None),
+        "direction_bound": bound,
         "components": {"metric": "components::run time on the same graph", **cc},
     }
     print(json.dumps(out))
