@@ -19,11 +19,17 @@
 //   pull: every v not yet seen by all roots ORs F[u] over its neighbours
 //         that are in Fb, stopping as soon as every root it still waits for
 //         is covered.
-// The host picks the mode per level from the previous level's counters (the
-// frontier's degree sum against the degree sum of vertices still open), as
-// Beamer's rule picks a direction.  Depths are written root-major
-// (depth[r * nv + v]) when found: 64 masked stores per wave, one per root,
-// each covering 64 consecutive vertices of that root's row.
+// Each level is two launches, k_ms_level (push or pull) and k_ms_post
+// (commit, or clearing the old frontier masks after a pull).  Both pick the
+// mode on the device from the counters of the frontier -- its degree sum
+// against the degree sum of vertices still open, as Beamer's rule picks a
+// direction -- so a search is a fixed sequence of launches with no host
+// round trip: the host captures a chunk of levels in a HIP graph once and
+// replays it; launches past the last level return at once.  The search runs
+// on the degree-ordered layout the single searches read (bd_graph.hip);
+// depths are written root-major in its ids (depth[r * nv + i]) when found, 64
+// masked stores per wave, one per root, each covering 64 consecutive
+// vertices of that root's row, and mapped to graph ids on readout.
 #include <string.h>
 
 #include <algorithm>
@@ -38,22 +44,41 @@ namespace {
 
 constexpr int kMsBlock = 256;
 constexpr int kMsWaveDeg = 64;  // lists at least this long are scanned by a whole wave
+constexpr int kMsChunk = 16;    // levels per captured graph
+
+// Device control of a bit-parallel search: done once a frontier is empty,
+// `levels` the number of non-empty frontiers.
+struct ms_ctl {
+    int32_t done;
+    int32_t levels;
+};
 
 struct ms_args {
     const int64_t *off;
     const int32_t *cols;
+    const int32_t *inv;  // graph id -> search id (null: the search runs on graph ids)
     uint64_t *F;
     uint64_t *N;
     uint64_t *seen;
     uint64_t *Fb;    // bit v: F[v] != 0 (frontier membership, V/64 words)
     uint64_t *Nb;    // bit v: N[v] != 0, written by the commit or pull pass
     int32_t *depth;  // n rows of nv, or null
-    unsigned long long *cnt;  // [kSlots][kLine]: frontier size, degree sum, open degree sum
+    // counters of frontier L in copy L % 3, kSlots lines each: frontier
+    // size, its degree sum, the degree sum of vertices still open
+    unsigned long long *cnt;
+    ms_ctl *ctl;
+    bd_level_rec *log;
+    int64_t log_cap;
     int64_t nv;
     uint64_t full;   // bits of the roots in this search
+    uint64_t alpha;  // push while frontier degree sum * alpha < open degree sum
     int32_t nroots;
     int32_t level;   // the step expands depth `level` into depth `level + 1`
 };
+
+__device__ __forceinline__ unsigned long long *ms_copy(const ms_args &a, int level) {
+    return a.cnt + (int64_t)(level % 3) * kSlots * kLine;
+}
 
 // Record the roots in `nf` as reaching v at depth d: root-major rows, one
 // masked store per root, so each store covers the wave's 64 vertices.
@@ -85,7 +110,8 @@ __device__ __forceinline__ void ms_push_to(const ms_args &a, int32_t u, uint64_t
     if (m) atomicOr((unsigned long long *)&a.N[u], (unsigned long long)m);
 }
 
-// Per-block counters added to slot blockIdx % kSlots.
+// Per-block counters of the next frontier, added to slot blockIdx % kSlots
+// of copy (level + 1) % 3.
 __device__ __forceinline__ void ms_count(const ms_args &a, unsigned long long c0,
                                          unsigned long long c1, unsigned long long c2) {
     __shared__ unsigned long long red[3][kMsBlock / 64];
@@ -105,19 +131,51 @@ __device__ __forceinline__ void ms_count(const ms_args &a, unsigned long long c0
             c1 += red[1][w];
             c2 += red[2][w];
         }
-        unsigned long long *slot = a.cnt + (blockIdx.x % kSlots) * kLine;
+        unsigned long long *slot = ms_copy(a, a.level + 1) + (blockIdx.x % kSlots) * kLine;
         if (c0) atomicAdd(&slot[0], c0);
         if (c1) atomicAdd(&slot[1], c1);
         if (c2) atomicAdd(&slot[2], c2);
     }
 }
 
-// Roots start their own frontier: bit i on roots[i].
+// The state of level `level` as every workgroup sees it: the counters of
+// its frontier (summed over the slots by the first wave) and the search's
+// done flag, loaded together.  push: the mode Beamer's rule picks.
+struct ms_state {
+    unsigned long long front, fdeg, open;
+    int32_t done;
+    int32_t push;
+};
+
+__device__ __forceinline__ void ms_load_state(const ms_args &a, ms_state &st) {
+    __shared__ ms_state sh;
+    if (threadIdx.x < 64) {
+        const int l = lane_id();
+        const unsigned long long *c = ms_copy(a, a.level) + l * kLine;
+        const unsigned long long f = c[0], d = c[1], o = c[2];
+        const int32_t done = __hip_atomic_load(&a.ctl->done, __ATOMIC_RELAXED,
+                                               __HIP_MEMORY_SCOPE_AGENT);
+        const unsigned long long fs = wave_sum(f), ds = wave_sum(d), os = wave_sum(o);
+        if (l == 0) {
+            sh.front = fs;
+            sh.fdeg = ds;
+            sh.open = os;
+            sh.done = done || fs == 0;
+            sh.push = ds * a.alpha < os;
+        }
+    }
+    __syncthreads();
+    st = sh;
+}
+
+// Roots start their own frontier: bit i on roots[i] (graph ids, mapped to
+// search ids).  Counts frontier 0 into copy 0.
 __global__ void k_ms_init(ms_args a, const int64_t *__restrict__ roots, int n) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     unsigned long long c0 = 0, c1 = 0, c2 = 0;
     if (i < n) {
-        const int64_t v = roots[i];
+        int64_t v = roots[i];
+        if (a.inv) v = a.inv[v];
         const uint64_t bit = 1ULL << i;
         const unsigned long long old = atomicOr((unsigned long long *)&a.F[v], bit);
         atomicOr((unsigned long long *)&a.seen[v], bit);
@@ -141,7 +199,7 @@ __global__ void k_ms_init(ms_args a, const int64_t *__restrict__ roots, int n) {
 }
 
 // Push: expand the frontier into N with device atomics.
-__global__ __launch_bounds__(kMsBlock) void k_ms_push(ms_args a) {
+__device__ __forceinline__ void ms_push(const ms_args &a) {
     const int lane = lane_id();
     const int64_t wave = ((int64_t)blockIdx.x * kMsBlock + threadIdx.x) >> 6;
     const int64_t nwaves = ((int64_t)gridDim.x * kMsBlock) >> 6;
@@ -155,10 +213,30 @@ __global__ __launch_bounds__(kMsBlock) void k_ms_push(ms_args a) {
             b = a.off[v];
             e = a.off[v + 1];
         }
-        const bool alone = f && e - b < kMsWaveDeg;
-        if (alone)
-            for (int64_t k = b; k < e; ++k) ms_push_to(a, ms_col(a, k), f);
-        uint64_t lm = __ballot(f && !alone);
+        // lists shorter than a wave are laid end to end and walked 64
+        // entries at a time, one per lane (the owner found by a search over
+        // the lanes' prefix); longer ones by the whole wave
+        const int32_t len = (int32_t)(f && e - b < kMsWaveDeg ? e - b : 0);
+        int32_t excl = wave_incl_scan32(len);
+        const int32_t total = lane_bcast(excl, 63);
+        excl -= len;
+        for (int32_t k0 = 0; k0 < total; k0 += 64) {
+            const int32_t idx = k0 + lane;
+            int i = 0;
+#pragma unroll
+            for (int st = 32; st > 0; st >>= 1) {
+                const int mid = i + st;
+                const int32_t em = __shfl(excl, mid < 64 ? mid : 63);
+                if (mid < 64 && em <= idx) i = mid;
+            }
+            // every lane takes part in the shuffles (a lane past the end
+            // still reads for the others)
+            const int64_t bi = __shfl(b, i);
+            const uint64_t fi = (uint64_t)__shfl((long long)f, i);
+            const int32_t pos = idx - __shfl(excl, i);
+            if (idx < total) ms_push_to(a, ms_col(a, bi + pos), fi);
+        }
+        uint64_t lm = __ballot(f && e - b >= kMsWaveDeg);
         while (lm) {
             const int src = __ffsll((unsigned long long)lm) - 1;
             lm &= lm - 1;
@@ -171,7 +249,7 @@ __global__ __launch_bounds__(kMsBlock) void k_ms_push(ms_args a) {
 
 // Commit after a push: keep the unseen bits of N, mark them seen, record
 // depths, clear F (it becomes the next push's target), count.
-__global__ __launch_bounds__(kMsBlock) void k_ms_commit(ms_args a) {
+__device__ __forceinline__ void ms_commit(const ms_args &a) {
     unsigned long long c0 = 0, c1 = 0, c2 = 0;
     const int64_t stride = (int64_t)gridDim.x * kMsBlock;
     const int64_t nround = (a.nv + 63) / 64 * 64;
@@ -204,7 +282,7 @@ __global__ __launch_bounds__(kMsBlock) void k_ms_commit(ms_args a) {
 
 // Pull: every open vertex ORs its neighbours' frontier masks, stopping once
 // all the roots it still waits for are covered; writes N for every vertex.
-__global__ __launch_bounds__(kMsBlock) void k_ms_pull(ms_args a) {
+__device__ __forceinline__ void ms_pull(const ms_args &a) {
     const int lane = lane_id();
     unsigned long long c0 = 0, c1 = 0, c2 = 0;
     const int64_t wave = ((int64_t)blockIdx.x * kMsBlock + threadIdx.x) >> 6;
@@ -264,12 +342,88 @@ __global__ __launch_bounds__(kMsBlock) void k_ms_pull(ms_args a) {
     ms_count(a, c0, c1, c2);
 }
 
+// Level L, first launch: the state, the record of levels L-1 and L
+// (workgroup 0), then the push or the pull.
+__global__ __launch_bounds__(kMsBlock) void k_ms_level(ms_args a) {
+    ms_state st;
+    ms_load_state(a, st);
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        const bool was_done = a.ctl->done != 0;
+        if (!was_done && a.level > 0 && a.level - 1 < a.log_cap) {
+            a.log[a.level - 1].next = (int64_t)st.front;
+            a.log[a.level - 1].next_deg = (int64_t)st.fdeg;
+        }
+        if (!was_done && st.front == 0) {
+            a.ctl->done = 1;
+            a.ctl->levels = a.level;
+        }
+        if (!st.done && a.level < a.log_cap) {
+            bd_level_rec r;
+            memset(&r, 0, sizeof(r));
+            r.level = a.level;
+            r.dir = st.push ? 0 : 1;
+            r.frontier = (int64_t)st.front;
+            r.frontier_deg = (int64_t)st.fdeg;
+            r.unvisited_deg = (int64_t)st.open;
+            r.ms = -1.0;
+            r.ms_other = -1.0;
+            r.launch = 2 * a.level;
+            r.launches = 2;
+            a.log[a.level] = r;
+        }
+    }
+    if (st.done) return;
+    if (st.push)
+        ms_push(a);
+    else
+        ms_pull(a);
+}
+
+// Level L, second launch: the commit after a push; after a pull, the old
+// frontier's masks are cleared (they become the next level's N).  Workgroup
+// 0 clears the counters copy that level L+1 fills.
+__global__ __launch_bounds__(kMsBlock) void k_ms_post(ms_args a) {
+    ms_state st;
+    ms_load_state(a, st);
+    if (st.done) return;
+    if (blockIdx.x == 0 && threadIdx.x < 64) {
+        unsigned long long *c = ms_copy(a, a.level + 2) + threadIdx.x * kLine;
+        c[0] = 0;
+        c[1] = 0;
+        c[2] = 0;
+    }
+    if (st.push) {
+        ms_commit(a);
+    } else {
+        const int64_t nw = (a.nv + 63) / 64;
+        for (int64_t w = (int64_t)blockIdx.x * kMsBlock + threadIdx.x; w < nw;
+             w += (int64_t)gridDim.x * kMsBlock)
+            for (uint64_t m = a.Fb[w]; m; m &= m - 1) a.F[w * 64 + __ffsll((unsigned long long)m) - 1] = 0;
+    }
+}
+
 __global__ void k_fill_i32(int32_t *p, int64_t n, int32_t x) {
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
          i += (int64_t)gridDim.x * blockDim.x)
         p[i] = x;
 }
 
+__global__ void k_fill_u64(uint64_t *p, int64_t n) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x)
+        p[i] = 0;
+}
+
+// Row r of the depths in graph ids: out[v] = depth[r * nv + inv[v]].
+__global__ void k_ms_row(const int32_t *__restrict__ depth, const int32_t *__restrict__ inv,
+                         int64_t nv, int32_t *__restrict__ out) {
+    for (int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v < nv;
+         v += (int64_t)gridDim.x * blockDim.x)
+        out[v] = depth[inv[v]];
+}
+
+// Masks, bitmaps, three counter copies with the control block, and (nrows
+// > 0) the depth rows.
 int ms_buffers(bd_ctx *c, int nrows) {
     int rc;
     for (int i = 0; i < 3; ++i)
@@ -277,7 +431,8 @@ int ms_buffers(bd_ctx *c, int nrows) {
             (rc = dev_alloc(c, (void **)&c->d_ms[i], sizeof(uint64_t) * c->nv, "root masks")))
             return rc;
     if (!c->d_ms[3] &&
-        (rc = dev_alloc(c, (void **)&c->d_ms[3], sizeof(unsigned long long) * kSlots * kLine,
+        (rc = dev_alloc(c, (void **)&c->d_ms[3],
+                        sizeof(unsigned long long) * 3 * kSlots * kLine + sizeof(ms_ctl),
                         "mask counters")))
         return rc;
     for (int i = 0; i < 2; ++i)
@@ -296,18 +451,28 @@ int ms_buffers(bd_ctx *c, int nrows) {
     return BD_OK;
 }
 
-int ms_read_counts(bd_ctx *c, unsigned long long out[3]) {
-    std::vector<unsigned long long> h(kSlots * kLine);
-    BD_HIP(c, hipMemcpyAsync(h.data(), c->d_ms[3], sizeof(unsigned long long) * h.size(),
-                             hipMemcpyDeviceToHost, c->stream));
-    BD_HIP(c, hipStreamSynchronize(c->stream));
-    out[0] = out[1] = out[2] = 0;
-    for (int s = 0; s < kSlots; ++s)
-        for (int k = 0; k < 3; ++k) out[k] += h[s * kLine + k];
+// The two launches of level L (F and N swap roles with the level's parity).
+int ms_enqueue_level(bd_ctx *c, ms_args a, int64_t L, int64_t grid) {
+    a.level = (int32_t)L;
+    uint64_t *m[2] = {c->d_ms[0], c->d_ms[1]};
+    uint64_t *bits[2] = {c->d_ms_bits[0], c->d_ms_bits[1]};
+    a.F = m[L & 1];
+    a.N = m[(L + 1) & 1];
+    a.Fb = bits[L & 1];
+    a.Nb = bits[(L + 1) & 1];
+    hipLaunchKernelGGL(k_ms_level, dim3(grid), dim3(kMsBlock), 0, c->stream, a);
+    hipLaunchKernelGGL(k_ms_post, dim3(grid), dim3(kMsBlock), 0, c->stream, a);
+    BD_HIP(c, hipGetLastError());
     return BD_OK;
 }
 
 }  // namespace
+
+void ms_free_graph(bd_ctx *c) {
+    if (c->ms_graph) (void)hipGraphExecDestroy((hipGraphExec_t)c->ms_graph);
+    c->ms_graph = nullptr;
+    c->ms_graph_key.clear();
+}
 
 int run_multi_root(bd_ctx *c, const int64_t *roots, int n, int32_t *depths, double *ms,
                    int64_t *levels) {
@@ -328,8 +493,9 @@ int run_multi_root(bd_ctx *c, const int64_t *roots, int n, int32_t *depths, doub
     } g{d_roots};
     const int64_t nv = c->nv;
     ms_args a;
-    a.off = c->d_off;
-    a.cols = c->d_cols;
+    a.off = c->d_soff;
+    a.cols = c->d_scols;
+    a.inv = c->d_inv;
     a.F = c->d_ms[0];
     a.N = c->d_ms[1];
     a.seen = c->d_ms[2];
@@ -337,84 +503,122 @@ int run_multi_root(bd_ctx *c, const int64_t *roots, int n, int32_t *depths, doub
     a.Nb = c->d_ms_bits[1];
     a.depth = depths ? c->d_ms_depth : nullptr;
     a.cnt = (unsigned long long *)c->d_ms[3];
+    a.ctl = (ms_ctl *)(a.cnt + 3 * kSlots * kLine);
+    a.log = c->d_log;
+    a.log_cap = c->log_cap;
     a.nv = nv;
     a.full = n == 64 ? ~0ULL : ((1ULL << n) - 1);
+    a.alpha = (uint64_t)std::max(1, c->tune.ms_alpha);
     a.nroots = n;
     a.level = 0;
-    const size_t mbytes = sizeof(uint64_t) * nv;
-    const size_t cbytes = sizeof(unsigned long long) * kSlots * kLine;
-    // clear(): untimed, as in the reference's trial loop
+    const int64_t grid = (int64_t)c->cu_count * std::max(1, c->tune.grid_per_cu);
+    // clear(): untimed, as in the reference's trial loop; kernels only
     BD_HIP(c, hipMemcpyAsync(d_roots, roots, sizeof(int64_t) * n, hipMemcpyHostToDevice,
                              c->stream));
-    for (int i = 0; i < 3; ++i) BD_HIP(c, hipMemsetAsync(c->d_ms[i], 0, mbytes, c->stream));
+    for (int i = 0; i < 3; ++i)
+        hipLaunchKernelGGL(k_fill_u64, dim3(grid_for(c, nv, 256)), dim3(256), 0, c->stream,
+                           c->d_ms[i], nv);
     for (int i = 0; i < 2; ++i)
-        BD_HIP(c, hipMemsetAsync(c->d_ms_bits[i], 0, sizeof(uint64_t) * c->nwords, c->stream));
+        hipLaunchKernelGGL(k_fill_u64, dim3(grid_for(c, c->nwords, 256)), dim3(256), 0,
+                           c->stream, c->d_ms_bits[i], c->nwords);
+    hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(256), 0, c->stream, (uint64_t *)a.cnt,
+                       (int64_t)(3 * kSlots * kLine + sizeof(ms_ctl) / 8));
     if (a.depth) {
         const int64_t cells = (int64_t)n * nv;
         hipLaunchKernelGGL(k_fill_i32, dim3(grid_for(c, cells, 256)), dim3(256), 0, c->stream,
                            a.depth, cells, -1);
-        BD_HIP(c, hipGetLastError());
     }
-    BD_HIP(c, hipMemsetAsync(a.cnt, 0, cbytes, c->stream));
+    BD_HIP(c, hipGetLastError());
     BD_HIP(c, hipStreamSynchronize(c->stream));
 
-    const int64_t grid = (int64_t)c->cu_count * std::max(1, c->tune.grid_per_cu);
+    // Level launches: a captured chunk of kMsChunk levels, replayed (and kept
+    // for the next search with the same buffers); host-driven (level_driver
+    // 2): the same launches, enqueued and waited for level by level.
+    const bool host_driven = c->tune.level_driver == 2;
+    char key[160];
+    snprintf(key, sizeof(key), "%p %p %p %ld %d %lu %ld", (void *)a.off, (void *)a.depth,
+             (void *)a.cnt, (long)nv, n, (unsigned long)a.alpha, (long)grid);
+    if (!host_driven && (!c->ms_graph || c->ms_graph_key != key)) {
+        ms_free_graph(c);
+        hipGraph_t graph = nullptr;
+        BD_HIP(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+        int erc = BD_OK;
+        for (int64_t L = 0; L < kMsChunk && !erc; ++L) erc = ms_enqueue_level(c, a, L, grid);
+        const hipError_t ce = hipStreamEndCapture(c->stream, &graph);
+        if (erc) {
+            if (graph) (void)hipGraphDestroy(graph);
+            return erc;
+        }
+        BD_HIP(c, ce);
+        hipGraphExec_t exec = nullptr;
+        const hipError_t ie = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(graph);
+        BD_HIP(c, ie);
+        c->ms_graph = (void *)exec;
+        c->ms_graph_key = key;
+    }
     BD_HIP(c, hipEventRecord(c->ev[0], c->stream));
     hipLaunchKernelGGL(k_ms_init, dim3((n + 63) / 64), dim3(64), 0, c->stream, a, d_roots, n);
     BD_HIP(c, hipGetLastError());
-    unsigned long long cnt[3];
-    if ((rc = ms_read_counts(c, cnt))) return rc;
-    bool n_clean = true;  // N holds zeros (ready for a push)
+    ms_ctl h{0, 0};
     int64_t L = 0;
-    // push while the frontier's edges * alpha are below the open edges
-    const unsigned long long alpha = (unsigned long long)std::max(1, c->tune.ms_alpha);
-    std::vector<bd_level_rec> log;
-    while (cnt[0] > 0) {
-        if (L > nv + 1) return set_err(c, BD_EHIP, "multi-root search did not terminate");
-        a.level = (int32_t)L;
-        const bool push = cnt[1] * alpha < cnt[2];
-        bd_level_rec r;
-        memset(&r, 0, sizeof(r));
-        r.level = L;
-        r.dir = push ? 0 : 1;
-        r.frontier = (int64_t)cnt[0];
-        r.frontier_deg = (int64_t)cnt[1];
-        r.unvisited_deg = (int64_t)cnt[2];
-        r.ms_other = -1.0;
-        BD_HIP(c, hipEventRecord(c->ev[2], c->stream));
-        BD_HIP(c, hipMemsetAsync(a.cnt, 0, cbytes, c->stream));
-        if (push) {
-            if (!n_clean) BD_HIP(c, hipMemsetAsync(a.N, 0, mbytes, c->stream));
-            hipLaunchKernelGGL(k_ms_push, dim3(grid), dim3(kMsBlock), 0, c->stream, a);
-            hipLaunchKernelGGL(k_ms_commit, dim3(grid), dim3(kMsBlock), 0, c->stream, a);
-            n_clean = true;  // commit cleared F, the next level's N
+    std::vector<float> level_ms;  // host-driven: each level's two launches, timed by events
+    for (;;) {
+        if (L > nv + 2) return set_err(c, BD_EHIP, "multi-root search did not terminate");
+        if (host_driven) {
+            BD_HIP(c, hipEventRecord(c->ev[2], c->stream));
+            if ((rc = ms_enqueue_level(c, a, L, grid))) return rc;
+            BD_HIP(c, hipEventRecord(c->ev[3], c->stream));
+            BD_HIP(c, hipEventSynchronize(c->ev[3]));
+            float lm = 0;
+            BD_HIP(c, hipEventElapsedTime(&lm, c->ev[2], c->ev[3]));
+            level_ms.push_back(lm);
+            L += 1;
+        } else if (L == 0) {
+            BD_HIP(c, hipGraphLaunch((hipGraphExec_t)c->ms_graph, c->stream));
+            L = kMsChunk;
         } else {
-            hipLaunchKernelGGL(k_ms_pull, dim3(grid), dim3(kMsBlock), 0, c->stream, a);
-            n_clean = false;  // F still holds depth L
+            // past the captured chunk (deep searches): plain launches
+            for (int k = 0; k < kMsChunk; ++k, ++L)
+                if ((rc = ms_enqueue_level(c, a, L, grid))) return rc;
         }
-        BD_HIP(c, hipGetLastError());
-        BD_HIP(c, hipEventRecord(c->ev[3], c->stream));
-        if ((rc = ms_read_counts(c, cnt))) return rc;
-        float lms = 0;
-        BD_HIP(c, hipEventElapsedTime(&lms, c->ev[2], c->ev[3]));
-        r.ms = lms;
-        r.next = (int64_t)cnt[0];
-        r.next_deg = (int64_t)cnt[1];
-        log.push_back(r);
-        std::swap(a.F, a.N);
-        std::swap(a.Fb, a.Nb);
-        ++L;
+        BD_HIP(c, hipMemcpyAsync(&h, a.ctl, sizeof(h), hipMemcpyDeviceToHost, c->stream));
+        BD_HIP(c, hipStreamSynchronize(c->stream));
+        if (h.done) break;
     }
     BD_HIP(c, hipEventRecord(c->ev[1], c->stream));
     BD_HIP(c, hipEventSynchronize(c->ev[1]));
     float t = 0;
     BD_HIP(c, hipEventElapsedTime(&t, c->ev[0], c->ev[1]));
     if (ms) *ms = t;
-    if (levels) *levels = L;
-    c->last_log = log;
+    if (levels) *levels = h.levels;
+    const int64_t nrec = std::min<int64_t>(h.levels, c->log_cap);
+    c->last_log.resize(nrec);
+    if (nrec)
+        BD_HIP(c, hipMemcpy(c->last_log.data(), c->d_log, sizeof(bd_level_rec) * nrec,
+                            hipMemcpyDeviceToHost));
+    for (int64_t i = 0; i < nrec && i < (int64_t)level_ms.size(); ++i) c->last_log[i].ms = level_ms[i];
     c->have_result = false;  // bd_bfs_levels now describes this search, not the last single one
-    if (depths)
-        BD_HIP(c, hipMemcpy(depths, a.depth, sizeof(int32_t) * n * nv, hipMemcpyDeviceToHost));
+    if (depths) {
+        if (!c->d_inv) {
+            BD_HIP(c, hipMemcpy(depths, a.depth, sizeof(int32_t) * n * nv, hipMemcpyDeviceToHost));
+        } else {
+            int32_t *row = nullptr;
+            if ((rc = dev_alloc(c, (void **)&row, sizeof(int32_t) * nv, "depth row"))) return rc;
+            struct rguard {
+                void *p;
+                ~rguard() { dev_free(p); }
+            } rg{row};
+            for (int r = 0; r < n; ++r) {
+                hipLaunchKernelGGL(k_ms_row, dim3(grid_for(c, nv, 256)), dim3(256), 0, c->stream,
+                                   a.depth + (int64_t)r * nv, c->d_inv, nv, row);
+                BD_HIP(c, hipGetLastError());
+                BD_HIP(c, hipMemcpyAsync(depths + (int64_t)r * nv, row, sizeof(int32_t) * nv,
+                                         hipMemcpyDeviceToHost, c->stream));
+            }
+            BD_HIP(c, hipStreamSynchronize(c->stream));
+        }
+    }
     return BD_OK;
 }
 

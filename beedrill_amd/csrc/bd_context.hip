@@ -67,6 +67,7 @@ static void free_search(bd_ctx *c) {
     for (auto &b : c->d_bm) dev_free(b);
     for (auto &q : c->d_q) dev_free(q);
     for (auto &t : c->d_tasks) dev_free(t);
+    ms_free_graph(c);
     for (auto &b : c->d_ms) dev_free(b);
     for (auto &b : c->d_ms_bits) dev_free(b);
     dev_free(c->d_ms_depth);

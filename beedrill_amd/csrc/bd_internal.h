@@ -183,6 +183,8 @@ struct bd_ctx {
     uint64_t *d_ms_bits[2] = {nullptr, nullptr};              // frontier bitmaps
     int32_t *d_ms_depth = nullptr;
     int64_t ms_depth_rows = 0;
+    void *ms_graph = nullptr;      // hipGraphExec_t of the captured level chunk
+    std::string ms_graph_key;      // the buffers and sizes it was captured with
 
     int64_t *d_static = nullptr;  // this device's address of bd_bfs.hip's `scratch`
 
@@ -210,6 +212,8 @@ int build_head_index(bd_ctx *c);
 // it is built in that order); frees it first when the order changed.
 int build_search_layout(bd_ctx *c);
 void free_search_layout(bd_ctx *c);
+// Drop the bit-parallel search's captured graph (its buffers are going).
+void ms_free_graph(bd_ctx *c);
 // Look up this device's address of the allocation-free entry points'
 // staging array (with the context's device current).
 int static_scratch_init(bd_ctx *c);

@@ -274,15 +274,29 @@ def summarize_pmc(fetch_csv, write_csv, out_path, kernel_re=STEP_KERNEL_RE):
 
 
 def batched_roots(ctx, roots, sequential_ms):
-    """The same 64 roots in one bit-parallel search (depths of every root
-    written), against one pass of the 64 sequential searches (the mean over
-    the timed passes of the sum of their device times)."""
+    """The same 64 roots in one bit-parallel search, against one pass of the
+    64 sequential searches (the mean over the timed passes of the sum of
+    their device times, which write one parent array per root).  Timed twice:
+    writing every root's depth row (64 rows of V entries) and without (the
+    search alone); best of 3 each.  levels: push (0) or pull (1) per level,
+    and each level's time from a host-driven run."""
     ctx.multi_root(roots[:8], want_depths=False)  # warm up
-    dp, ms, levels = ctx.multi_root(roots, want_depths=True)
-    del dp
-    return {"roots": len(roots), "ms": round(ms, 4), "levels": int(levels),
-            "sequential_ms": round(sequential_ms, 4),
-            "speedup_vs_sequential": round(sequential_ms / ms, 3) if ms > 0 else None}
+    ms_d = min(ctx.multi_root(roots, want_depths=True)[1] for _ in range(3))
+    ms_s = min(ctx.multi_root(roots, want_depths=False)[1] for _ in range(3))
+    _, _, levels = ctx.multi_root(roots, want_depths=False)
+    t = ctx.get_tuning()
+    try:
+        ctx.set_tuning(level_driver=2)
+        ctx.multi_root(roots, want_depths=False)
+        per_level = [{"mode": "push" if r["dir"] == 0 else "pull", "frontier": r["frontier"],
+                      "ms": round(r["ms"], 3)} for r in ctx.levels()]
+    finally:
+        ctx.set_tuning(level_driver=t["level_driver"])
+    return {"roots": len(roots), "ms": round(ms_d, 4), "search_only_ms": round(ms_s, 4),
+            "levels": int(levels), "sequential_ms": round(sequential_ms, 4),
+            "speedup_vs_sequential": round(sequential_ms / ms_d, 3) if ms_d > 0 else None,
+            "search_only_speedup": round(sequential_ms / ms_s, 3) if ms_s > 0 else None,
+            "per_level_host_driven": per_level}
 
 
 def hmean_policy(ctx, roots, policy):
@@ -580,7 +594,9 @@ def main():
         log(f"direction bound over {bound['roots']} roots: {policy} {bound['policy_over_beamer']}x, "
             f"best single flip {bound['best_over_beamer']}x Beamer ({bound['seconds']} s)")
     batch = batched_roots(ctx, roots, results["beamer"]["ms_per_pass"])
-    log(f"64 roots batched: {batch['ms']:.3f} ms vs {batch['sequential_ms']:.3f} ms sequential")
+    log(f"64 roots batched: {batch['ms']:.3f} ms with depth rows, {batch['search_only_ms']:.3f} ms "
+        f"search alone, vs {batch['sequential_ms']:.3f} ms sequential; levels "
+        f"{[(x['mode'], x['ms']) for x in batch['per_level_host_driven']]}")
     roof = roofline(ctx, roots[:args.roofline_roots], policy, nv, n_isolated)
     log(f"roofline: {roof['achieved']} GB/s over {roof['launches']} step launches; bottom-up "
         f"steps {roof['bu_levels']['achieved']} GB/s")

@@ -425,6 +425,34 @@ def test_multi_root_s16_and_lattice(ctx, orc):
         np.testing.assert_array_equal(dp[r], orc.bfs_depth(off, adj, int(root)))
 
 
+def test_multi_root_graph_replays_match_host_driven(ctx, orc):
+    """The level launches captured once in a HIP graph and replayed, against
+    the same launches enqueued level by level from the host: the same depth
+    rows and level records on every replay, with ordinary searches (eager
+    work and synchronisation) between the replays."""
+    ctx.generate_rmat(14, 16, seed=9)
+    off, adj = ctx.csr()
+    roots = ctx.pick_roots(64)
+    t0 = ctx.get_tuning()
+    try:
+        ctx.set_tuning(level_driver=2)
+        ref, _, ref_levels = ctx.multi_root(roots)
+        ref_rec = [(r["dir"], r["frontier"], r["frontier_deg"], r["next"]) for r in ctx.levels()]
+        ctx.set_tuning(level_driver=1)
+        for rep in range(4):
+            dp, ms, levels = ctx.multi_root(roots)
+            assert levels == ref_levels and ms > 0
+            np.testing.assert_array_equal(dp, ref)
+            assert [(r["dir"], r["frontier"], r["frontier_deg"], r["next"])
+                    for r in ctx.levels()] == ref_rec
+            ctx.bfs(int(roots[rep]))
+    finally:
+        ctx.set_tuning(**t0)
+    assert len(ref_rec) == ref_levels
+    for r in (0, 31, 63):
+        np.testing.assert_array_equal(ref[r], orc.bfs_depth(off, adj, int(roots[r])))
+
+
 def test_multi_root_errors(ctx):
     nv, s, d = path_graph(10)
     ctx.from_edges(nv, s, d)
