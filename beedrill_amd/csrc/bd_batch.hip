@@ -45,6 +45,22 @@ namespace {
 constexpr int kMsBlock = 256;
 constexpr int kMsWaveDeg = 64;  // lists at least this long are scanned by a whole wave
 constexpr int kMsChunk = 16;    // levels per captured graph
+// Frontier lists of kMsWaveDeg entries or more are pushed as slices of up
+// to kMsSlice entries, one wave each, taken by all waves of the grid: the
+// step that finds a frontier vertex files its slices (a vertex group's own
+// wave would otherwise walk every long list of its 64 vertices, and in the
+// degree-ordered layout the high-degree vertices share the first groups).
+constexpr int kMsSlice = 1024;
+// Slice counts of frontier L sit in its counter copy: line s holds, in
+// word 3, the slices filed in segment s (by waves w with w % kSlots == s),
+// and line 0, word 4, those filed in the overflow segment.
+constexpr int kMsSegCount = 3, kMsOvfCount = 4;
+
+struct ms_slice {
+    int64_t beg;
+    int32_t len;
+    int32_t v;
+};
 
 // Device control of a bit-parallel search: done once a frontier is empty,
 // `levels` the number of non-empty frontiers.
@@ -67,6 +83,9 @@ struct ms_args {
     // size, its degree sum, the degree sum of vertices still open
     unsigned long long *cnt;
     ms_ctl *ctl;
+    ms_slice *slices[2];  // frontier L's slices in slices[L & 1]
+    int64_t seg_cap;      // slices per segment; the overflow segment holds max_slices
+    int64_t max_slices;   // most slices one frontier can have
     bd_level_rec *log;
     int64_t log_cap;
     int64_t nv;
@@ -138,6 +157,46 @@ __device__ __forceinline__ void ms_count(const ms_args &a, unsigned long long c0
     }
 }
 
+// File the slices of the lanes' new frontier vertices whose lists have
+// kMsWaveDeg entries or more (frontier `level`, so slices[level & 1] and
+// counter copy level % 3).  Wave-collective; `wave` numbers the caller's wave.
+__device__ __forceinline__ void ms_file_slices(const ms_args &a, int level, int64_t wave,
+                                               bool is_new, int64_t v, int64_t b, int64_t e) {
+    const int lane = lane_id();
+    const int64_t deg = e - b;
+    const int32_t ns = (is_new && deg >= kMsWaveDeg) ? (int32_t)((deg + kMsSlice - 1) / kMsSlice) : 0;
+    if (!__ballot(ns > 0)) return;
+    int32_t excl = wave_incl_scan32(ns);
+    const int32_t total = lane_bcast(excl, 63);
+    excl -= ns;
+    const int seg = (int)(wave % kSlots);
+    unsigned long long *copy = ms_copy(a, level);
+    int64_t base = 0;
+    if (lane == 0) {
+        const unsigned long long p =
+            atomicAdd(&copy[seg * kLine + kMsSegCount], (unsigned long long)total);
+        if ((int64_t)p + total <= a.seg_cap) {
+            base = seg * a.seg_cap + (int64_t)p;
+        } else {
+            // the segment is full: leave its count as it was and use the
+            // overflow segment (no slot of the segment is written past its end)
+            atomicAdd(&copy[seg * kLine + kMsSegCount], (unsigned long long)(-(int64_t)total));
+            const unsigned long long q =
+                atomicAdd(&copy[kMsOvfCount], (unsigned long long)total);
+            base = kSlots * a.seg_cap + (int64_t)q;
+        }
+    }
+    base = lane_bcast(base, 0);
+    ms_slice *out = a.slices[level & 1];
+    for (int32_t k = 0; k < ns; ++k) {
+        ms_slice sl;
+        sl.beg = b + (int64_t)k * kMsSlice;
+        sl.len = (int32_t)(e - sl.beg < kMsSlice ? e - sl.beg : kMsSlice);
+        sl.v = (int32_t)v;
+        out[base + excl + k] = sl;
+    }
+}
+
 // The state of level `level` as every workgroup sees it: the counters of
 // its frontier (summed over the slots by the first wave) and the search's
 // done flag, loaded together.  push: the mode Beamer's rule picks.
@@ -186,6 +245,16 @@ __global__ void k_ms_init(ms_args a, const int64_t *__restrict__ roots, int n) {
             c1 = (unsigned long long)(a.off[v + 1] - a.off[v]);
         }
     }
+    {
+        int64_t v = 0, b = 0, e = 0;
+        if (c0) {
+            v = roots[i];
+            if (a.inv) v = a.inv[v];
+            b = a.off[v];
+            e = a.off[v + 1];
+        }
+        ms_file_slices(a, 0, blockIdx.x, c0 != 0, v, b, e);
+    }
     // open degree: every vertex is open before the search
     if (i == 0) c2 = (unsigned long long)a.off[a.nv];
     c0 = wave_sum(c0);
@@ -198,7 +267,11 @@ __global__ void k_ms_init(ms_args a, const int64_t *__restrict__ roots, int n) {
     }
 }
 
-// Push: expand the frontier into N with device atomics.
+// Push: expand the frontier into N with device atomics.  Lists shorter
+// than a wave are laid end to end, per group of 64 vertices, and walked 64
+// entries at a time, one per lane (the owner found by a search over the
+// lanes' prefix); longer lists were filed as slices when their vertex
+// joined the frontier, and every wave of the grid takes slices in turn.
 __device__ __forceinline__ void ms_push(const ms_args &a) {
     const int lane = lane_id();
     const int64_t wave = ((int64_t)blockIdx.x * kMsBlock + threadIdx.x) >> 6;
@@ -213,9 +286,6 @@ __device__ __forceinline__ void ms_push(const ms_args &a) {
             b = a.off[v];
             e = a.off[v + 1];
         }
-        // lists shorter than a wave are laid end to end and walked 64
-        // entries at a time, one per lane (the owner found by a search over
-        // the lanes' prefix); longer ones by the whole wave
         const int32_t len = (int32_t)(f && e - b < kMsWaveDeg ? e - b : 0);
         int32_t excl = wave_incl_scan32(len);
         const int32_t total = lane_bcast(excl, 63);
@@ -236,14 +306,27 @@ __device__ __forceinline__ void ms_push(const ms_args &a) {
             const int32_t pos = idx - __shfl(excl, i);
             if (idx < total) ms_push_to(a, ms_col(a, bi + pos), fi);
         }
-        uint64_t lm = __ballot(f && e - b >= kMsWaveDeg);
-        while (lm) {
-            const int src = __ffsll((unsigned long long)lm) - 1;
-            lm &= lm - 1;
-            const int64_t kb = lane_bcast(b, src), ke = lane_bcast(e, src);
-            const uint64_t fs = (uint64_t)lane_bcast((int64_t)f, src);
-            for (int64_t k = kb + lane; k < ke; k += 64) ms_push_to(a, ms_col(a, k), fs);
+    }
+    // the slices: segment counts and their prefix, read by every lane
+    const unsigned long long *copy = ms_copy(a, a.level);
+    const int64_t cs = (int64_t)copy[lane * kLine + kMsSegCount];
+    const int64_t co = (int64_t)copy[kMsOvfCount];
+    const int64_t incl = (int64_t)wave_scan64((uint64_t)cs, [](uint64_t x, uint64_t y) { return x + y; });
+    const int64_t nseg = lane_bcast(incl, 63);
+    const int64_t nslices = nseg + co;
+    const ms_slice *in = a.slices[a.level & 1];
+    for (int64_t t = wave; t < nslices; t += nwaves) {
+        int64_t at;
+        if (t < nseg) {
+            // segment: the lane whose inclusive prefix first exceeds t
+            const int sg = __ffsll((unsigned long long)__ballot(incl > t)) - 1;
+            at = (int64_t)sg * a.seg_cap + (t - (lane_bcast(incl, sg) - lane_bcast(cs, sg)));
+        } else {
+            at = kSlots * a.seg_cap + (t - nseg);
         }
+        const ms_slice sl = in[at];
+        const uint64_t f = a.F[sl.v];
+        for (int32_t k = lane; k < sl.len; k += 64) ms_push_to(a, ms_col(a, sl.beg + k), f);
     }
 }
 
@@ -276,6 +359,12 @@ __device__ __forceinline__ void ms_commit(const ms_args &a) {
         const uint64_t word = __ballot(nf != 0);
         if (lane_id() == 0 && v < a.nv) a.Nb[v >> 6] = word;
         ms_record(a, v, valid, nf, a.level + 1);
+        int64_t b = 0, e = 0;
+        if (nf) {
+            b = a.off[v];
+            e = a.off[v + 1];
+        }
+        ms_file_slices(a, a.level + 1, v >> 6, nf != 0, v, b, e);
     }
     ms_count(a, c0, c1, c2);
 }
@@ -328,6 +417,7 @@ __device__ __forceinline__ void ms_pull(const ms_args &a) {
         const uint64_t nf = acc & want;
         const uint64_t word = __ballot(nf != 0);
         if (lane == 0 && base < a.nv) a.Nb[base >> 6] = word;
+        ms_file_slices(a, a.level + 1, wave, nf != 0, v, b, e);
         if (valid) {
             a.N[v] = nf;
             if (nf) {
@@ -391,6 +481,8 @@ __global__ __launch_bounds__(kMsBlock) void k_ms_post(ms_args a) {
         c[0] = 0;
         c[1] = 0;
         c[2] = 0;
+        c[kMsSegCount] = 0;
+        c[kMsOvfCount] = 0;
     }
     if (st.push) {
         ms_commit(a);
@@ -439,6 +531,17 @@ int ms_buffers(bd_ctx *c, int nrows) {
         if (!c->d_ms_bits[i] && (rc = dev_alloc(c, (void **)&c->d_ms_bits[i],
                                                 sizeof(uint64_t) * c->nwords, "frontier bits")))
             return rc;
+    if (!c->d_ms_slices[0]) {
+        // slices of one frontier: ceil(deg / kMsSlice) per vertex of degree
+        // >= kMsWaveDeg, so at most nnz / kMsSlice + nnz / kMsWaveDeg
+        c->ms_max_slices = c->nnz / kMsSlice + c->nnz / kMsWaveDeg + 1;
+        c->ms_seg_cap = 2 * ((c->ms_max_slices + kSlots - 1) / kSlots) + 4096;
+        for (int i = 0; i < 2; ++i)
+            if ((rc = dev_alloc(c, (void **)&c->d_ms_slices[i],
+                                sizeof(ms_slice) * (kSlots * c->ms_seg_cap + c->ms_max_slices),
+                                "push slices")))
+                return rc;
+    }
     if (nrows > c->ms_depth_rows) {
         dev_free(c->d_ms_depth);
         c->d_ms_depth = nullptr;
@@ -506,6 +609,10 @@ int run_multi_root(bd_ctx *c, const int64_t *roots, int n, int32_t *depths, doub
     a.ctl = (ms_ctl *)(a.cnt + 3 * kSlots * kLine);
     a.log = c->d_log;
     a.log_cap = c->log_cap;
+    a.slices[0] = (ms_slice *)c->d_ms_slices[0];
+    a.slices[1] = (ms_slice *)c->d_ms_slices[1];
+    a.seg_cap = c->ms_seg_cap;
+    a.max_slices = c->ms_max_slices;
     a.nv = nv;
     a.full = n == 64 ? ~0ULL : ((1ULL << n) - 1);
     a.alpha = (uint64_t)std::max(1, c->tune.ms_alpha);

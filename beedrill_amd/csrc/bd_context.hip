@@ -68,6 +68,9 @@ static void free_search(bd_ctx *c) {
     for (auto &q : c->d_q) dev_free(q);
     for (auto &t : c->d_tasks) dev_free(t);
     ms_free_graph(c);
+    for (auto &b : c->d_ms_slices) dev_free(b);
+    for (auto &b : c->d_ms_slices) b = nullptr;
+    c->ms_max_slices = c->ms_seg_cap = 0;
     for (auto &b : c->d_ms) dev_free(b);
     for (auto &b : c->d_ms_bits) dev_free(b);
     dev_free(c->d_ms_depth);

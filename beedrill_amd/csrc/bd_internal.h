@@ -183,6 +183,8 @@ struct bd_ctx {
     uint64_t *d_ms_bits[2] = {nullptr, nullptr};              // frontier bitmaps
     int32_t *d_ms_depth = nullptr;
     int64_t ms_depth_rows = 0;
+    void *d_ms_slices[2] = {nullptr, nullptr};  // long frontier lists as push slices
+    int64_t ms_max_slices = 0, ms_seg_cap = 0;
     void *ms_graph = nullptr;      // hipGraphExec_t of the captured level chunk
     std::string ms_graph_key;      // the buffers and sizes it was captured with
 
