@@ -108,6 +108,7 @@ struct step_args {
     td_task *tout;
     int32_t *parent;
     int32_t *cand;         // candidate parents (-1 = none)
+    uint8_t *dirty;        // per visited word: a candidate was written into it
     ctl_block *c;
     seg_counters *seg;
     bd_level_rec *log;
@@ -361,6 +362,7 @@ __device__ __forceinline__ void td_visit(const step_args &a, step_smem<BLOCK> &s
             if (!(w & bit)) {
                 a.cand[u] = src;
                 *mw = w | bit;
+                a.dirty[u >> 6] = 1;  // same value from every writer: plain byte store
             }
         }
         return;
@@ -742,13 +744,22 @@ __device__ __forceinline__ void bu_pending(const step_args &a, step_smem<BLOCK> 
 // it.  Returns the count (wave-uniform); `unv` receives the lane's clear
 // bits.  One pass over the bits instead of a rank search per vertex.
 __device__ __forceinline__ int bu_list_chunk(const step_args &a, uint16_t *list, int64_t w0,
-                                             int CW, uint64_t &vw, uint64_t &unv) {
+                                             int CW, uint64_t &vw, uint64_t &unv,
+                                             bool claim = false) {
     const int lane = lane_id();
     const int64_t wi = w0 + lane;
     const bool own = lane < CW && wi < a.live_nwords;
     vw = own ? a.vis[wi] : ~0ULL;
     if (own) a.clr[wi] = 0;
     unv = ~vw;
+    if (claim && own) {
+        // a claim sweep lists only the words the write half wrote a
+        // candidate into (and resets their flags)
+        if (a.dirty[wi])
+            a.dirty[wi] = 0;
+        else
+            unv = 0;
+    }
     const int cntw = __popcll(unv);
     int pos = wave_incl_scan32(cntw);
     const int total = lane_bcast(pos, 63);
@@ -777,7 +788,7 @@ __device__ __forceinline__ void td_claim_body(const step_args &a, step_smem<BLOC
         const int64_t w0 = chunk * CW;
         s.b_found[wid][lane] = 0;
         uint64_t vw, unv;
-        const int total = bu_list_chunk(a, list, w0, CW, vw, unv);
+        const int total = bu_list_chunk(a, list, w0, CW, vw, unv, true);
         // kBuBatch rounds of 64 vertices at a time: their candidates are
         // loaded together, then the degrees of the claimed ones
         for (int r0 = 0; r0 < total; r0 += 64 * kBuBatch) {
@@ -1476,10 +1487,14 @@ __global__ void k_clear(step_args a, int32_t *__restrict__ parent, int64_t nv,
 // what its write half wrote), so this runs only after a search that did not
 // finish, and after allocation: clearing them with every search would push
 // another V-sized array through the caches just before the timed steps.
-__global__ void k_fill_cand(int32_t *__restrict__ cand, int64_t nv) {
+__global__ void k_fill_cand(int32_t *__restrict__ cand, int64_t nv, uint8_t *__restrict__ dirty,
+                            int64_t nwords) {
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nv;
          i += (int64_t)gridDim.x * blockDim.x)
         cand[i] = -1;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nwords;
+         i += (int64_t)gridDim.x * blockDim.x)
+        dirty[i] = 0;
 }
 
 // Profiling: undo the first direction's step at level L (its counters and
@@ -1755,6 +1770,7 @@ step_args make_args(bd_ctx *c, int32_t level, int64_t alpha, int64_t beta, int p
     a.tout = c->d_tasks[(level + 1) & 1];
     a.parent = c->d_parent;
     a.cand = c->d_cand;
+    a.dirty = c->d_dirty;
     a.c = c->d_ctl;
     a.seg = c->d_seg;
     a.log = c->d_log;
@@ -1914,7 +1930,7 @@ int start_search(bd_ctx *c, int64_t source, int64_t alpha, int64_t beta, int pol
                        c->nwords);
     if (!c->cand_clean)
         hipLaunchKernelGGL(k_fill_cand, dim3(grid_for(c, c->nv, 256)), dim3(256), 0, c->stream,
-                           c->d_cand, c->nv);
+                           c->d_cand, c->nv, c->d_dirty, c->nwords);
     c->cand_clean = false;  // until this search finishes
     c->depth_valid = false;
     BD_HIP(c, hipGetLastError());

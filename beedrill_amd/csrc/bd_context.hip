@@ -41,6 +41,8 @@ static void free_search(bd_ctx *c) {
     dev_free(c->d_depth);
     dev_free(c->d_parent);
     dev_free(c->d_cand);
+    dev_free(c->d_dirty);
+    c->d_dirty = nullptr;
     dev_free(c->d_comp);
     dev_free(c->d_cc_flag);
     dev_free(c->d_cc_tasks);
@@ -122,6 +124,7 @@ int alloc_search(bd_ctx *c) {
     if ((rc = dev_alloc(c, (void **)&c->d_parent, sizeof(int32_t) * nv, "parent"))) return rc;
     if ((rc = dev_alloc(c, (void **)&c->d_cand, sizeof(int32_t) * nv, "candidate parents")))
         return rc;
+    if ((rc = dev_alloc(c, (void **)&c->d_dirty, nw, "candidate words"))) return rc;
     if ((rc = dev_alloc(c, (void **)&c->d_vis, sizeof(uint64_t) * nw, "visited"))) return rc;
     if ((rc = dev_alloc(c, (void **)&c->d_vis_save, sizeof(uint64_t) * nw, "visited copy")))
         return rc;

@@ -131,6 +131,7 @@ struct bd_ctx {
     int32_t *d_depth = nullptr;
     int32_t *d_parent = nullptr;
     int32_t *d_cand = nullptr;    // candidate parents of a remote-write top-down step (-1 = none)
+    uint8_t *d_dirty = nullptr;   // per visited word: a candidate was written into it (0 = none)
     int32_t *d_comp = nullptr;    // connected-component labels (components_)
     int32_t *d_cc_flag = nullptr; // [0] an edge pass changed a label, [1..] counters
     bool have_cc = false;         // d_comp holds a finished run
