@@ -51,6 +51,14 @@ constexpr int kMsChunk = 16;    // levels per captured graph
 // wave would otherwise walk every long list of its 64 vertices, and in the
 // degree-ordered layout the high-degree vertices share the first groups).
 constexpr int kMsSlice = 1024;
+// A pull gives each wave 64 consecutive vertices, so lists of kMsHubDeg
+// entries or more are taken out of it: they are cut into slices once per
+// graph (k_ms_hubs), every wave of the grid takes slices of the vertices
+// still open and ORs what it finds into N[v], and the level's second launch
+// finishes those vertices.  Otherwise the hubs that share a wave's 64
+// vertices (all of them at the front in plain degree order) are walked one
+// after another by that wave.
+constexpr int kMsHubDeg = 512;
 // Slice counts of frontier L sit in its counter copy: line s holds, in
 // word 3, the slices filed in segment s (by waves w with w % kSlots == s),
 // and line 0, word 4, those filed in the overflow segment.
@@ -84,6 +92,9 @@ struct ms_args {
     unsigned long long *cnt;
     ms_ctl *ctl;
     ms_slice *slices[2];  // frontier L's slices in slices[L & 1]
+    const int32_t *hubs;         // vertices with kMsHubDeg or more neighbours
+    const ms_slice *hub_slices;  // their lists in slices of up to kMsSlice entries
+    int64_t nhubs, nhub_slices;
     int64_t seg_cap;      // slices per segment; the overflow segment holds max_slices
     int64_t max_slices;   // most slices one frontier can have
     bd_level_rec *log;
@@ -376,6 +387,21 @@ __device__ __forceinline__ void ms_pull(const ms_args &a) {
     unsigned long long c0 = 0, c1 = 0, c2 = 0;
     const int64_t wave = ((int64_t)blockIdx.x * kMsBlock + threadIdx.x) >> 6;
     const int64_t nwaves = ((int64_t)gridDim.x * kMsBlock) >> 6;
+    // slices of the open hubs; seen[v] of a hub does not change during the
+    // launch, N[v] does (other slices OR into it), so lane 0's copy is used
+    for (int64_t i = wave; i < a.nhub_slices; i += nwaves) {
+        const ms_slice sl = a.hub_slices[i];
+        const uint64_t want = ~a.seen[sl.v] & a.full;
+        if (!want) continue;
+        uint64_t got = (uint64_t)lane_bcast((int64_t)a.N[sl.v], 0);
+        const int64_t end = sl.beg + sl.len;
+        for (int64_t k0 = sl.beg; k0 < end && (got & want) != want; k0 += 64) {
+            const int64_t k = k0 + lane;
+            got |= wave_or(k < end ? ms_frontier(a, ms_col(a, k)) : 0ULL);
+        }
+        if (lane == 0 && (got & want))
+            atomicOr((unsigned long long *)&a.N[sl.v], (unsigned long long)(got & want));
+    }
     for (int64_t base = wave * 64; base < a.nv; base += nwaves * 64) {
         const int64_t v = base + lane;
         const bool valid = v < a.nv;
@@ -389,6 +415,7 @@ __device__ __forceinline__ void ms_pull(const ms_args &a) {
                 e = a.off[v + 1];
             }
         }
+        if (e - b >= kMsHubDeg) want = 0;  // a hub: its slices, then k_ms_post
         uint64_t acc = 0;
         const bool alone = want && e - b < kMsWaveDeg;
         if (alone) {
@@ -418,7 +445,7 @@ __device__ __forceinline__ void ms_pull(const ms_args &a) {
         const uint64_t word = __ballot(nf != 0);
         if (lane == 0 && base < a.nv) a.Nb[base >> 6] = word;
         ms_file_slices(a, a.level + 1, wave, nf != 0, v, b, e);
-        if (valid) {
+        if (valid && e - b < kMsHubDeg) {
             a.N[v] = nf;
             if (nf) {
                 a.seen[v] = s | nf;
@@ -427,6 +454,42 @@ __device__ __forceinline__ void ms_pull(const ms_args &a) {
             }
             if (want & ~nf) c2 += (unsigned long long)(e - b);
         }
+        ms_record(a, v, valid, nf, a.level + 1);
+    }
+    ms_count(a, c0, c1, c2);
+}
+
+// After a pull: the hubs, whose slices ORed the roots found into N[v] (only
+// bits v still waited for), join the next frontier as the pull's own
+// vertices do -- seen, the Nb bit (the pull wrote Nb's words whole, so the
+// bit is ORed in now), the counters, push slices and depths.
+__device__ __forceinline__ void ms_finish_hubs(const ms_args &a) {
+    const int lane = lane_id();
+    unsigned long long c0 = 0, c1 = 0, c2 = 0;
+    const int64_t wave = ((int64_t)blockIdx.x * kMsBlock + threadIdx.x) >> 6;
+    const int64_t nwaves = ((int64_t)gridDim.x * kMsBlock) >> 6;
+    for (int64_t base = wave * 64; base < a.nhubs; base += nwaves * 64) {
+        const bool valid = base + lane < a.nhubs;
+        int64_t v = 0, b = 0, e = 0;
+        uint64_t want = 0, nf = 0;
+        if (valid) {
+            v = a.hubs[base + lane];
+            const uint64_t s = a.seen[v];
+            want = ~s & a.full;
+            if (want) {
+                b = a.off[v];
+                e = a.off[v + 1];
+                nf = a.N[v];
+                if (nf) {
+                    a.seen[v] = s | nf;
+                    atomicOr((unsigned long long *)&a.Nb[v >> 6], 1ULL << (v & 63));
+                    c0 += 1;
+                    c1 += (unsigned long long)(e - b);
+                }
+                if (want & ~nf) c2 += (unsigned long long)(e - b);
+            }
+        }
+        ms_file_slices(a, a.level + 1, wave, nf != 0, v, b, e);
         ms_record(a, v, valid, nf, a.level + 1);
     }
     ms_count(a, c0, c1, c2);
@@ -491,6 +554,7 @@ __global__ __launch_bounds__(kMsBlock) void k_ms_post(ms_args a) {
         for (int64_t w = (int64_t)blockIdx.x * kMsBlock + threadIdx.x; w < nw;
              w += (int64_t)gridDim.x * kMsBlock)
             for (uint64_t m = a.Fb[w]; m; m &= m - 1) a.F[w * 64 + __ffsll((unsigned long long)m) - 1] = 0;
+        ms_finish_hubs(a);
     }
 }
 
@@ -504,6 +568,28 @@ __global__ void k_fill_u64(uint64_t *p, int64_t n) {
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
          i += (int64_t)gridDim.x * blockDim.x)
         p[i] = 0;
+}
+
+// The hubs (lists of kMsHubDeg entries or more) and their lists as slices,
+// in the order atomics hand out places: a pull ORs what the slices find, so
+// the order changes no result.
+__global__ void k_ms_hubs(const int64_t *__restrict__ off, int64_t nv, int32_t *__restrict__ hubs,
+                          ms_slice *__restrict__ sl, unsigned long long *__restrict__ cnt) {
+    for (int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v < nv;
+         v += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t b = off[v], e = off[v + 1];
+        if (e - b < kMsHubDeg) continue;
+        hubs[atomicAdd(&cnt[0], 1ULL)] = (int32_t)v;
+        const int64_t ns = (e - b + kMsSlice - 1) / kMsSlice;
+        const int64_t p = (int64_t)atomicAdd(&cnt[1], (unsigned long long)ns);
+        for (int64_t k = 0; k < ns; ++k) {
+            ms_slice x;
+            x.beg = b + k * kMsSlice;
+            x.len = (int32_t)std::min<int64_t>(kMsSlice, e - x.beg);
+            x.v = (int32_t)v;
+            sl[p + k] = x;
+        }
+    }
 }
 
 // Row r of the depths in graph ids: out[v] = depth[r * nv + inv[v]].
@@ -541,6 +627,30 @@ int ms_buffers(bd_ctx *c, int nrows) {
                                 sizeof(ms_slice) * (kSlots * c->ms_seg_cap + c->ms_max_slices),
                                 "push slices")))
                 return rc;
+    }
+    if (!c->d_ms_hubs || c->ms_hub_order != c->s_order) {
+        // the hubs of the search layout, once per graph and layout
+        dev_free(c->d_ms_hubs);
+        dev_free(c->d_ms_hub_slices);
+        c->d_ms_hubs = nullptr;
+        c->d_ms_hub_slices = nullptr;
+        const int64_t max_hubs = c->nnz / kMsHubDeg + 1;
+        const int64_t max_sl = c->nnz / kMsSlice + max_hubs;
+        if ((rc = dev_alloc(c, (void **)&c->d_ms_hubs, sizeof(int32_t) * max_hubs, "hubs")))
+            return rc;
+        if ((rc = dev_alloc(c, &c->d_ms_hub_slices, sizeof(ms_slice) * max_sl, "hub slices")))
+            return rc;
+        unsigned long long *cnt = (unsigned long long *)c->d_ms[3];
+        BD_HIP(c, hipMemsetAsync(cnt, 0, 2 * sizeof(unsigned long long), c->stream));
+        hipLaunchKernelGGL(k_ms_hubs, dim3(grid_for(c, c->nv, 256)), dim3(256), 0, c->stream,
+                           c->d_soff, c->nv, c->d_ms_hubs, (ms_slice *)c->d_ms_hub_slices, cnt);
+        BD_HIP(c, hipGetLastError());
+        unsigned long long h[2] = {0, 0};
+        BD_HIP(c, hipMemcpyAsync(h, cnt, sizeof(h), hipMemcpyDeviceToHost, c->stream));
+        BD_HIP(c, hipStreamSynchronize(c->stream));
+        c->ms_nhubs = (int64_t)h[0];
+        c->ms_nhub_slices = (int64_t)h[1];
+        c->ms_hub_order = c->s_order;
     }
     if (nrows > c->ms_depth_rows) {
         dev_free(c->d_ms_depth);
@@ -613,6 +723,10 @@ int run_multi_root(bd_ctx *c, const int64_t *roots, int n, int32_t *depths, doub
     a.slices[1] = (ms_slice *)c->d_ms_slices[1];
     a.seg_cap = c->ms_seg_cap;
     a.max_slices = c->ms_max_slices;
+    a.hubs = c->d_ms_hubs;
+    a.hub_slices = (const ms_slice *)c->d_ms_hub_slices;
+    a.nhubs = c->ms_nhubs;
+    a.nhub_slices = c->ms_nhub_slices;
     a.nv = nv;
     a.full = n == 64 ? ~0ULL : ((1ULL << n) - 1);
     a.alpha = (uint64_t)std::max(1, c->tune.ms_alpha);
@@ -643,8 +757,9 @@ int run_multi_root(bd_ctx *c, const int64_t *roots, int n, int32_t *depths, doub
     // 2): the same launches, enqueued and waited for level by level.
     const bool host_driven = c->tune.level_driver == 2;
     char key[160];
-    snprintf(key, sizeof(key), "%p %p %p %ld %d %lu %ld", (void *)a.off, (void *)a.depth,
-             (void *)a.cnt, (long)nv, n, (unsigned long)a.alpha, (long)grid);
+    snprintf(key, sizeof(key), "%p %p %p %p %ld %ld %d %lu %ld", (void *)a.off, (void *)a.depth,
+             (void *)a.cnt, (const void *)a.hub_slices, (long)a.nhub_slices, (long)nv, n,
+             (unsigned long)a.alpha, (long)grid);
     if (!host_driven && (!c->ms_graph || c->ms_graph_key != key)) {
         ms_free_graph(c);
         hipGraph_t graph = nullptr;

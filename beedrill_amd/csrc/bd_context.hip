@@ -73,6 +73,12 @@ static void free_search(bd_ctx *c) {
     for (auto &b : c->d_ms_slices) dev_free(b);
     for (auto &b : c->d_ms_slices) b = nullptr;
     c->ms_max_slices = c->ms_seg_cap = 0;
+    dev_free(c->d_ms_hubs);
+    dev_free(c->d_ms_hub_slices);
+    c->d_ms_hubs = nullptr;
+    c->d_ms_hub_slices = nullptr;
+    c->ms_nhubs = c->ms_nhub_slices = 0;
+    c->ms_hub_order = 0;
     for (auto &b : c->d_ms) dev_free(b);
     for (auto &b : c->d_ms_bits) dev_free(b);
     dev_free(c->d_ms_depth);
@@ -177,20 +183,20 @@ extern "C" int bd_create(bd_ctx **out, int device) {
     c->tune.block = 128;
     c->tune.grid_per_cu = 10;
     c->tune.bu_words = 16;
-    c->tune.bu_cap = 32;
+    c->tune.bu_cap = 64;
     c->tune.wave_threshold = 256;
     c->tune.task_threshold = 16;
-    c->tune.task_edges = 768;
+    c->tune.task_edges = 1536;
     c->tune.level_driver = 1;
     c->tune.time_steps = 2;
     c->tune.bu_head = 1;
     c->tune.step_kernels = 2;
-    c->tune.bu_grid_per_cu = 8;
+    c->tune.bu_grid_per_cu = 4;
     c->tune.learned_margin_milli = 250;
     c->tune.ms_alpha = 4;
     c->tune.learned_min_skew = 8;
     c->tune.td_split_edges = 1 << 16;
-    c->tune.vertex_order = 1;
+    c->tune.vertex_order = 4;
     c->tune.bu_batch = 4;
     hipError_t e = hipSetDevice(device);
     if (e == hipSuccess) {

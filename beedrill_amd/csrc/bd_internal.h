@@ -186,6 +186,10 @@ struct bd_ctx {
     int64_t ms_depth_rows = 0;
     void *d_ms_slices[2] = {nullptr, nullptr};  // long frontier lists as push slices
     int64_t ms_max_slices = 0, ms_seg_cap = 0;
+    int32_t *d_ms_hubs = nullptr;      // search ids of the vertices a pull cuts into slices
+    void *d_ms_hub_slices = nullptr;   // their lists as slices
+    int64_t ms_nhubs = 0, ms_nhub_slices = 0;
+    int ms_hub_order = 0;              // the layout (s_order) they were listed in
     void *ms_graph = nullptr;      // hipGraphExec_t of the captured level chunk
     std::string ms_graph_key;      // the buffers and sizes it was captured with
 

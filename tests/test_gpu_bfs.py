@@ -425,6 +425,33 @@ def test_multi_root_s16_and_lattice(ctx, orc):
         np.testing.assert_array_equal(dp[r], orc.bfs_depth(off, adj, int(root)))
 
 
+def test_multi_root_every_layout_with_hubs(ctx, orc):
+    """The batch on every search layout, on a graph with hubs (lists of 512
+    entries or more, which a pull cuts into slices scanned by the whole grid
+    and finishes in its second launch): every depth row is the oracle's, and
+    the level records agree across layouts."""
+    ctx.generate_rmat(16, 16, seed=3)
+    off, adj = ctx.csr()
+    assert int(np.diff(off).max()) >= 4 * 512
+    roots = ctx.pick_roots(64)
+    ref = [orc.bfs_depth(off, adj, int(r)) for r in roots]
+    t0 = ctx.get_tuning()
+    recs = None
+    try:
+        for order in (1, 2, 3, 4):
+            ctx.set_tuning(vertex_order=order)
+            dp, _, levels = ctx.multi_root(roots)
+            for r in range(64):
+                np.testing.assert_array_equal(dp[r], ref[r], err_msg=f"order {order} row {r}")
+            rec = [(r["dir"], r["frontier"], r["frontier_deg"], r["next"]) for r in ctx.levels()]
+            assert any(d == 1 for d, *_ in rec), "no pull level"
+            if recs is None:
+                recs = rec
+            assert rec == recs, order
+    finally:
+        ctx.set_tuning(**t0)
+
+
 def test_multi_root_graph_replays_match_host_driven(ctx, orc):
     """The level launches captured once in a HIP graph and replayed, against
     the same launches enqueued level by level from the host: the same depth
