@@ -307,8 +307,8 @@ typedef struct {
                                 vertices last, the 4096 largest one per
                                 64-bit visited word; 2 = the graph's own
                                 ids (no copy); 3 = vertices with edges
-                                first, in graph order; 4 = degree order
-                                without the spreading                     */
+                                first, in graph order; 4 (default) =
+                                degree order without the spreading        */
     int32_t bu_batch;        /* bottom-up: 1 = one round of 64 vertices per
                                 wave at a time (the next round's head
                                 entries prefetched); 4 = four rounds' head
