@@ -68,7 +68,7 @@ class RunStats(ctypes.Structure):
 TUNING_FIELDS = ("block", "grid_per_cu", "wave_threshold", "task_threshold", "task_edges",
                  "bu_words", "bu_cap", "level_driver", "time_steps", "bu_head", "step_kernels",
                  "bu_grid_per_cu", "learned_margin_milli", "ms_alpha", "learned_min_skew",
-                 "td_split_edges", "vertex_order", "bu_batch")
+                 "td_split_edges", "vertex_order", "bu_batch", "learned_near")
 
 
 class Tuning(ctypes.Structure):

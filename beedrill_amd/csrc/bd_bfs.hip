@@ -132,6 +132,7 @@ struct step_args {
     int32_t policy, script_len, log_cap;
     int32_t mlp_h1, mlp_h2, has_mlp;
     float mlp_margin;      // log2-cost margin for the network to override Beamer
+    int64_t mlp_near;      // also consulted at top-down steps this close to Beamer's switch
 };
 
 // ----------------------------------------------------------------- helpers
@@ -1050,11 +1051,18 @@ __device__ __forceinline__ void choose_dir(const step_args &a, step_smem<BLOCK> 
     // the network runs only for LEARNED with a network loaded (kernel
     // arguments, the same in every lane), and only where Beamer's rules
     // switch direction (hybrid_bfs.cc:167 turning bottom-up, :178-179
-    // leaving it): it decides whether to switch now or to stay.  Elsewhere
+    // leaving it): it decides whether to switch now or to stay; and at a
+    // top-down step Beamer keeps top-down whose frontier edges come within
+    // a factor mlp_near of the switch (scout * alpha * near > the unvisited
+    // edges before this step's charge): whether to switch early.  Elsewhere
     // Beamer's choice stands without a network evaluation, whose latency
     // every step would otherwise pay.
     if (a.policy == BD_POLICY_LEARNED && a.has_mlp) wave_lds_sync();
-    if (a.policy == BD_POLICY_LEARNED && a.has_mlp && last_dir >= 0 && c->dir != last_dir) {
+    if (a.policy == BD_POLICY_LEARNED && a.has_mlp && last_dir >= 0 &&
+        (c->dir != last_dir ||
+         (a.mlp_near > 1 && last_dir == BD_DIR_TD && c->dir == BD_DIR_TD &&
+          gt_quotient(c->scout, c->edges_to_check + c->scout,
+                      c->alpha > (1LL << 46) ? (1LL << 62) : c->alpha * a.mlp_near)))) {
         const int l = lane_id();
         if (l < kMlpIn)
             s.x[l] = mlp_feature_k(l, c->frontier, c->frontier_deg, c->max_frontier_deg,
@@ -1815,6 +1823,7 @@ step_args make_args(bd_ctx *c, int32_t level, int64_t alpha, int64_t beta, int p
                         (double)std::max(1, c->tune.learned_min_skew) * (double)c->nnz;
     a.has_mlp = (c->has_mlp && skewed) ? 1 : 0;
     a.mlp_margin = (float)c->tune.learned_margin_milli * 1e-3f;
+    a.mlp_near = std::max(1, c->tune.learned_near);
     return a;
 }
 

@@ -313,6 +313,13 @@ typedef struct {
                                 wave at a time (the next round's head
                                 entries prefetched); 4 = four rounds' head
                                 entries loaded together                   */
+    int32_t learned_near;    /* LEARNED: besides Beamer's switch points, the
+                                network is consulted at a top-down step
+                                Beamer keeps top-down when the frontier's
+                                edges * alpha * learned_near exceed the
+                                unvisited edges (an early switch to
+                                bottom-up); 1 = switch points only;
+                                default 4                                 */
 } bd_tuning;
 int bd_set_tuning(bd_ctx *ctx, const bd_tuning *t);
 int bd_get_tuning(bd_ctx *ctx, bd_tuning *t);

@@ -560,8 +560,10 @@ def test_learned_decisions_follow_the_network(ctx, orc, s16):
     network in fp32 on the host prefers it from the step records' features
     (steps where the two predictions are within bf16 rounding are skipped).
     (2) A network that always prefers bottom-up: Beamer's directions up to
-    its first bottom-up step, bottom-up from there on.  (3) One that always
-    prefers top-down: top-down throughout."""
+    its first bottom-up step, bottom-up from there on; with learned_near 4 up
+    to the first top-down step whose frontier edges * alpha * 4 exceed the
+    unvisited edges, if that comes earlier.  (3) One that always prefers
+    top-down: top-down throughout."""
     import os
     ctx.generate_rmat(16, 16)
     off, adj, m = s16
@@ -596,18 +598,29 @@ def test_learned_decisions_follow_the_network(ctx, orc, s16):
             # output biases: log2 cost (top-down, bottom-up)
             fixed[nparams - n_out:] = [10.0, -10.0] if want == bd.DIR_BU else [-10.0, 10.0]
             ctx.mlp_set(fixed, 32, 32)
-            for root in roots:
-                ctx.bfs(root, 15, 18, "beamer")
-                beamer = [r["dir"] for r in ctx.levels()]
-                st = ctx.bfs(root, 15, 18, "learned")
-                dirs = [r["dir"] for r in ctx.levels()]
-                assert len(dirs) == len(beamer)
-                if want == bd.DIR_BU:
-                    k = beamer.index(bd.DIR_BU) if bd.DIR_BU in beamer else len(beamer)
-                    assert dirs == beamer[:k] + [bd.DIR_BU] * (len(beamer) - k), (root, dirs)
-                else:
-                    assert dirs == [bd.DIR_TD] * len(beamer), (root, dirs)
-                assert ctx.check(root) and st["reached"] == 1 + sum(r["next"] for r in ctx.levels())
+            for near in (1, 4):
+                ctx.set_tuning(learned_near=near)
+                for root in roots:
+                    ctx.bfs(root, 15, 18, "beamer")
+                    blv = ctx.levels()
+                    beamer = [r["dir"] for r in blv]
+                    st = ctx.bfs(root, 15, 18, "learned")
+                    dirs = [r["dir"] for r in ctx.levels()]
+                    assert len(dirs) == len(beamer)
+                    if want == bd.DIR_BU:
+                        k = beamer.index(bd.DIR_BU) if bd.DIR_BU in beamer else len(beamer)
+                        for i in range(1, k):
+                            # Beamer's counters before step i's charge
+                            scout = blv[i]["frontier_deg"]
+                            if scout * 15 * near > blv[i]["edges_to_check"] + scout:
+                                k = i
+                                break
+                        assert dirs == beamer[:k] + [bd.DIR_BU] * (len(beamer) - k), \
+                            (root, near, dirs)
+                    else:
+                        assert dirs == [bd.DIR_TD] * len(beamer), (root, dirs)
+                    assert ctx.check(root)
+                    assert st["reached"] == 1 + sum(r["next"] for r in ctx.levels())
     finally:
         ctx.set_tuning(**t0)
         ctx.mlp_set(params, 32, 32)
@@ -625,7 +638,8 @@ def test_learned_policy_is_beamer_on_graphs_without_hubs(ctx, orc):
     ctx.mlp_set(params, 32, 32)
     t0 = ctx.get_tuning()
     try:
-        ctx.set_tuning(learned_margin_milli=1)
+        # switch points only, so the expected directions follow from Beamer's
+        ctx.set_tuning(learned_margin_milli=1, learned_near=1)
         ctx.generate_lattice(30, 40, seed=3)
         off, adj = ctx.csr()
         for root in (0, 611):
