@@ -109,6 +109,10 @@ struct step_args {
     int32_t *parent;
     int32_t *cand;         // candidate parents (-1 = none)
     uint8_t *dirty;        // per visited word: a candidate was written into it
+    const uint64_t *hub_bm;     // bit v: deg(v) >= kHubDeg
+    const int32_t *hub_med;     // hubs under kHubBigDeg neighbours
+    const td_task *hub_big;     // the other hubs' lists in slices
+    int64_t n_hub_med, n_hub_big;
     ctl_block *c;
     seg_counters *seg;
     bd_level_rec *log;
@@ -607,15 +611,22 @@ __device__ __forceinline__ void td_body(const step_args &a, step_smem<BLOCK> &s,
 // Top-down step whose frontier is only in the depth-L bitmap (the previous
 // step was bottom-up, which does not build lists).  Each workgroup takes
 // ranges of BLOCK frontier words, ranks their set bits with a block-wide
-// scan, and feeds them to td_tile BLOCK vertices at a time.
+// scan, and feeds them to td_tile BLOCK vertices at a time.  When the
+// frontier may hold many hubs (its largest degree is at least kHubDeg and
+// it has kHubPassEdges edges or more), the ranges skip them and the hubs are
+// spread over the grid: 64 medium hubs checked per wave, the big ones as
+// slices, one per wave.  In degree order the hubs share the first ranges,
+// whose workgroup would otherwise walk all their lists.
 template <int BLOCK>
 __device__ __forceinline__ void td_bitmap_body(const step_args &a, step_smem<BLOCK> &s, bool tdw, unsigned long long &cnt,
                                unsigned long long &dsum, unsigned long long &dmax) {
     const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+    const bool hubs = s.st.max_frontier_deg >= kHubDeg && s.st.frontier_deg >= kHubPassEdges;
     const int64_t nranges = (a.live_nwords + BLOCK - 1) / BLOCK;
     for (int64_t r = blockIdx.x; r < nranges; r += gridDim.x) {
         const int64_t wi = r * BLOCK + tid;
-        const uint64_t w = wi < a.live_nwords ? a.front[wi] : 0ULL;
+        uint64_t w = wi < a.live_nwords ? a.front[wi] : 0ULL;
+        if (hubs && w) w &= ~a.hub_bm[wi];
         const int32_t c = __popcll(w);
         s.w_bits[tid] = w;
         const int32_t incl = wave_incl_scan32(c);
@@ -645,6 +656,58 @@ __device__ __forceinline__ void td_bitmap_body(const step_args &a, step_smem<BLO
             td_tile<BLOCK>(a, s, tdw, v, cnt, dsum, dmax);
         }
         __syncthreads();
+    }
+    if (!hubs) return;
+    const int64_t nmed = (a.n_hub_med + 63) / 64;
+    const int64_t nh = nmed + (s.st.max_frontier_deg >= kHubBigDeg ? a.n_hub_big : 0);
+    const int64_t waves = (int64_t)gridDim.x * (BLOCK / 64);
+    for (int64_t t0 = (int64_t)blockIdx.x * (BLOCK / 64); t0 < nh; t0 += waves) {
+        const int64_t t = t0 + wid;
+        if (t < nmed) {
+            const int64_t i = t * 64 + lane;
+            const int32_t v = i < a.n_hub_med ? a.hub_med[i] : -1;
+            const bool in_f = v >= 0 && ((a.front[v >> 6] >> (v & 63)) & 1);
+            int64_t b = 0, e = 0;
+            if (in_f) {
+                b = a.off[v];
+                e = a.off[v + 1];
+            }
+            for (uint64_t m = __ballot(in_f); m; m &= m - 1) {
+                const int j = __ffsll((unsigned long long)m) - 1;
+                const int64_t hb = lane_bcast(b, j), he = lane_bcast(e, j);
+                const int32_t src = lane_bcast(v, j);
+                for (int64_t k0 = hb; k0 < he; k0 += 64 * kTdRounds) {
+                    bool valid[kTdRounds];
+                    int32_t u[kTdRounds], sr[kTdRounds];
+#pragma unroll
+                    for (int r = 0; r < kTdRounds; ++r) {
+                        const int64_t k = k0 + 64 * r + lane;
+                        valid[r] = k < he;
+                        u[r] = valid[r] ? __builtin_nontemporal_load(a.cols + k) : 0;
+                        sr[r] = src;
+                    }
+                    td_visit_n(a, s, tdw, valid, u, sr, cnt, dsum, dmax);
+                }
+            }
+        } else if (t < nh) {
+            const td_task tk = a.hub_big[t - nmed];
+            if ((a.front[tk.src >> 6] >> (tk.src & 63)) & 1) {
+                for (int64_t k0 = tk.beg; k0 < tk.end; k0 += 64 * kTdRounds) {
+                    bool valid[kTdRounds];
+                    int32_t u[kTdRounds], sr[kTdRounds];
+#pragma unroll
+                    for (int r = 0; r < kTdRounds; ++r) {
+                        const int64_t k = k0 + 64 * r + lane;
+                        valid[r] = k < tk.end;
+                        u[r] = valid[r] ? __builtin_nontemporal_load(a.cols + k) : 0;
+                        sr[r] = tk.src;
+                    }
+                    td_visit_n(a, s, tdw, valid, u, sr, cnt, dsum, dmax);
+                }
+            }
+        }
+        __syncthreads();
+        if (s.qn >= kQCap / 2) flush_queue<BLOCK>(a, s);
     }
 }
 
@@ -1779,6 +1842,11 @@ step_args make_args(bd_ctx *c, int32_t level, int64_t alpha, int64_t beta, int p
     a.parent = c->d_parent;
     a.cand = c->d_cand;
     a.dirty = c->d_dirty;
+    a.hub_bm = c->d_hub_bm;
+    a.hub_med = c->d_hub_med;
+    a.hub_big = c->d_hub_big;
+    a.n_hub_med = c->n_hub_med;
+    a.n_hub_big = c->n_hub_big;
     a.c = c->d_ctl;
     a.seg = c->d_seg;
     a.log = c->d_log;

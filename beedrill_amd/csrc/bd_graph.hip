@@ -86,6 +86,37 @@ __global__ void k_dead_bitmap(const int64_t *__restrict__ off, int64_t nv, int64
     }
 }
 
+// The hubs of a layout: bit v of bm set when deg(v) >= kHubDeg; those under
+// kHubBigDeg listed in med, the others' lists cut into slices of kHubSlice
+// in big (places handed out by atomics: the order is not a result).
+__global__ void k_hub_lists(const int64_t *__restrict__ off, int64_t nv, int64_t nwords,
+                            uint64_t *__restrict__ bm, int32_t *__restrict__ med,
+                            td_task *__restrict__ big, unsigned long long *__restrict__ cnt) {
+    const int64_t total = nwords * 64;
+    for (int64_t v = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; v < total;
+         v += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t b = v < nv ? off[v] : 0, e = v < nv ? off[v + 1] : 0;
+        const bool hub = e - b >= kHubDeg;
+        const uint64_t mask = __ballot(hub);
+        if ((threadIdx.x & 63) == 0) bm[v >> 6] = mask;
+        if (!hub) continue;
+        if (e - b < kHubBigDeg) {
+            med[atomicAdd(&cnt[0], 1ULL)] = (int32_t)v;
+        } else {
+            const int64_t ns = (e - b + kHubSlice - 1) / kHubSlice;
+            const int64_t p = (int64_t)atomicAdd(&cnt[1], (unsigned long long)ns);
+            for (int64_t k = 0; k < ns; ++k) {
+                td_task t;
+                t.beg = b + k * kHubSlice;
+                t.end = std::min<int64_t>(t.beg + kHubSlice, e);
+                t.src = (int32_t)v;
+                t.pad = 0;
+                big[p + k] = t;
+            }
+        }
+    }
+}
+
 __global__ void k_validate_csr(const int64_t *__restrict__ off, const int32_t *__restrict__ cols,
                                int64_t nv, int64_t nnz, int *__restrict__ err) {
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < nv;
@@ -450,6 +481,13 @@ void free_search_layout(bd_ctx *c) {
     dev_free(c->d_perm);
     dev_free(c->d_inv);
     dev_free(c->d_head);
+    dev_free(c->d_hub_bm);
+    dev_free(c->d_hub_med);
+    dev_free(c->d_hub_big);
+    c->d_hub_bm = nullptr;
+    c->d_hub_med = nullptr;
+    c->d_hub_big = nullptr;
+    c->n_hub_med = c->n_hub_big = 0;
     c->d_soff = nullptr;
     c->d_scols = nullptr;
     c->d_sdead = nullptr;
@@ -458,6 +496,32 @@ void free_search_layout(bd_ctx *c) {
     c->d_head = nullptr;
     c->nv_active = 0;
     c->s_order = 0;
+}
+
+// The hub lists of the layout just built (d_soff).
+static int build_hubs(bd_ctx *c) {
+    const int64_t nnz = c->nnz;
+    int rc;
+    dev_buf cnt;
+    if ((rc = dev_alloc(c, &cnt.p, 2 * sizeof(unsigned long long), "hub counts"))) return rc;
+    if ((rc = dev_alloc(c, (void **)&c->d_hub_bm, sizeof(uint64_t) * c->nwords, "hub bitmap")))
+        return rc;
+    const int64_t max_med = nnz / kHubDeg + 1;
+    const int64_t max_big = nnz / kHubSlice + nnz / kHubBigDeg + 1;
+    if ((rc = dev_alloc(c, (void **)&c->d_hub_med, sizeof(int32_t) * max_med, "hubs"))) return rc;
+    if ((rc = dev_alloc(c, (void **)&c->d_hub_big, sizeof(td_task) * max_big, "hub slices")))
+        return rc;
+    BD_HIP(c, hipMemsetAsync(cnt.p, 0, 2 * sizeof(unsigned long long), c->stream));
+    hipLaunchKernelGGL(k_hub_lists, dim3(grid1d(c, c->nwords * 64)), dim3(kBlock), 0, c->stream,
+                       c->d_soff, c->nv, c->nwords, c->d_hub_bm, c->d_hub_med, c->d_hub_big,
+                       cnt.as<unsigned long long>());
+    BD_HIP(c, hipGetLastError());
+    unsigned long long h[2] = {0, 0};
+    BD_HIP(c, hipMemcpyAsync(h, cnt.p, sizeof(h), hipMemcpyDeviceToHost, c->stream));
+    BD_HIP(c, hipStreamSynchronize(c->stream));
+    c->n_hub_med = (int64_t)h[0];
+    c->n_hub_big = (int64_t)h[1];
+    return BD_OK;
 }
 
 int build_search_layout(bd_ctx *c) {
@@ -484,7 +548,7 @@ int build_search_layout(bd_ctx *c) {
         c->d_scols = c->d_cols;
         c->d_sdead = c->d_dead;
         c->s_order = order;
-        return BD_OK;
+        return build_hubs(c);
     }
     if ((rc = dev_alloc(c, &skeys.p, sizeof(uint64_t) * nv, "sorted order keys"))) return rc;
     if ((rc = sort_keys_u64(c, keys.as<uint64_t>(), skeys.as<uint64_t>(), nv, 32 + sh))) return rc;
@@ -519,7 +583,7 @@ int build_search_layout(bd_ctx *c) {
     BD_HIP(c, hipGetLastError());
     BD_HIP(c, hipStreamSynchronize(c->stream));
     c->s_order = order;
-    return BD_OK;
+    return build_hubs(c);
 }
 
 }  // namespace bd

@@ -23,6 +23,15 @@ struct td_task {
     int32_t pad;
 };
 
+// Hubs of the search layout: lists of at least kHubDeg entries; those of
+// kHubBigDeg or more are cut into slices of kHubSlice entries.
+constexpr int64_t kHubDeg = 256;
+constexpr int64_t kHubBigDeg = 8192;
+constexpr int64_t kHubSlice = 4096;
+// A top-down step over a bitmap frontier spreads the hubs over the grid only
+// when the frontier has this many edges (below it, the hubs it holds are few).
+constexpr int64_t kHubPassEdges = 1 << 20;
+
 // Frontier lists (queue of ordinary vertices, task slices of high-degree
 // vertices) are segmented: workgroup b appends to segment b % kSeg, so no
 // address is shared by more than a few dozen workgroups -- on MI355X a
@@ -125,6 +134,12 @@ struct bd_ctx {
     int32_t *d_perm = nullptr;
     int32_t *d_inv = nullptr;
     int64_t nv_active = 0;        // vertices with edges
+    // hubs of the search layout (lists of kHubDeg entries or more), which a
+    // top-down step over a bitmap frontier spreads over the whole grid
+    uint64_t *d_hub_bm = nullptr;  // bit v: v is a hub
+    int32_t *d_hub_med = nullptr;  // hubs with fewer than kHubBigDeg neighbours
+    bd::td_task *d_hub_big = nullptr;  // the others' lists, in slices of kHubSlice
+    int64_t n_hub_med = 0, n_hub_big = 0;
     int s_order = 0;              // vertex_order the layout was built with (0: none yet)
 
     // search state

@@ -166,6 +166,39 @@ def test_s16_scripted_directions_same_depth(ctx, orc, s16):
         run_and_compare(ctx, orc, off, adj, m, int(root), policy="script", levels=False)
 
 
+def test_bitmap_top_down_over_hubs_every_layout(ctx, orc):
+    """A top-down step right after a bottom-up one reads its frontier from
+    the bitmap; when that frontier holds hubs (lists of 256 or more) and has
+    2^20 edges or more, the hubs are taken off the workgroups' ranges and
+    spread over the grid.  Scripted directions force such steps at levels 2
+    and 3, with claims and with remote writes, on every search layout (RMAT
+    scale 17): depths are the oracle's and the parent trees pass check()."""
+    ctx.generate_rmat(17, 16)
+    off, adj = ctx.csr()
+    m = ctx.info()[2]
+    roots = [int(r) for r in ctx.pick_roots(2)]
+    t0 = ctx.get_tuning()
+    hub_steps = 0
+    try:
+        for order in (1, 2, 3, 4):
+            for split in (2147483647, 1):
+                ctx.set_tuning(vertex_order=order, td_split_edges=split)
+                for script in ([0, 1, 0, 0], [0, 0, 1, 0]):
+                    ctx.set_script(np.array(script + [0] * 28, np.int32))
+                    for root in roots:
+                        run_and_compare(ctx, orc, off, adj, m, root, policy="script",
+                                        levels=False)
+                        lv = ctx.levels()
+                        k = script.index(1) + 1
+                        if len(lv) > k and lv[k]["dir"] == bd.DIR_TD and \
+                                lv[k]["max_frontier_deg"] >= 256 and \
+                                lv[k]["frontier_deg"] >= 1 << 20:
+                            hub_steps += 1
+    finally:
+        ctx.set_tuning(**t0)
+    assert hub_steps >= 8
+
+
 def test_s16_host_driven_levels_identical(ctx, orc, s16):
     ctx.generate_rmat(16, 16)
     off, adj, m = s16
