@@ -1790,7 +1790,10 @@ __global__ __launch_bounds__(256) void k_head_index(const int64_t *__restrict__ 
                      : 0xffffffffu;
             head[v] = ((uint64_t)(e - b) << 32) | id;
         }
-        uint64_t lm = __ballot(valid && !alone);
+        // lists of kHubBigDeg or more: k_head_big and k_head_big_done
+        const bool big = valid && e - b >= kHubBigDeg;
+        if (big) head[v] = 0;
+        uint64_t lm = __ballot(valid && !alone && !big);
         while (lm) {
             const int src = __ffsll((unsigned long long)lm) - 1;
             lm &= lm - 1;
@@ -1808,6 +1811,45 @@ __global__ __launch_bounds__(256) void k_head_index(const int64_t *__restrict__ 
     }
     dmax = wave_max(dmax);
     if (lane == 0 && dmax) atomicMax(max_deg, dmax);
+}
+
+// The lists of kHubBigDeg entries or more, one hub slice per wave: each
+// slice's best key (degree, then first position) is combined into head[v]
+// (cleared by k_head_index) with a 64-bit atomic maximum.
+__global__ __launch_bounds__(256) void k_head_big(const int64_t *__restrict__ off,
+                                                  const int32_t *__restrict__ cols,
+                                                  const td_task *__restrict__ big, int64_t nbig,
+                                                  uint64_t *__restrict__ head) {
+    const int lane = threadIdx.x & 63;
+    const int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const int64_t nwaves = ((int64_t)gridDim.x * blockDim.x) >> 6;
+    for (int64_t t = wave; t < nbig; t += nwaves) {
+        const td_task tk = big[t];
+        const int64_t b = off[tk.src];
+        unsigned long long best = 0;
+        for (int64_t k = tk.beg + lane; k < tk.end; k += 64) {
+            const unsigned long long key = head_key(off, cols[k], k - b);
+            best = key > best ? key : best;
+        }
+        best = wave_max(best);
+        if (lane == 0) atomicMax((unsigned long long *)&head[tk.src], best);
+    }
+}
+
+// Then each such vertex's key becomes its entry: degree and neighbour id
+// (the slice that starts its list does it).
+__global__ void k_head_big_done(const int64_t *__restrict__ off, const int32_t *__restrict__ cols,
+                                const td_task *__restrict__ big, int64_t nbig,
+                                uint64_t *__restrict__ head) {
+    for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < nbig;
+         t += (int64_t)gridDim.x * blockDim.x) {
+        const td_task tk = big[t];
+        const int64_t b = off[tk.src], e = off[tk.src + 1];
+        if (tk.beg != b) continue;
+        const unsigned long long best = head[tk.src];
+        head[tk.src] = ((uint64_t)(e - b) << 32) |
+                       (uint32_t)cols[b + (int64_t)(0xffffffffULL - (best & 0xffffffffULL))];
+    }
 }
 
 // Streams a buffer larger than the last-level cache so a timed step starts
@@ -2079,6 +2121,14 @@ int build_head_index(bd_ctx *c) {
     BD_HIP(c, hipMemsetAsync(d_max, 0, sizeof(unsigned long long), c->stream));
     hipLaunchKernelGGL(k_head_index, dim3(grid_for(c, c->nv, 256)), dim3(256), 0, c->stream,
                        c->d_soff, c->d_scols, c->nv, c->d_head, d_max);
+    if (c->n_hub_big) {
+        hipLaunchKernelGGL(k_head_big, dim3(grid_for(c, c->n_hub_big * 64, 256)), dim3(256), 0,
+                           c->stream, c->d_soff, c->d_scols, c->d_hub_big, c->n_hub_big,
+                           c->d_head);
+        hipLaunchKernelGGL(k_head_big_done, dim3(grid_for(c, c->n_hub_big, 256)), dim3(256), 0,
+                           c->stream, c->d_soff, c->d_scols, c->d_hub_big, c->n_hub_big,
+                           c->d_head);
+    }
     BD_HIP(c, hipGetLastError());
     unsigned long long h = 0;
     BD_HIP(c, hipMemcpyAsync(&h, d_max, sizeof(h), hipMemcpyDeviceToHost, c->stream));

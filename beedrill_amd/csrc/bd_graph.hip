@@ -459,7 +459,8 @@ __global__ __launch_bounds__(256) void k_order_cols(
         }
         if (valid && e - b < 64)
             for (int64_t k = b; k < e; ++k) scols[sb + (k - b)] = inv[cols[k]];
-        uint64_t lm = __ballot(valid && e - b >= 64);
+        // lists of kHubBigDeg or more: k_order_cols_big
+        uint64_t lm = __ballot(valid && e - b >= 64 && e - b < kHubBigDeg);
         while (lm) {
             const int src = __ffsll((unsigned long long)lm) - 1;
             lm &= lm - 1;
@@ -467,6 +468,23 @@ __global__ __launch_bounds__(256) void k_order_cols(
                           so = lane_bcast(sb, src);
             for (int64_t k = kb + lane; k < ke; k += 64) scols[so + (k - kb)] = inv[cols[k]];
         }
+    }
+}
+
+// The lists of kHubBigDeg entries or more, one slice of the layout's hub
+// slices per wave (in degree order they would share a few waves above).
+__global__ __launch_bounds__(256) void k_order_cols_big(
+    const int64_t *__restrict__ off, const int32_t *__restrict__ cols,
+    const int32_t *__restrict__ perm, const int32_t *__restrict__ inv,
+    const int64_t *__restrict__ soff, const td_task *__restrict__ big, int64_t nbig,
+    int32_t *__restrict__ scols) {
+    const int lane = threadIdx.x & 63;
+    const int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const int64_t nwaves = ((int64_t)gridDim.x * blockDim.x) >> 6;
+    for (int64_t t = wave; t < nbig; t += nwaves) {
+        const td_task tk = big[t];
+        const int64_t ob = off[perm[tk.src]] - soff[tk.src];  // old minus new list start
+        for (int64_t k = tk.beg + lane; k < tk.end; k += 64) scols[k] = inv[cols[ob + k]];
     }
 }
 
@@ -576,14 +594,19 @@ int build_search_layout(bd_ctx *c) {
     if ((rc = dev_alloc(c, &tmp.p, tb, "scan temp"))) return rc;
     BD_HIP(c, rocprim::inclusive_scan(tmp.p, tb, sdeg.as<int64_t>(), c->d_soff + 1, (size_t)nv,
                                       rocprim::plus<int64_t>(), c->stream));
+    if ((rc = build_hubs(c))) return rc;
     hipLaunchKernelGGL(k_order_cols, dim3(grid1d(c, nv)), dim3(kBlock), 0, c->stream, c->d_off,
                        c->d_cols, c->d_perm, c->d_inv, c->d_soff, nv, c->d_scols);
+    if (c->n_hub_big)
+        hipLaunchKernelGGL(k_order_cols_big, dim3(grid1d(c, c->n_hub_big * 64)), dim3(kBlock), 0,
+                           c->stream, c->d_off, c->d_cols, c->d_perm, c->d_inv, c->d_soff,
+                           c->d_hub_big, c->n_hub_big, c->d_scols);
     hipLaunchKernelGGL(k_dead_bitmap, dim3(grid1d(c, c->nwords * 64)), dim3(kBlock), 0, c->stream,
                        c->d_soff, nv, c->nwords, c->d_sdead);
     BD_HIP(c, hipGetLastError());
     BD_HIP(c, hipStreamSynchronize(c->stream));
     c->s_order = order;
-    return build_hubs(c);
+    return BD_OK;
 }
 
 }  // namespace bd
