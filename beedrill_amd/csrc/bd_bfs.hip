@@ -346,6 +346,16 @@ __device__ __forceinline__ void flush_queue(const step_args &a, step_smem<BLOCK>
 // send every later edge into a popular vertex to the atomic, and atomics on
 // one word serialize (~12 ns each).  With tdw (the write half of a top-down
 // step with remote writes) the edge only writes a candidate parent.
+// A remote-write step marks the visited words it writes candidates into,
+// and its claim sweep lists only those, when its frontier has fewer than
+// kDirtyWordEdges edges per live visited word.  Heavier steps write into
+// nearly every word, and the marks would only add a scattered store per
+// candidate.  Both launches of the step see the same frontier_deg.
+constexpr int64_t kDirtyWordEdges = 8;
+__device__ __forceinline__ bool use_dirty(const step_args &a, int64_t frontier_deg) {
+    return frontier_deg < kDirtyWordEdges * a.live_nwords;
+}
+
 template <int BLOCK>
 __device__ __forceinline__ void td_visit(const step_args &a, step_smem<BLOCK> &s, bool tdw, bool valid, int32_t u,
                                          int32_t src, unsigned long long &cnt,
@@ -367,7 +377,8 @@ __device__ __forceinline__ void td_visit(const step_args &a, step_smem<BLOCK> &s
             if (!(w & bit)) {
                 a.cand[u] = src;
                 *mw = w | bit;
-                a.dirty[u >> 6] = 1;  // same value from every writer: plain byte store
+                // same value from every writer: a plain byte store
+                if (use_dirty(a, s.st.frontier_deg)) a.dirty[u >> 6] = 1;
             }
         }
         return;
@@ -817,8 +828,8 @@ __device__ __forceinline__ int bu_list_chunk(const step_args &a, uint16_t *list,
     if (own) a.clr[wi] = 0;
     unv = ~vw;
     if (claim && own) {
-        // a claim sweep lists only the words the write half wrote a
-        // candidate into (and resets their flags)
+        // a claim sweep of a light step lists only the words the write half
+        // wrote a candidate into (and resets their flags)
         if (a.dirty[wi])
             a.dirty[wi] = 0;
         else
@@ -852,7 +863,8 @@ __device__ __forceinline__ void td_claim_body(const step_args &a, step_smem<BLOC
         const int64_t w0 = chunk * CW;
         s.b_found[wid][lane] = 0;
         uint64_t vw, unv;
-        const int total = bu_list_chunk(a, list, w0, CW, vw, unv, true);
+        const int total =
+            bu_list_chunk(a, list, w0, CW, vw, unv, use_dirty(a, s.st.frontier_deg));
         // kBuBatch rounds of 64 vertices at a time: their candidates are
         // loaded together, then the degrees of the claimed ones
         for (int r0 = 0; r0 < total; r0 += 64 * kBuBatch) {

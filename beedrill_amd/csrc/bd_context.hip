@@ -183,7 +183,7 @@ extern "C" int bd_create(bd_ctx **out, int device) {
     c->tune.block = 128;
     c->tune.grid_per_cu = 10;
     c->tune.bu_words = 16;
-    c->tune.bu_cap = 64;
+    c->tune.bu_cap = 32;
     c->tune.wave_threshold = 256;
     c->tune.task_threshold = 16;
     c->tune.task_edges = 1536;
