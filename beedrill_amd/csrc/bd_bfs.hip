@@ -992,25 +992,43 @@ __device__ __forceinline__ void bu_body(const step_args &a, step_smem<BLOCK> &s,
         };
         // R rounds of 64 vertices at a time: every round's head entry, then
         // every round's frontier word, are loaded before any is used, so the
-        // wave waits one chain of two loads per 64 * R vertices
+        // wave waits one chain of two loads per 64 * R vertices; the next
+        // batch's head entries are issued before this batch is visited
         auto batched = [&](auto rounds) {
             constexpr int R = decltype(rounds)::value;
-            for (int r0 = 0; r0 < total; r0 += 64 * R) {
-                bool valid[R];
-                int idx[R];
-                uint64_t hd[R], fw[R];
+            bool valid[R];
+            int idx[R];
+            uint64_t hd[R];
 #pragma unroll
-                for (int k = 0; k < R; ++k) {
-                    const int g = r0 + 64 * k + lane;
-                    valid[k] = g < total;
-                    idx[k] = valid[k] ? list[g] : 0;
-                    hd[k] = head_of(valid[k], idx[k]);
-                }
+            for (int k = 0; k < R; ++k) {
+                const int g = 64 * k + lane;
+                valid[k] = g < total;
+                idx[k] = valid[k] ? list[g] : 0;
+                hd[k] = head_of(valid[k], idx[k]);
+            }
+            for (int r0 = 0; r0 < total; r0 += 64 * R) {
+                uint64_t fw[R];
 #pragma unroll
                 for (int k = 0; k < R; ++k) fw[k] = front_of(valid[k], hd[k]);
+                bool validn[R];
+                int idxn[R];
+                uint64_t hdn[R];
+#pragma unroll
+                for (int k = 0; k < R; ++k) {
+                    const int g = r0 + 64 * (R + k) + lane;
+                    validn[k] = g < total;
+                    idxn[k] = validn[k] ? list[g] : 0;
+                    hdn[k] = head_of(validn[k], idxn[k]);
+                }
 #pragma unroll
                 for (int k = 0; k < R; ++k) visit(valid[k], idx[k], hd[k], fw[k]);
                 drain();
+#pragma unroll
+                for (int k = 0; k < R; ++k) {
+                    valid[k] = validn[k];
+                    idx[k] = idxn[k];
+                    hd[k] = hdn[k];
+                }
             }
         };
         if (a.bu_batch > 1) {
