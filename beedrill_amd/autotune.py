@@ -33,7 +33,7 @@ import beedrill_amd as bd  # noqa: E402
 # Values tried per field (all within the ranges bd_set_tuning accepts).
 SPACE = {
     "vertex_order": [1, 2, 3, 4],
-    "bu_batch": [1, 4],
+    "bu_batch": [1, 4, 8],
     "bu_grid_per_cu": [4, 6, 8, 10],
     "bu_words": [4, 8, 16, 24, 32],
     "bu_cap": [8, 16, 32, 64, 128],

@@ -1031,7 +1031,41 @@ __device__ __forceinline__ void bu_body(const step_args &a, step_smem<BLOCK> &s,
                 }
             }
         };
-        if (a.bu_batch > 1) {
+        if (a.bu_batch >= 8) {
+            // 2 * kBuBatch rounds: their neighbour halves of the head entries
+            // (4 bytes each) in one trip, then the degree halves (the same
+            // lines) with the frontier words; the queue is drained after
+            // every kBuBatch rounds, so it never holds more than it did
+            constexpr int R = kBuBatch + 2;
+            const uint32_t *hw = (const uint32_t *)(a.head + w0 * 64);
+            const uint32_t *fh = (const uint32_t *)a.front;  // half words of the frontier
+            for (int r0 = 0; r0 < total; r0 += 64 * R) {
+                bool valid[R];
+                int idx[R];
+                uint32_t lo[R], hi[R], fw[R];
+#pragma unroll
+                for (int k = 0; k < R; ++k) {
+                    const int g = r0 + 64 * k + lane;
+                    valid[k] = g < total;
+                    idx[k] = valid[k] ? list[g] : 0;
+                    lo[k] = (valid[k] && a.bu_head) ? hw[2 * idx[k]] : 0u;
+                }
+#pragma unroll
+                for (int k = 0; k < R; ++k) {
+                    const bool hv = valid[k] && a.bu_head && (int32_t)lo[k] >= 0;
+                    hi[k] = (valid[k] && a.bu_head) ? hw[2 * idx[k] + 1] : 0u;
+                    fw[k] = hv ? fh[(int32_t)lo[k] >> 5] : 0u;
+                }
+#pragma unroll
+                for (int k = 0; k < R; ++k) {
+                    // the half word back in its place in the 64-bit word
+                    const uint64_t f64 = (lo[k] & 32) ? ((uint64_t)fw[k] << 32) : (uint64_t)fw[k];
+                    visit(valid[k], idx[k], ((uint64_t)hi[k] << 32) | lo[k], f64);
+                    if (k == kBuBatch - 1) drain();
+                }
+                drain();
+            }
+        } else if (a.bu_batch > 1) {
             batched(std::integral_constant<int, kBuBatch>{});
         } else {
             // rounds of 64 vertices, software-pipelined: the next round's

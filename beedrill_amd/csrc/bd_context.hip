@@ -197,7 +197,7 @@ extern "C" int bd_create(bd_ctx **out, int device) {
     c->tune.learned_min_skew = 8;
     c->tune.td_split_edges = 1 << 16;
     c->tune.vertex_order = 4;
-    c->tune.bu_batch = 4;
+    c->tune.bu_batch = 8;
     c->tune.learned_near = 4;
     hipError_t e = hipSetDevice(device);
     if (e == hipSuccess) {
@@ -325,7 +325,7 @@ extern "C" int bd_set_tuning(bd_ctx *c, const bd_tuning *t) {
     if (t->td_split_edges > 0) n.td_split_edges = t->td_split_edges;
     if (t->vertex_order > 4) return set_err(c, BD_EINVAL, "vertex_order must be 1 to 4");
     if (t->vertex_order > 0) n.vertex_order = t->vertex_order;
-    if (t->bu_batch > 0) n.bu_batch = t->bu_batch >= 4 ? 4 : 1;
+    if (t->bu_batch > 0) n.bu_batch = t->bu_batch >= 8 ? 8 : (t->bu_batch >= 4 ? 4 : 1);
     if (t->learned_near > 0) n.learned_near = std::min(t->learned_near, 1 << 16);
     bool realloc = (n.task_edges != c->tune.task_edges || n.task_threshold != c->tune.task_threshold ||
                     n.vertex_order != c->tune.vertex_order);

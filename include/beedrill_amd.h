@@ -312,7 +312,10 @@ typedef struct {
     int32_t bu_batch;        /* bottom-up: 1 = one round of 64 vertices per
                                 wave at a time (the next round's head
                                 entries prefetched); 4 = four rounds' head
-                                entries loaded together                   */
+                                entries loaded together, the next four in
+                                flight meanwhile; 8 (default) = six rounds'
+                                neighbour halves in one trip, then their
+                                degree halves with the frontier words     */
     int32_t learned_near;    /* LEARNED: besides Beamer's switch points, the
                                 network is consulted at a top-down step
                                 Beamer keeps top-down when the frontier's

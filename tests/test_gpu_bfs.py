@@ -219,6 +219,12 @@ def test_s16_host_driven_levels_identical(ctx, orc, s16):
         # one kernel carrying both directions; odd bottom-up grid
         ctx.set_tuning(bu_head=1, step_kernels=2)
         run_and_compare(ctx, orc, off, adj, m, root)
+        # every bottom-up batching mode, with and without the head index
+        for batch in (1, 4, 8):
+            for head in (1, 2):
+                ctx.set_tuning(bu_batch=batch, bu_head=head)
+                run_and_compare(ctx, orc, off, adj, m, root)
+        ctx.set_tuning(bu_head=1, bu_batch=8)
         ctx.set_tuning(step_kernels=1, bu_grid_per_cu=3, bu_words=5)
         run_and_compare(ctx, orc, off, adj, m, root)
         ctx.set_tuning(level_driver=2)
