@@ -1032,10 +1032,12 @@ __device__ __forceinline__ void bu_body(const step_args &a, step_smem<BLOCK> &s,
             }
         };
         if (a.bu_batch >= 8) {
-            // 2 * kBuBatch rounds: their neighbour halves of the head entries
+            // kBuBatch + 2 rounds: their neighbour halves of the head entries
             // (4 bytes each) in one trip, then the degree halves (the same
             // lines) with the frontier words; the queue is drained after
-            // every kBuBatch rounds, so it never holds more than it did
+            // kBuBatch rounds, so it never holds more than 64 * (kBuBatch + 1)
+            // (six rounds fit in the registers of five waves per SIMD;
+            // eight, with the list entries read again, ran slower)
             constexpr int R = kBuBatch + 2;
             const uint32_t *hw = (const uint32_t *)(a.head + w0 * 64);
             const uint32_t *fh = (const uint32_t *)a.front;  // half words of the frontier
