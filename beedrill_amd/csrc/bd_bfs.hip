@@ -865,28 +865,30 @@ __device__ __forceinline__ void td_claim_body(const step_args &a, step_smem<BLOC
         uint64_t vw, unv;
         const int total =
             bu_list_chunk(a, list, w0, CW, vw, unv, use_dirty(a, s.st.frontier_deg));
-        // kBuBatch rounds of 64 vertices at a time: their candidates are
-        // loaded together, then the degrees of the claimed ones
-        for (int r0 = 0; r0 < total; r0 += 64 * kBuBatch) {
-            int idx[kBuBatch];
-            int32_t p[kBuBatch];
+        // kClaimRounds rounds of 64 vertices at a time: their candidates
+        // are loaded together, then the degrees of the claimed ones (the
+        // degree half of the head entry)
+        constexpr int kClaimRounds = 2 * kBuBatch;
+        const uint32_t *hw = (const uint32_t *)a.head;
+        for (int r0 = 0; r0 < total; r0 += 64 * kClaimRounds) {
+            int idx[kClaimRounds];
+            int32_t p[kClaimRounds];
 #pragma unroll
-            for (int k = 0; k < kBuBatch; ++k) {
+            for (int k = 0; k < kClaimRounds; ++k) {
                 const int g = r0 + 64 * k + lane;
                 idx[k] = g < total ? list[g] : 0;
                 p[k] = g < total ? __builtin_nontemporal_load(a.cand + w0 * 64 + idx[k]) : -1;
             }
-            unsigned long long d[kBuBatch];
+            uint32_t d[kClaimRounds];
 #pragma unroll
-            for (int k = 0; k < kBuBatch; ++k) {
+            for (int k = 0; k < kClaimRounds; ++k) {
                 const int64_t v = w0 * 64 + idx[k];
                 d[k] = 0;
                 if (p[k] >= 0)
-                    d[k] = a.bu_head ? (unsigned long long)(a.head[v] >> 32)
-                                     : (unsigned long long)(a.off[v + 1] - a.off[v]);
+                    d[k] = a.bu_head ? hw[2 * v + 1] : (uint32_t)(a.off[v + 1] - a.off[v]);
             }
 #pragma unroll
-            for (int k = 0; k < kBuBatch; ++k) {
+            for (int k = 0; k < kClaimRounds; ++k) {
                 if (p[k] < 0) continue;
                 const int64_t v = w0 * 64 + idx[k];
                 a.cand[v] = -1;
