@@ -1511,12 +1511,14 @@ __global__ __launch_bounds__(64) void k_init(step_args a, int64_t source,
         }
         for (int k = 0; k < 4; ++k) g->acc[w][l][k] = 0;
     }
-    ctl_block *c = &a.c[0];
+    // the state is built in LDS and published with one coalesced copy
+    ctl_block *c = &s.st;
+    for (int i = l; i < kCtlWords; i += 64) ((uint64_t *)&s.st)[i] = 0;
     if (l == 0) {
         g->task_overflow = 0;
         g->t_end = 0;
-        memset(&a.c[0], 0, 2 * sizeof(ctl_block));
     }
+    wave_lds_sync();
     int64_t nq = 0, nt = 0;  // wave-uniform list lengths
     unsigned long long cnt = 0, dsum = 0, dmax = 0, lost = 0;
     for (int64_t i0 = 0; i0 < nsrc; i0 += 64) {
@@ -1591,11 +1593,14 @@ __global__ __launch_bounds__(64) void k_init(step_args a, int64_t source,
         c->unvisited_deg = a.nnz - (int64_t)dsum;
         c->reached = (int64_t)cnt;
         c->reached_deg = (int64_t)dsum;
+    }
+    {
         const int64_t q0 = nq < a.qseg_cap ? nq : a.qseg_cap;
         const int64_t t0 = nt < a.tseg_cap ? nt : a.tseg_cap;
-        for (int k = 0; k < kSeg + 2; ++k) {
+        const int64_t tall = nt < t0 + a.task_cap ? nt : t0 + a.task_cap;
+        for (int k = l; k < kSeg + 2; k += 64) {
             c->qpref[k] = k == 0 ? 0 : (k == kSeg + 1 ? nq : q0);
-            c->tpref[k] = k == 0 ? 0 : (k == kSeg + 1 ? (nt < t0 + a.task_cap ? nt : t0 + a.task_cap) : t0);
+            c->tpref[k] = k == 0 ? 0 : (k == kSeg + 1 ? tall : t0);
         }
     }
     wave_lds_sync();
@@ -1604,6 +1609,14 @@ __global__ __launch_bounds__(64) void k_init(step_args a, int64_t source,
     mlp_wave_regs w;
     if (a.policy == BD_POLICY_LEARNED && a.has_mlp) mlp_load(a.mlp_ops, w);
     choose_dir(a, s, c, -1, 0, 0, w);
+    wave_lds_sync();
+    // publish as the state before step 0; the other copy starts zeroed
+    uint64_t *d0 = (uint64_t *)&a.c[0];
+    uint64_t *d1 = (uint64_t *)&a.c[1];
+    for (int i = l; i < kCtlWords; i += 64) {
+        d0[i] = ((const uint64_t *)&s.st)[i];
+        d1[i] = 0;
+    }
 }
 
 // clear() (hybrid_bfs.cc:433-443) for the device layout: visited = the
