@@ -179,7 +179,9 @@ extern "C" int bd_create(bd_ctx **out, int device) {
     bd_ctx *c = new bd_ctx();
     c->device = device;
     // launch shape and thresholds: beedrill_amd/autotune.py's best on MI355X
-    // (RMAT scale 24, generator seed 1, models/tuning_rmat_s24.json)
+    // (RMAT scale 24, generator seed 1, models/tuning_rmat_s24.json), except
+    // bu_cap (its 64 gained 0.8% there and cost an Erdos-Renyi graph 6%) and
+    // bu_batch 8 (six rounds per trip, added after that run)
     c->tune.block = 128;
     c->tune.grid_per_cu = 10;
     c->tune.bu_words = 16;
