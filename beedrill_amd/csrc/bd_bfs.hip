@@ -1529,6 +1529,8 @@ __global__ __launch_bounds__(64) void k_init(step_args a, int64_t source,
         bool won = false;
         int64_t b = 0, e = 0;
         if (valid) {
+            // the list bounds are loaded beside the claim, not after it
+            const int64_t lb = a.off[v], le = a.off[v + 1];
             // claims go through bm[0] (zeroed by k_clear): vis already holds
             // the degree-0 vertices, which can still be sources
             const uint64_t bit = 1ULL << (v & 63);
@@ -1538,8 +1540,8 @@ __global__ __launch_bounds__(64) void k_init(step_args a, int64_t source,
             if (won) {
                 atomicOr((unsigned long long *)&a.vis[v >> 6], (unsigned long long)bit);
                 a.parent[v] = (int32_t)v;
-                b = a.off[v];
-                e = a.off[v + 1];
+                b = lb;
+                e = le;
                 const unsigned long long d = (unsigned long long)(e - b);
                 ++cnt;
                 dsum += d;
@@ -1605,9 +1607,9 @@ __global__ __launch_bounds__(64) void k_init(step_args a, int64_t source,
     }
     wave_lds_sync();
     // level 0: with no previous step Beamer applies only the outer test to
-    // scout = deg(sources) (hybrid_bfs.cc:163-167)
-    mlp_wave_regs w;
-    if (a.policy == BD_POLICY_LEARNED && a.has_mlp) mlp_load(a.mlp_ops, w);
+    // scout = deg(sources) (hybrid_bfs.cc:163-167); the network is consulted
+    // only after a step (last_dir >= 0), so its operands are not loaded
+    mlp_wave_regs w{};
     choose_dir(a, s, c, -1, 0, 0, w);
     wave_lds_sync();
     // publish as the state before step 0; the other copy starts zeroed
