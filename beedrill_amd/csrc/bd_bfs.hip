@@ -1149,7 +1149,7 @@ __device__ __forceinline__ void beamer_next(const step_args &a, ctl_block *c, in
 // its network on the wave.
 template <int BLOCK>
 __device__ __forceinline__ void choose_dir(const step_args &a, step_smem<BLOCK> &s, ctl_block *c, int32_t last_dir,
-                           int64_t nnew, int64_t nnew_deg, const mlp_wave_regs &w) {
+                           int64_t nnew, int64_t nnew_deg) {
     if (lane_id() == 0) {
         switch (a.policy) {
         case BD_POLICY_TD_ONLY:
@@ -1194,6 +1194,10 @@ __device__ __forceinline__ void choose_dir(const step_args &a, step_smem<BLOCK> 
          (a.mlp_near > 1 && last_dir == BD_DIR_TD && c->dir == BD_DIR_TD &&
           gt_quotient(c->scout, c->edges_to_check + c->scout,
                       c->alpha > (1LL << 46) ? (1LL << 62) : c->alpha * a.mlp_near)))) {
+        // the operands are loaded only here: every workgroup loading the
+        // 6 KB table at every step cost more than a consultation
+        mlp_wave_regs w;
+        mlp_load(a.mlp_ops, w);
         const int l = lane_id();
         if (l < kMlpIn)
             s.x[l] = mlp_feature_k(l, c->frontier, c->frontier_deg, c->max_frontier_deg,
@@ -1271,8 +1275,6 @@ __device__ __forceinline__ bool rebuild_state(const step_args &a, step_smem<BLOC
     uint64_t cw[3];
 #pragma unroll
     for (int i = 0; i < 3; ++i) cw[i] = l + 64 * i < kCtlWords ? src[l + 64 * i] : 0;
-    mlp_wave_regs w;
-    if (a.policy == BD_POLICY_LEARNED && a.has_mlp) mlp_load(a.mlp_ops, w);
     unsigned long long ac[4];
 #pragma unroll
     for (int k = 0; k < 4; ++k) ac[k] = g->acc[pa][l][k];
@@ -1364,7 +1366,7 @@ __device__ __forceinline__ bool rebuild_state(const step_args &a, step_smem<BLOC
         const int32_t last = (a.policy == BD_POLICY_REMOTE_WRITES_HYBRID && s.s_split)
                                  ? BD_DIR_BU
                                  : s.s_dir;
-        if (s.s_new > 0) choose_dir(a, s, c, last, s.s_new, s.s_new_deg, w);
+        if (s.s_new > 0) choose_dir(a, s, c, last, s.s_new, s.s_new_deg);
     }
     return false;
 }
@@ -1609,8 +1611,7 @@ __global__ __launch_bounds__(64) void k_init(step_args a, int64_t source,
     // level 0: with no previous step Beamer applies only the outer test to
     // scout = deg(sources) (hybrid_bfs.cc:163-167); the network is consulted
     // only after a step (last_dir >= 0), so its operands are not loaded
-    mlp_wave_regs w{};
-    choose_dir(a, s, c, -1, 0, 0, w);
+    choose_dir(a, s, c, -1, 0, 0);
     wave_lds_sync();
     // publish as the state before step 0; the other copy starts zeroed
     uint64_t *d0 = (uint64_t *)&a.c[0];
