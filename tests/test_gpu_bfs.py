@@ -122,6 +122,94 @@ def test_isolated_source_and_self_loops(ctx, orc):
     assert st["steps"] == 1 and st["traversed_edges"] == 0
 
 
+def test_graph_without_edges(ctx, orc):
+    # The reference's loaders refuse such a graph (edge_list.cc:104), and its
+    # root picking would never end on one; an edge list with no edges still
+    # makes a graph of isolated vertices here, searched like any other.
+    nv = 70  # one visited word and a ragged second
+    empty = np.zeros(0, np.int64)
+    ctx.from_edges(nv, empty, empty)
+    off, adj, m = oracle_csr(orc, nv, empty, empty)
+    goff, gcols = ctx.csr()
+    np.testing.assert_array_equal(goff, off)
+    assert gcols.size == 0 and m == 0
+    for root in (0, 63, 69):
+        st = run_and_compare(ctx, orc, off, adj, m, root)
+        assert st["steps"] == 1 and st["traversed_edges"] == 0
+        assert ctx.check(root) and not ctx.check((root + 1) % nv)
+        run_and_compare(ctx, orc, off, adj, m, root, policy="td_only")
+    with pytest.raises(bd.BeedrillError) as e:
+        ctx.pick_roots(1)
+    assert e.value.code == -4
+
+
+# Degrees on either side of every cut the kernels make by list length: task
+# slices from 16 entries, one lane per list below 64, hub lists from 256, the
+# batch's pull slices from 512 (1024 a slice), task slices of 1536, hub
+# slices of 4096 from 8192.
+THRESHOLD_DEGREES = (15, 16, 17, 63, 64, 65, 255, 256, 257, 511, 512, 513, 1023, 1024, 1025,
+                     1535, 1536, 1537, 4095, 4096, 4097, 8191, 8192, 8193, 12289)
+
+
+def threshold_graph(seed=11, leaves=16384, extra=20000, isolated=37):
+    """Hubs 0..24 of exactly THRESHOLD_DEGREES neighbours, drawn without
+    repeats from a shared pool of leaves (so the hubs meet only through
+    leaves), random leaf-leaf edges, isolated vertices at the end; the edges
+    in shuffled order and direction."""
+    rng = np.random.default_rng(seed)
+    nh = len(THRESHOLD_DEGREES)
+    s, d = [], []
+    for h, deg in enumerate(THRESHOLD_DEGREES):
+        s.append(np.full(deg, h, np.int64))
+        d.append(nh + rng.choice(leaves, deg, replace=False).astype(np.int64))
+    s.append(nh + rng.integers(0, leaves, extra))
+    d.append(nh + rng.integers(0, leaves, extra))
+    s, d = np.concatenate(s), np.concatenate(d)
+    flip = rng.random(s.size) < 0.5
+    s, d = np.where(flip, d, s), np.where(flip, s, d)
+    order = rng.permutation(s.size)
+    return nh + leaves + isolated, s[order], d[order]
+
+
+@pytest.mark.parametrize("vertex_order", [1, 2, 3, 4])
+def test_lists_at_every_kernel_threshold(ctx, orc, vertex_order):
+    """Searches over lists at and beside each length threshold, on every
+    search layout and through several launch shapes: depths, parents and
+    Beamer's step records are the oracle's (a list cut one entry short or
+    long loses or repeats neighbours)."""
+    nv, s, d = threshold_graph()
+    t0 = ctx.get_tuning()
+    try:
+        ctx.set_tuning(vertex_order=vertex_order)
+        ctx.from_edges(nv, s, d)
+        off, adj, m = oracle_csr(orc, nv, s, d)
+        deg = np.diff(off)
+        nh = len(THRESHOLD_DEGREES)
+        for h, want in enumerate(THRESHOLD_DEGREES):
+            assert deg[h] == want, (h, deg[h], want)
+        hub_big, hub_small, leaf = nh - 1, 0, int(np.argmax(deg[nh:] > 3)) + nh
+        for root in (hub_big, hub_small, leaf):
+            run_and_compare(ctx, orc, off, adj, m, root)
+            for policy in ("td_only", "bu_only"):
+                run_and_compare(ctx, orc, off, adj, m, root, policy=policy)
+            for alpha, beta in ((2, 2), (1000, 1)):
+                run_and_compare(ctx, orc, off, adj, m, root, alpha=alpha, beta=beta)
+        shapes = ({"bu_cap": 8, "bu_batch": 1}, {"bu_batch": 4, "task_edges": 128},
+                  {"td_split_edges": 1}, {"bu_head": 2}, {"wave_threshold": 16})
+        for shape in shapes:
+            ctx.set_tuning(**shape)
+            for root in (hub_big, leaf):
+                run_and_compare(ctx, orc, off, adj, m, root)
+            ctx.set_tuning(**{k: t0[k] for k in shape})
+        roots = ctx.pick_roots(64)
+        dp, _, _ = ctx.multi_root(roots)
+        for r in range(64):
+            np.testing.assert_array_equal(dp[r], orc.bfs_depth(off, adj, int(roots[r])),
+                                          err_msg=str(r))
+    finally:
+        ctx.set_tuning(**t0)
+
+
 # ------------------------------------------------------------ RMAT scale 16
 
 @pytest.fixture(scope="module")
