@@ -452,6 +452,10 @@ def components_measure(ctx, args, m):
                      "avg_launch_ms": round(pass_ms / launches, 4),
                      "labels_read_per_launch": round(sum(r["labels_read"] for r in runs) / launches)},
         "copy_gbs": round(copy_gbs, 1),
+        # built once per graph by the warm-up run, outside `value`: slices of
+        # the long lists and the short lists regrouped by column block
+        "regroup_once": {"ms": round(runs[0]["prep_ms"], 3),
+                         "bytes": int(runs[0]["prep_bytes"])},
         "cpu_baseline": cpu,
     }
 
