@@ -234,6 +234,23 @@ def test_aggregate_matches_scipy(orc):
     assert g["mean"] == pytest.approx(x.mean()) and g["min"] == x.min() and g["max"] == x.max()
 
 
+def test_bench_aggregate_is_aggregate_stats(orc):
+    """The aggregate the benchmark line reports (aggregate_stats::compute,
+    combined.cc:108-146): closed forms on three values, then the oracle's
+    restatement on 64 values spread like per-root GTEPS."""
+    import bench
+    g = bench.aggregate([1.0, 2.0, 4.0])
+    assert g["n"] == 3 and g["min"] == 1.0 and g["max"] == 4.0
+    assert g["hmean"] == pytest.approx(12 / 7, rel=1e-15)
+    assert g["mean"] == pytest.approx(7 / 3, rel=1e-15)
+    assert g["stddev"] == pytest.approx(np.sqrt(7 / 3), rel=1e-15)  # sum of squares 14/3, N-1 = 2
+    assert bench.hmean([2.0, 2.0]) == 2.0
+    x = np.random.default_rng(3).uniform(0.3e9, 3.0e9, 64)
+    b, o = bench.aggregate(x), orc.aggregate(x)
+    for k in ("hmean", "mean", "stddev", "min", "max"):
+        assert b[k] == pytest.approx(o[k], rel=1e-12), k
+
+
 # ------------------------------------------------------ bottom-up head index
 
 def test_head_index_matches_lexsort(orc):
